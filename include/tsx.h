@@ -1,0 +1,136 @@
+/*
+ * libtsx: tomographic forward projection and backprojection on AMD MI355X
+ * (gfx950), behind a plain C ABI.
+ *
+ * This library takes the place of the ASTRA projector that tomosipo drives.
+ * tomosipo reaches ASTRA through exactly two calls, and each entry point
+ * below names the call it replaces:
+ *
+ *   astra.create_projector("cuda3d", proj_geom, vol_geom, options)
+ *       tomosipo/astra.py:80-98   -> tsx_plan_create / tsx_plan_destroy
+ *   astra.experimental.direct_FPBP3D(projector, vol, proj, mode, "FP"|"BP")
+ *       tomosipo/astra.py:104-153 -> tsx_direct_fpbp3d, tsx_forward, tsx_backward
+ *
+ * Geometry crosses the boundary in the form tomosipo serialises it for ASTRA:
+ * the volume dictionary (geometry/volume.py:286-301: GridColCount = x,
+ * GridRowCount = y, GridSliceCount = z, WindowMin/Max per axis) and the
+ * vector projection dictionary (geometry/cone_vec.py:174-191,
+ * geometry/parallel_vec.py:175-192: DetectorRowCount, DetectorColCount and
+ * n x 12 Vectors [ray|src, det, u, v], each 3-vector in (x, y, z) order).
+ * Parametric "parallel3d" / "cone" geometries are converted to vector form by
+ * the caller, as tomosipo's to_vec() does.
+ *
+ * Data layout (tomosipo links/base.py:21-31, links/torch.py:94-99): volume
+ * float32 [z][y][x], projections float32 [row][angle][col], both C-contiguous
+ * in device memory on the plan's device. The library never allocates or frees
+ * caller buffers. Work is enqueued on the caller's HIP stream and is
+ * asynchronous; the plan is immutable after creation and may be shared by
+ * threads.
+ *
+ * Every function returns TSX_OK (0) or a negative tsx_status; the message of
+ * the last failure on the calling thread is returned by tsx_last_error().
+ */
+#ifndef TSX_H
+#define TSX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    TSX_OK = 0,
+    TSX_ERR_INVALID_ARGUMENT = -1, /* tomosipo raises ValueError/TypeError for these */
+    TSX_ERR_UNSUPPORTED = -2,      /* a geometry or option this library does not implement */
+    TSX_ERR_HIP = -3,              /* a HIP runtime call failed */
+    TSX_ERR_NO_DEVICE = -4,        /* compute requested on a host-only plan */
+    TSX_ERR_OUT_OF_MEMORY = -5
+} tsx_status;
+
+/* Projection geometry kinds (the ASTRA "type" strings). */
+enum { TSX_PARALLEL3D_VEC = 0, TSX_CONE_VEC = 1 };
+
+/* Mode constants of direct_FPBP3D (tomosipo/astra.py:132-135). */
+enum { TSX_MODE_ADD = 0, TSX_MODE_SET = 1 };
+
+/* Operation selector of direct_FPBP3D: "FP" / "BP". */
+enum { TSX_OP_FP = 0, TSX_OP_BP = 1 };
+
+typedef struct {
+    /* projection dictionary */
+    int32_t type;              /* TSX_PARALLEL3D_VEC or TSX_CONE_VEC */
+    int32_t det_row_count;     /* DetectorRowCount (v, rows) */
+    int32_t det_col_count;     /* DetectorColCount (u, columns) */
+    int32_t n_angles;          /* rows of Vectors */
+    const double *vectors;     /* n_angles x 12, row-major, ASTRA order and axes */
+    /* volume dictionary */
+    int32_t grid_col_count;    /* x */
+    int32_t grid_row_count;    /* y */
+    int32_t grid_slice_count;  /* z */
+    double window_min[3];      /* (x, y, z) */
+    double window_max[3];      /* (x, y, z) */
+    /* projector options (astra.py:94-97) */
+    int32_t det_supersampling; /* DetectorSuperSampling, >= 1 */
+    int32_t vox_supersampling; /* VoxelSuperSampling; only 1 is supported (BP is the exact transpose) */
+} tsx_geometry;
+
+typedef struct tsx_plan tsx_plan;
+
+/* Replaces astra.create_projector (tomosipo/astra.py:80-98). Validates the
+ * geometry, computes the per-angle ray parameters in FP64 and uploads them to
+ * `device`. device = -1 builds a host-only plan: validation and
+ * tsx_plan_info work, compute calls return TSX_ERR_NO_DEVICE. */
+int tsx_plan_create(const tsx_geometry *geometry, int device, tsx_plan **out);
+
+/* Frees the plan's device tables. tomosipo never frees its projector id
+ * (Operator.py:63-174); here the owner calls this. NULL is accepted. */
+int tsx_plan_destroy(tsx_plan *plan);
+
+typedef struct {
+    int32_t device;
+    int32_t n_angles, det_rows, det_cols;
+    int64_t vol_shape[3];      /* (z, y, x) */
+    int64_t plan_bytes;        /* device bytes held by the plan */
+    int32_t axis_count[3];     /* number of angles stepping along x, y, z */
+} tsx_plan_info;
+
+int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);
+
+/* Dominant (stepping) axis of every angle: 0 = x, 1 = y, 2 = z. */
+int tsx_plan_axes(const tsx_plan *plan, int32_t *axes_out);
+
+/* Replaces astra.experimental.direct_FPBP3D(projector, vol, proj, mode, op)
+ * (tomosipo/astra.py:147-153). mode: TSX_MODE_SET overwrites the output,
+ * TSX_MODE_ADD accumulates into it. op: TSX_OP_FP writes `proj` from `vol`,
+ * TSX_OP_BP writes `vol` from `proj`. */
+int tsx_direct_fpbp3d(const tsx_plan *plan, float *vol, float *proj, int mode, int op,
+                      void *hip_stream);
+
+/* Forward projection: proj (=|+=) A vol. Same kernel as direct_fpbp3d(FP). */
+int tsx_forward(const tsx_plan *plan, const float *vol, float *proj, int additive,
+                void *hip_stream);
+
+/* Backprojection, the exact transpose of tsx_forward: vol (=|+=) A^T proj. */
+int tsx_backward(const tsx_plan *plan, float *vol, const float *proj, int additive,
+                 void *hip_stream);
+
+/* Batched variants: `batch` independent volumes / projection sets with the
+ * given element strides, in one launch. Replaces the per-slice loop of
+ * tomosipo's autograd wrapper (torch_support.py:49-53, 70-74). */
+int tsx_forward_batched(const tsx_plan *plan, const float *vol, int64_t vol_stride, float *proj,
+                        int64_t proj_stride, int64_t batch, int additive, void *hip_stream);
+int tsx_backward_batched(const tsx_plan *plan, float *vol, int64_t vol_stride, const float *proj,
+                         int64_t proj_stride, int64_t batch, int additive, void *hip_stream);
+
+/* Message describing the last failure on this thread ("" if none). */
+const char *tsx_last_error(void);
+
+/* Library version string. */
+const char *tsx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSX_H */

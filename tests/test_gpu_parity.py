@@ -1,0 +1,170 @@
+"""Parity of the HIP projector with the FP64 oracle (GPU).
+
+Tolerances (floating point, FP32 kernels vs FP64 oracle):
+  forward and backprojection: relative L2 error <= 1e-5;
+  matched adjoint: |<Ax, y> - <x, A^T y>| / (|Ax| |y|) <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import tomosipo_amd as ts
+from tests.conftest import rel_l2
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+DEV = "cuda"
+
+
+def _cone_vec_helix(n_angles, rows, cols, size_v, size_u, shift, jitter, seed):
+    base = ts.cone(angles=n_angles, shape=(rows, cols), size=(size_v, size_u), cone_angle=0.5).to_vec()
+    src = base.src_pos
+    src[:, 0] += np.linspace(-shift, shift, n_angles)
+    src += np.random.default_rng(seed).normal(0.0, jitter, size=src.shape)
+    return ts.cone_vec(shape=base.det_shape, src_pos=src, det_pos=base.det_pos, det_v=base.det_v,
+                       det_u=base.det_u)
+
+
+def _check_fp_bp(vg, pg, x, ss=1, tol=TOL):
+    A = ts.operator(vg, pg, detector_supersampling=ss)
+    g = oracle.oracle_geometry(vg, pg, det_supersampling=ss)
+    print(f"[parity] vol={vg.shape} det={pg.det_shape} angles={pg.num_angles} ss={ss}", flush=True)
+    xd = torch.from_numpy(x).to(DEV)
+    y = A(xd)
+    y_ref = oracle.fp(g, x)
+    e_fp = rel_l2(y.cpu().numpy(), y_ref)
+    yin = y_ref.astype(np.float32)
+    b = A.T(torch.from_numpy(yin).to(DEV))
+    b_ref = oracle.bp(g, yin)
+    e_bp = rel_l2(b.cpu().numpy(), b_ref)
+    # matched adjoint on the GPU results, with an independent y
+    yr = np.random.default_rng(7).random(A.range_shape).astype(np.float32)
+    bt = A.T(torch.from_numpy(yr).to(DEV)).double().cpu().numpy()
+    yt = y.double().cpu().numpy()
+    adj = abs(np.sum(yt * yr) - np.sum(x.astype(np.float64) * bt)) / (np.linalg.norm(yt) * np.linalg.norm(yr))
+    print(f"[parity]   fp={e_fp:.3e} bp={e_bp:.3e} adjoint={adj:.3e}", flush=True)
+    assert e_fp <= tol
+    assert e_bp <= tol
+    assert adj <= tol
+
+
+def test_c1_parallel3d_vec_64():
+    vg = ts.volume(shape=64, size=1)
+    pg = ts.parallel(angles=96, shape=(64, 96), size=(1, 1.5)).to_vec()
+    x = ts.phantom.box_noise((64, 64, 64), 16, 40, 0.01, seed=1)
+    _check_fp_bp(vg, pg, x)
+
+
+def test_cone_small():
+    vg = ts.volume(shape=48, size=1)
+    pg = ts.cone(angles=40, shape=(48, 72), size=(np.sqrt(2), 1.5 * np.sqrt(2)), cone_angle=0.5)
+    x = ts.phantom.box_noise((48, 48, 48), 12, 30, 0.01, seed=2)
+    _check_fp_bp(vg, pg, x)
+
+
+def test_cone_vec_helix_jitter():
+    vg = ts.volume(shape=40, size=1)
+    pg = _cone_vec_helix(36, 40, 60, np.sqrt(2), 1.5 * np.sqrt(2), 0.25, 1e-3, seed=3)
+    x = ts.phantom.ellipsoids(40, count=8, seed=3)
+    _check_fp_bp(vg, pg, x)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_random_geometries(seed):
+    rng = np.random.default_rng(100 + seed)
+    shape = tuple(int(s) for s in rng.integers(8, 24, size=3))
+    vg = ts.volume(shape=shape, pos=rng.normal(0, 0.2, 3), size=rng.uniform(0.5, 1.5, 3))
+    T = ts.rotate(pos=0, axis=rng.normal(size=3), angles=rng.normal())
+    rows, cols = (int(s) for s in rng.integers(8, 30, size=2))
+    if seed % 2 == 0:
+        pg = T * ts.cone(angles=rng.uniform(0, 2 * np.pi, 17), shape=(rows, cols),
+                         size=(rng.uniform(1, 3), rng.uniform(1, 3)), src_orig_dist=rng.uniform(2, 5),
+                         src_det_dist=rng.uniform(5, 9)).to_vec()
+    else:
+        pg = T * ts.parallel(angles=rng.uniform(0, np.pi, 17), shape=(rows, cols),
+                             size=(rng.uniform(1, 3), rng.uniform(1, 3))).to_vec()
+    x = rng.random(vg.shape).astype(np.float32)
+    _check_fp_bp(vg, pg, x)
+
+
+@pytest.mark.parametrize("ss", [2, 3])
+def test_detector_supersampling(ss):
+    vg = ts.volume(shape=24, size=1)
+    pg = ts.cone(angles=12, shape=(16, 20), size=(1.4, 2.0), cone_angle=0.5)
+    x = ts.phantom.ellipsoids(24, count=5, seed=ss)
+    _check_fp_bp(vg, pg, x, ss=ss)
+
+
+def test_additive_and_out():
+    vg = ts.volume(shape=32, size=1)
+    pg = ts.parallel(angles=20, shape=(32, 48), size=(1, 1.5))
+    A = ts.operator(vg, pg)
+    B = ts.operator(vg, pg, additive=True)
+    x = torch.from_numpy(ts.phantom.box_noise((32, 32, 32), 8, 20, 0.01, seed=4)).to(DEV)
+    y = torch.zeros(A.range_shape, device=DEV)
+    B(x, out=y)
+    B(x, out=y)
+    torch.testing.assert_close(y, 2 * A(x), rtol=1e-6, atol=1e-6)
+    b = torch.zeros(A.domain_shape, device=DEV)
+    B.T(y, out=b)
+    B.T(y, out=b)
+    torch.testing.assert_close(b, 2 * A.T(y), rtol=1e-6, atol=1e-6)
+
+
+def test_numpy_and_data_paths():
+    vg = ts.volume(shape=32, size=1)
+    pg = ts.cone(angles=20, shape=(32, 48), size=(1.4, 2.1), cone_angle=0.5)
+    A = ts.operator(vg, pg)
+    x = ts.phantom.box_noise((32, 32, 32), 8, 20, 0.01, seed=5)
+    y_np = A(x)
+    assert isinstance(y_np, np.ndarray) and y_np.dtype == np.float32
+    y_t = A(torch.from_numpy(x).to(DEV)).cpu().numpy()
+    np.testing.assert_array_equal(y_np, y_t)
+    vd = ts.data(vg, x.copy())
+    yd = A(vd)
+    assert isinstance(yd, ts.Data.Data)
+    np.testing.assert_array_equal(yd.data, y_np)
+    np.testing.assert_array_equal(A.T(yd).data, A.T(y_np))
+    # CPU torch tensors stay on the CPU
+    y_cpu = A(torch.from_numpy(x))
+    assert y_cpu.device.type == "cpu"
+    np.testing.assert_array_equal(y_cpu.numpy(), y_np)
+
+
+def test_batched_equals_single():
+    from tomosipo_amd.torch_support import to_autograd
+
+    vg = ts.volume(shape=24, size=1)
+    pg = ts.parallel(angles=30, shape=(24, 36), size=(1, 1.5))
+    A = ts.operator(vg, pg)
+    f = to_autograd(A, num_extra_dims=2)
+    xs = torch.from_numpy(np.stack([ts.phantom.ellipsoids(24, 6, seed=s) for s in range(6)])).to(DEV)
+    xs = xs.reshape(2, 3, 24, 24, 24)
+    ys = f(xs)
+    for a in range(2):
+        for c in range(3):
+            assert torch.equal(ys[a, c], A(xs[a, c].contiguous()))
+    g = to_autograd(A.T, num_extra_dims=2)
+    bs = g(ys)
+    for a in range(2):
+        for c in range(3):
+            assert torch.equal(bs[a, c], A.T(ys[a, c].contiguous()))
+
+
+def test_autograd_grad_equals_ATAx():
+    from tomosipo_amd.torch_support import to_autograd, AutogradOperator
+
+    vg = ts.volume(shape=(1, 64, 64))
+    pg = ts.parallel(angles=96, shape=(1, 96))
+    A = ts.operator(vg, pg)
+    x = torch.ones(A.domain_shape, dtype=torch.float32, device=DEV, requires_grad=True)
+    y = to_autograd(A)(x)
+    y.backward(y)
+    torch.testing.assert_close(x.grad, A.T(A(x.detach())))
+    A_ag = AutogradOperator(A)
+    p = torch.ones(A.range_shape, dtype=torch.float32, device=DEV, requires_grad=True)
+    q = A_ag.T(p)
+    q.backward(q)
+    torch.testing.assert_close(p.grad, A(A.T(p.detach())))

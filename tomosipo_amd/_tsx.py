@@ -1,0 +1,200 @@
+"""ctypes binding of ``libtsx.so`` (declared in ``include/tsx.h``).
+
+The projector boundary of tomosipo is ``astra.create_projector`` plus
+``astra.experimental.direct_FPBP3D`` (tomosipo ``astra.py:80-153``); this
+module is the Python side of its replacement. Device pointers and the HIP
+stream come from torch tensors (``tensor.data_ptr()``,
+``torch.cuda.current_stream().cuda_stream``) so no data is copied.
+
+There is no CPU fallback: if the library is missing, or a projection is
+requested without a HIP device, the call raises.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "csrc", "libtsx.so")
+
+TSX_PARALLEL3D_VEC = 0
+TSX_CONE_VEC = 1
+TSX_MODE_ADD = 0
+TSX_MODE_SET = 1
+TSX_OP_FP = 0
+TSX_OP_BP = 1
+
+_STATUS = {
+    -1: "TSX_ERR_INVALID_ARGUMENT",
+    -2: "TSX_ERR_UNSUPPORTED",
+    -3: "TSX_ERR_HIP",
+    -4: "TSX_ERR_NO_DEVICE",
+    -5: "TSX_ERR_OUT_OF_MEMORY",
+}
+
+EXPORTED = (
+    "tsx_plan_create", "tsx_plan_destroy", "tsx_plan_info_get", "tsx_plan_axes",
+    "tsx_direct_fpbp3d", "tsx_forward", "tsx_backward", "tsx_forward_batched",
+    "tsx_backward_batched", "tsx_last_error", "tsx_version",
+)
+
+
+class TsxError(RuntimeError):
+    def __init__(self, code, message):
+        self.code = code
+        super().__init__(f"{_STATUS.get(code, code)}: {message}")
+
+
+class TsxGeometry(ctypes.Structure):
+    _fields_ = [
+        ("type", ctypes.c_int32),
+        ("det_row_count", ctypes.c_int32),
+        ("det_col_count", ctypes.c_int32),
+        ("n_angles", ctypes.c_int32),
+        ("vectors", ctypes.c_void_p),
+        ("grid_col_count", ctypes.c_int32),
+        ("grid_row_count", ctypes.c_int32),
+        ("grid_slice_count", ctypes.c_int32),
+        ("window_min", ctypes.c_double * 3),
+        ("window_max", ctypes.c_double * 3),
+        ("det_supersampling", ctypes.c_int32),
+        ("vox_supersampling", ctypes.c_int32),
+    ]
+
+
+class TsxPlanInfo(ctypes.Structure):
+    _fields_ = [
+        ("device", ctypes.c_int32),
+        ("n_angles", ctypes.c_int32),
+        ("det_rows", ctypes.c_int32),
+        ("det_cols", ctypes.c_int32),
+        ("vol_shape", ctypes.c_int64 * 3),
+        ("plan_bytes", ctypes.c_int64),
+        ("axis_count", ctypes.c_int32 * 3),
+    ]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load():
+    """Load ``libtsx.so``; raises ``ImportError`` if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m tomosipo_amd._build` "
+                "(hipcc, gfx950). tomosipo_amd has no CPU fallback.")
+        lib = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i64, i32 = ctypes.c_int64, ctypes.c_int
+        lib.tsx_plan_create.argtypes = [ctypes.POINTER(TsxGeometry), i32, ctypes.POINTER(P)]
+        lib.tsx_plan_destroy.argtypes = [P]
+        lib.tsx_plan_info_get.argtypes = [P, ctypes.POINTER(TsxPlanInfo)]
+        lib.tsx_plan_axes.argtypes = [P, P]
+        lib.tsx_direct_fpbp3d.argtypes = [P, P, P, i32, i32, P]
+        lib.tsx_forward.argtypes = [P, P, P, i32, P]
+        lib.tsx_backward.argtypes = [P, P, P, i32, P]
+        lib.tsx_forward_batched.argtypes = [P, P, i64, P, i64, i64, i32, P]
+        lib.tsx_backward_batched.argtypes = [P, P, i64, P, i64, i64, i32, P]
+        for name in EXPORTED:
+            if name not in ("tsx_last_error", "tsx_version"):
+                getattr(lib, name).restype = ctypes.c_int
+        lib.tsx_last_error.restype = ctypes.c_char_p
+        lib.tsx_version.restype = ctypes.c_char_p
+        _lib = lib
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise TsxError(rc, load().tsx_last_error().decode())
+
+
+def make_geometry(vol_geom, proj_geom, det_supersampling=1, vox_supersampling=1):
+    """``TsxGeometry`` from the ASTRA dictionaries; returns (struct, keep-alive array)."""
+    kinds = {"parallel3d_vec": TSX_PARALLEL3D_VEC, "cone_vec": TSX_CONE_VEC}
+    if proj_geom["type"] not in kinds:
+        raise ValueError(f"projector needs a vector geometry; got '{proj_geom['type']}'")
+    vectors = np.ascontiguousarray(proj_geom["Vectors"], dtype=np.float64)
+    if vectors.ndim != 2 or vectors.shape[1] != 12:
+        raise ValueError(f"Vectors must have shape (n, 12); got {vectors.shape}")
+    o = vol_geom["option"]
+    g = TsxGeometry()
+    g.type = kinds[proj_geom["type"]]
+    g.det_row_count = int(proj_geom["DetectorRowCount"])
+    g.det_col_count = int(proj_geom["DetectorColCount"])
+    g.n_angles = vectors.shape[0]
+    g.vectors = vectors.ctypes.data
+    g.grid_col_count = int(vol_geom["GridColCount"])
+    g.grid_row_count = int(vol_geom["GridRowCount"])
+    g.grid_slice_count = int(vol_geom["GridSliceCount"])
+    g.window_min = (ctypes.c_double * 3)(o["WindowMinX"], o["WindowMinY"], o["WindowMinZ"])
+    g.window_max = (ctypes.c_double * 3)(o["WindowMaxX"], o["WindowMaxY"], o["WindowMaxZ"])
+    g.det_supersampling = int(det_supersampling)
+    g.vox_supersampling = int(vox_supersampling)
+    return g, vectors
+
+
+class Plan:
+    """Owner of one ``tsx_plan`` (immutable geometry tables on one device)."""
+
+    def __init__(self, vol_geom, proj_geom, device=-1, det_supersampling=1, vox_supersampling=1):
+        lib = load()
+        g, self._vectors = make_geometry(vol_geom, proj_geom, det_supersampling, vox_supersampling)
+        h = ctypes.c_void_p()
+        rc = lib.tsx_plan_create(ctypes.byref(g), int(device), ctypes.byref(h))
+        if rc != 0:
+            msg = lib.tsx_last_error().decode()
+            if rc in (-1, -2):
+                raise ValueError(msg)
+            raise TsxError(rc, msg)
+        self._h = h
+        self.device = int(device)
+        self._lib = lib
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._lib.tsx_plan_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self):
+        inf = TsxPlanInfo()
+        _check(self._lib.tsx_plan_info_get(self._h, ctypes.byref(inf)))
+        return {
+            "device": inf.device, "n_angles": inf.n_angles, "det_rows": inf.det_rows,
+            "det_cols": inf.det_cols, "vol_shape": tuple(inf.vol_shape),
+            "plan_bytes": inf.plan_bytes, "axis_count": tuple(inf.axis_count),
+        }
+
+    def axes(self):
+        n = self.info()["n_angles"]
+        out = np.zeros(n, dtype=np.int32)
+        _check(self._lib.tsx_plan_axes(self._h, out.ctypes.data))
+        return out
+
+    def forward(self, vol_ptr, proj_ptr, additive, stream, batch=1, vol_stride=0, proj_stride=0):
+        _check(self._lib.tsx_forward_batched(self._h, vol_ptr, vol_stride, proj_ptr, proj_stride,
+                                             batch, int(bool(additive)), stream))
+
+    def backward(self, vol_ptr, proj_ptr, additive, stream, batch=1, vol_stride=0, proj_stride=0):
+        _check(self._lib.tsx_backward_batched(self._h, vol_ptr, vol_stride, proj_ptr, proj_stride,
+                                              batch, int(bool(additive)), stream))
+
+
+def version():
+    return load().tsx_version().decode()

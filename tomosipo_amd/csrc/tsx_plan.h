@@ -1,0 +1,29 @@
+// Per-angle projector parameters shared by the host plan builder and the HIP
+// kernels. All vectors are stored in the angle's own axis order
+// (m, c1, c2): m is the stepping (dominant) axis, c1 and c2 the two in-slice
+// axes, c2 being the one with the smaller memory stride.
+#pragma once
+
+#include <stdint.h>
+
+namespace tsx {
+
+struct AngleParams {
+    // Ray family, grid units (voxel centres on integers). Pixel (i, j) maps to
+    // the point P0 + j*U + i*V of a virtual detector plane through the volume
+    // centre that is parallel to the real detector; rays are unchanged.
+    double P0[3], U[3], V[3];
+    double S[3];          // cone: source position; parallel: ray direction
+    double N[3];          // U x V
+    double Bj[3], Bi[3];  // dual basis: (X - P0) . Bj = j, (X - P0) . Bi = i for X on the plane
+    double kappa;         // cone: (P0 - S) . N
+    double EN;            // parallel: S . N
+    double h[3];          // voxel size along (m, c1, c2)
+    int32_t m, c1, c2;    // axis ids, 0 = x, 1 = y, 2 = z
+    int32_t cone;
+    int32_t n[3];         // volume extent along (m, c1, c2)
+    int32_t pad;
+    int64_t st[3];        // element strides along (m, c1, c2)
+};
+
+}  // namespace tsx

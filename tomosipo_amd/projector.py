@@ -1,0 +1,136 @@
+"""Projector creation and direct projection on the HIP library.
+
+Replaces tomosipo's ASTRA bridge (``tomosipo/astra.py:80-215``):
+
+* ``create_projector(vg, pg, ...)`` takes the place of
+  ``create_astra_projector`` (``astra.py:80-98``): the geometry is serialised
+  to the ASTRA dictionaries (``vg.to_astra()``, ``pg.to_vec().to_astra()``) and
+  validated by ``tsx_plan_create`` on the host at once; device tables are
+  built lazily, once per device.
+* ``direct_fp`` / ``direct_bp`` / ``direct_project`` follow ``astra.py:104-215``:
+  links must be compatible (same device) or ``ValueError``; SET overwrites,
+  ADD accumulates.
+
+Torch tensors on a HIP device go to the kernels by pointer on the current
+stream of their device (asynchronous, like any torch op). numpy arrays and
+CPU tensors are copied to device 0, projected and copied back, which is what
+ASTRA does for host arrays (``links/numpy.py:70-77``). Without a HIP device
+the call raises ``RuntimeError``; there is no CPU path.
+"""
+import threading
+
+import numpy as np
+
+from . import _tsx
+from .links.base import are_compatible
+
+
+class Projector:
+    def __init__(self, volume_geometry, projection_geometry, *, voxel_supersampling=1,
+                 detector_supersampling=1):
+        from .geometry import VolumeGeometry
+
+        if not isinstance(volume_geometry, VolumeGeometry):
+            raise TypeError(f"Expected an axis-aligned VolumeGeometry. Got {type(volume_geometry)}.")
+        self.vol_geom = volume_geometry.to_astra()
+        self.proj_geom = projection_geometry.to_vec().to_astra()
+        self.voxel_supersampling = int(voxel_supersampling)
+        self.detector_supersampling = int(detector_supersampling)
+        self.vol_shape = tuple(volume_geometry.shape)
+        rows, cols = projection_geometry.det_shape
+        self.proj_shape = (rows, projection_geometry.num_angles, cols)
+        # Host-only plan: validates now, as astra.create_projector would.
+        self._host_plan = self._make_plan(-1)
+        self._plans = {}
+        self._lock = threading.Lock()
+
+    def _make_plan(self, device):
+        return _tsx.Plan(self.vol_geom, self.proj_geom, device=device,
+                         det_supersampling=self.detector_supersampling,
+                         vox_supersampling=self.voxel_supersampling)
+
+    def plan(self, device):
+        """The device plan for ``device`` (created on first use)."""
+        p = self._plans.get(device)
+        if p is None:
+            with self._lock:
+                p = self._plans.get(device)
+                if p is None:
+                    p = self._make_plan(device)
+                    self._plans[device] = p
+        return p
+
+    def axes(self):
+        return self._host_plan.axes()
+
+
+def create_projector(volume_geometry, projection_geometry, *, voxel_supersampling=1,
+                     detector_supersampling=1):
+    return Projector(volume_geometry, projection_geometry, voxel_supersampling=voxel_supersampling,
+                     detector_supersampling=detector_supersampling)
+
+
+def _torch():
+    try:
+        import torch
+    except ImportError:
+        raise RuntimeError("tomosipo_amd projects on a HIP device through torch; torch is not importable")
+    return torch
+
+
+def _require_device(torch):
+    if not torch.cuda.is_available():
+        raise RuntimeError("tomosipo_amd needs a HIP (ROCm) device for projections; none is "
+                           "available and there is no CPU fallback")
+
+
+def _run(projector, vol_t, proj_t, forward, additive, batch=1, vol_stride=0, proj_stride=0):
+    """Launch on device tensors (float32, contiguous, same device)."""
+    torch = _torch()
+    dev = vol_t.device.index if vol_t.device.index is not None else torch.cuda.current_device()
+    plan = projector.plan(dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    if forward:
+        plan.forward(vol_t.data_ptr(), proj_t.data_ptr(), additive, stream, batch, vol_stride,
+                     proj_stride)
+    else:
+        plan.backward(vol_t.data_ptr(), proj_t.data_ptr(), additive, stream, batch, vol_stride,
+                      proj_stride)
+
+
+def direct_project(projector, vol_link, proj_link, forward=None, additive=False):
+    if forward is None:
+        raise ValueError("project must be given a forward argument (True/False).")
+    if not are_compatible(vol_link, proj_link):
+        raise ValueError(
+            "Cannot perform projection on volume and projection data, because they are not "
+            "compatible. Usually, this indicates that the data are located on different computing "
+            "devices. ")
+    torch = _torch()
+    vol, proj = vol_link.data, proj_link.data
+    on_device = isinstance(vol, torch.Tensor) and vol.is_cuda
+    with vol_link.context():
+        if on_device:
+            _run(projector, vol, proj, forward, additive)
+            return
+        _require_device(torch)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        to_dev = (lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)) \
+            if isinstance(vol, np.ndarray) else (lambda a: a.detach().to(dev))
+        vol_d = to_dev(vol)
+        proj_d = to_dev(proj)
+        _run(projector, vol_d, proj_d, forward, additive)
+        dst, src = (proj, proj_d) if forward else (vol, vol_d)
+        if isinstance(dst, np.ndarray):
+            dst[...] = src.cpu().numpy()
+        else:
+            with torch.no_grad():
+                dst.copy_(src.cpu())
+
+
+def direct_fp(projector, vol_link, proj_link, additive=False):
+    return direct_project(projector, vol_link, proj_link, forward=True, additive=additive)
+
+
+def direct_bp(projector, vol_link, proj_link, additive=False):
+    return direct_project(projector, vol_link, proj_link, forward=False, additive=additive)
