@@ -1,0 +1,210 @@
+"""Benchmark: GUPS of forward + back projection on 512^3 cone-beam instances.
+
+Workload (one "step"): 16 independent cone_vec instances, each a 512^3
+float32 unit-cube volume, 720 angles over 2*pi, a 512 x 768 detector of size
+(sqrt 2, 1.5 sqrt 2) through the origin with source distance sqrt(2)/0.5,
+the source shifted linearly in z over +-0.25 (helical-like) plus seeded
+per-angle jitter sigma = 1e-3. One forward projection of every instance, then
+one backprojection of every instance. Inputs are resident in device memory
+before timing starts.
+
+Metric: GUPS = 2 * 512^3 * 720 * 16 / (time of one step) (voxel-angle
+updates per second over FP and BP together).
+
+The CPU baseline is the FP64 oracle (C, OpenMP) on the host cores of the same
+machine, timed on a bounded subset of the angles of instance 0.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N_VOX = 512
+N_ANGLES = 720
+DET = (512, 768)
+SIZE = (np.sqrt(2), 1.5 * np.sqrt(2))
+# Algorithmic per-sample work of the Joseph projector: 4 float32 corners
+# (16 B) and 14 FP32 flops per bilinear sample; the exact-transpose
+# backprojection has the same non-zeros. Every ray samples every slice of its
+# stepping axis: samples per voxel-angle = rows * cols * N / N^3.
+FLOP_PER_SAMPLE = 14.0
+GATHER_B_PER_SAMPLE = 16.0
+FP32_PEAK_TFLOPS = 157.3
+LDS_PEAK_TBS = 78.6
+HBM_PEAK_TBS = 8.0
+
+
+def instance_geometry(i, ts):
+    base = ts.cone(angles=N_ANGLES, shape=DET, size=SIZE, cone_angle=0.5).to_vec()
+    src = base.src_pos
+    src[:, 0] += np.linspace(-0.25, 0.25, N_ANGLES)
+    src += np.random.default_rng(1000 + i).normal(0.0, 1e-3, size=src.shape)
+    pg = ts.cone_vec(shape=base.det_shape, src_pos=src, det_pos=base.det_pos, det_v=base.det_v,
+                     det_u=base.det_u)
+    vg = ts.volume(shape=N_VOX, size=1)
+    return vg, pg
+
+
+def phantom(i, torch, dev):
+    """Box of ones on [128, 384)^3 plus seeded N(0, 0.01) noise, built on the device."""
+    g = torch.Generator(device=dev)
+    g.manual_seed(2000 + i)
+    x = torch.randn((N_VOX,) * 3, generator=g, device=dev, dtype=torch.float32) * 0.01
+    x[128:384, 128:384, 128:384] += 1.0
+    return x
+
+
+def cpu_baseline(ts, target_s=12.0):
+    """FP64 oracle FP + BP on a subset of angles of instance 0; GUPS over both."""
+    import oracle
+
+    vg, pg = instance_geometry(0, ts)
+    x = np.zeros((N_VOX,) * 3, dtype=np.float64)
+    x[128:384, 128:384, 128:384] = 1.0
+    # calibrate on one angle, then pick the subset (every k-th angle)
+    g1 = oracle.oracle_geometry(vg, pg[0:1])
+    t0 = time.perf_counter()
+    y1 = oracle.fp(g1, x)
+    oracle.bp(g1, y1)
+    t1 = time.perf_counter() - t0
+    n_sub = int(max(1, min(N_ANGLES, target_s / max(t1, 1e-3))))
+    step = max(1, N_ANGLES // n_sub)
+    sub = pg[::step]
+    g = oracle.oracle_geometry(vg, sub)
+    t0 = time.perf_counter()
+    y = oracle.fp(g, x)
+    t_fp = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    oracle.bp(g, y)
+    t_bp = time.perf_counter() - t0
+    na = sub.num_angles
+    gups = 2 * N_VOX ** 3 * na / (t_fp + t_bp) / 1e9
+    return {
+        "value": round(gups, 4),
+        "unit": "GUPS",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": (f"instance 0 of the workload, every {step}th angle ({na} of {N_ANGLES}): "
+                   f"one FP ({t_fp:.2f} s) and one BP ({t_bp:.2f} s) in FP64"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); each instance of this "
+              "workload is independent, so N GPUs would run N/16 of the instances each with no "
+              "communication. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import tomosipo_amd as ts
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    ops, xs, ys, bs = [], [], [], []
+    for i in range(args.instances):
+        vg, pg = instance_geometry(i, ts)
+        A = ts.operator(vg, pg)
+        ops.append(A)
+        xs.append(phantom(i, torch, dev))
+        ys.append(torch.empty(A.range_shape, device=dev))
+        bs.append(torch.empty(A.domain_shape, device=dev))
+    torch.cuda.synchronize()
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record()
+        for A, x, y in zip(ops, xs, ys):
+            A(x, out=y)
+        if ev is not None:
+            ev[1].record()
+        for A, y, b in zip(ops, ys, bs):
+            A.T(y, out=b)
+        if ev is not None:
+            ev[2].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    fp_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs])) / args.instances
+    bp_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs])) / args.instances
+    ms_per_step = wall * 1e3 / args.steps
+    va = float(N_VOX) ** 3 * N_ANGLES * args.instances
+    gups = 2 * va / (ms_per_step * 1e-3) / 1e9
+
+    # roofline of the dominant kernel (per-instance launch)
+    samples = float(DET[0]) * DET[1] * N_VOX * N_ANGLES
+    dom, dom_ms = ("backprojection", bp_ms) if bp_ms >= fp_ms else ("forward", fp_ms)
+    achieved_tf = samples * FLOP_PER_SAMPLE / (dom_ms * 1e-3) / 1e12
+    gather_tbs = samples * GATHER_B_PER_SAMPLE / (dom_ms * 1e-3) / 1e12
+    roofline = {
+        "bound": "mfma",
+        "achieved": round(achieved_tf, 3),
+        "peak": FP32_PEAK_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4),
+        "traffic": None,
+        "kernel": dom,
+        "kernel_ms": round(dom_ms, 3),
+        "basis": "FP32 vector flops, 14 per bilinear sample (no MFMA: projection is a gather)",
+    }
+    rec = {
+        "metric": "GUPS (voxels x angles / s), FP+BP, 512^3 cone_vec, 720 angles, 512x768 detector",
+        "value": round(gups, 2),
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded box + N(0, 0.01) phantoms, helical cone_vec with seeded jitter)",
+        "config": {
+            "workload": f"{args.instances} x cone_vec 512^3, 720 angles, 512x768 detector, FP then BP",
+            "instances": args.instances,
+            "fp_ms_per_instance": round(fp_ms, 3),
+            "bp_ms_per_instance": round(bp_ms, 3),
+            "gups_fp": round(N_VOX ** 3 * N_ANGLES / (fp_ms * 1e-3) / 1e9, 2),
+            "gups_bp": round(N_VOX ** 3 * N_ANGLES / (bp_ms * 1e-3) / 1e9, 2),
+        },
+        "roofline": roofline,
+        "roofline_gather": {
+            "achieved": round(gather_tbs, 3), "peak": LDS_PEAK_TBS, "unit": "TB/s",
+            "frac": round(gather_tbs / LDS_PEAK_TBS, 4),
+            "basis": "16 B of bilinear corners per sample against aggregate LDS bandwidth",
+        },
+    }
+    if not args.no_cpu:
+        rec["cpu_baseline"] = cpu_baseline(ts, args.cpu_seconds)
+        rec["cpu_baseline"]["gpu_over_cpu"] = round(gups / rec["cpu_baseline"]["value"], 1)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

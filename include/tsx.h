@@ -93,6 +93,8 @@ typedef struct {
     int64_t vol_shape[3];      /* (z, y, x) */
     int64_t plan_bytes;        /* device bytes held by the plan */
     int32_t axis_count[3];     /* number of angles stepping along x, y, z */
+    int32_t fp_kernel;         /* 0 = general, 1 = padded per-axis layouts */
+    int32_t bp_kernel;         /* 0 = general, 1 = separable (upright detector) */
 } tsx_plan_info;
 
 int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);

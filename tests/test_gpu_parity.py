@@ -168,3 +168,26 @@ def test_autograd_grad_equals_ATAx():
     q = A_ag.T(p)
     q.backward(q)
     torch.testing.assert_close(p.grad, A(A.T(p.detach())))
+
+
+@pytest.mark.parametrize("kind", ["parallel", "cone"])
+def test_fast_and_general_kernels_agree(kind, monkeypatch):
+    """The padded FP / separable BP and the general kernels compute the same operator."""
+    vg = ts.volume(shape=(40, 36, 44), size=(1.0, 0.9, 1.1))
+    if kind == "parallel":
+        pg = ts.parallel(angles=45, shape=(40, 60), size=(1.2, 1.8))
+    else:
+        pg = _cone_vec_helix(45, 40, 60, 1.2, 1.8, 0.2, 1e-3, seed=9)
+    A = ts.operator(vg, pg)
+    info = A.projector._host_plan.info()
+    assert info["fp_kernel"] == "padded" and info["bp_kernel"] == "separable", info
+    x = torch.from_numpy(ts.phantom.ellipsoids(40, 6, seed=1)[:, :36, :44].copy()).to(DEV)
+    y_fast, b_fast = A(x), A.T(A(x))
+    monkeypatch.setenv("TSX_FORCE_GENERAL", "1")
+    y_gen, b_gen = A(x), A.T(A(x))
+    assert rel_l2(y_fast.cpu().numpy(), y_gen.cpu().numpy()) <= TOL
+    assert rel_l2(b_fast.cpu().numpy(), b_gen.cpu().numpy()) <= TOL
+    g = oracle.oracle_geometry(vg, pg)
+    y_ref = oracle.fp(g, x.cpu().numpy())
+    assert rel_l2(y_fast.cpu().numpy(), y_ref) <= TOL
+    assert rel_l2(y_gen.cpu().numpy(), y_ref) <= TOL

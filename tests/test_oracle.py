@@ -1,0 +1,141 @@
+"""Pinning the FP64 oracle (CPU).
+
+Each check is chosen so that a plausible convention mistake fails it:
+closed forms (axis-aligned sums, chord lengths, a single voxel), an
+independent world-coordinate dense-matrix restatement, the exact adjoint, and
+the transform equivariance that tomosipo's own operator tests assert
+(tests/test_Operator.py:52-112).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import numpy_twin
+import tomosipo_amd as ts
+
+
+def _c1():
+    vg = ts.volume(shape=64, size=1)
+    pg = ts.parallel(angles=96, shape=(64, 96), size=(1, 1.5))
+    return vg, pg
+
+
+def test_closed_form_theta_zero_and_ninety():
+    """Parallel beam, pixels aligned with voxels: y[i, a, j] = h * sum along the ray axis."""
+    vg, pg = _c1()
+    g = oracle.oracle_geometry(vg, pg)
+    x = np.random.default_rng(0).random(g.vol_shape)
+    y = oracle.fp(g, x)
+    # angle 0: rays along -y, u along +x; pixel j covers voxel column j - 16
+    np.testing.assert_allclose(y[:, 0, 16:80], x.sum(axis=1) / 64, rtol=1e-13, atol=1e-15)
+    assert np.all(y[:, 0, :16] == 0) and np.all(y[:, 0, 80:] == 0)
+    # angle pi/2 (index 48): rays along +x, u along +y
+    np.testing.assert_allclose(y[:, 48, 16:80], x.sum(axis=2) / 64, rtol=1e-12, atol=1e-15)
+
+
+def test_closed_form_chord_length_all_ones():
+    """All-ones volume: a ray whose every sample has full support gets n_slices * l."""
+    vg = ts.volume(shape=32, size=1)
+    t = 0.3
+    pg = ts.parallel(angles=[t], shape=(32, 48), size=(1, 1.5))
+    g = oracle.oracle_geometry(vg, pg)
+    y = oracle.fp(g, np.ones(g.vol_shape))
+    # stepping along y (|cos t| > |sin t|); the central pixel ray passes the centre
+    length = (1 / 32) * np.sqrt(1 + np.tan(t) ** 2)
+    assert y[16, 0, 24] == pytest.approx(32 * length, rel=1e-12)
+
+
+def test_single_voxel_single_pixel():
+    """1-voxel volume, 1-pixel detector through its centre: FP = voxel size."""
+    vg = ts.volume(shape=1, size=0.5)
+    for t, pg in ((0.2, ts.parallel(angles=[0.2], shape=1, size=0.1)),
+                  (1.1, ts.cone(angles=[1.1], shape=1, size=0.1, src_orig_dist=3, src_det_dist=5))):
+        g = oracle.oracle_geometry(vg, pg)
+        y = oracle.fp(g, np.ones((1, 1, 1)))
+        # the ray through the centre samples the only slice at the voxel centre;
+        # the stepping axis is the larger of |cos t| (y) and |sin t| (x)
+        e = max(abs(np.cos(t)), abs(np.sin(t)))
+        assert y[0, 0, 0] == pytest.approx(0.5 / e, rel=1e-12)
+
+
+def _tiny_geometries(rng):
+    out = []
+    for k in range(6):
+        shape = tuple(int(s) for s in rng.integers(2, 6, size=3))
+        vg = ts.volume(shape=shape, pos=rng.normal(0, 0.1, 3), size=rng.uniform(0.5, 1.5, 3))
+        T = ts.rotate(pos=0, axis=rng.normal(size=3), angles=rng.normal())
+        rows, cols = (int(s) for s in rng.integers(2, 6, size=2))
+        angles = rng.uniform(0, 2 * np.pi, 3)
+        if k % 2 == 0:
+            pg = T * ts.cone(angles=angles, shape=(rows, cols), size=rng.uniform(1, 3, 2),
+                             src_orig_dist=rng.uniform(2, 4), src_det_dist=rng.uniform(4, 7)).to_vec()
+        else:
+            pg = T * ts.parallel(angles=angles, shape=(rows, cols), size=rng.uniform(1, 3, 2)).to_vec()
+        out.append((vg, pg, 1 + k % 3))
+    return out
+
+
+def test_numpy_twin_matches_c_oracle(rng):
+    for vg, pg, ss in _tiny_geometries(rng):
+        g = oracle.oracle_geometry(vg, pg, det_supersampling=ss)
+        x = rng.random(g.vol_shape)
+        y = rng.random(g.sino_shape)
+        np.testing.assert_allclose(oracle.fp(g, x), numpy_twin.fp(g, x), rtol=1e-11, atol=1e-13)
+        np.testing.assert_allclose(oracle.bp(g, y), numpy_twin.bp(g, y), rtol=1e-11, atol=1e-13)
+
+
+def test_adjoint_exact_fp64(rng):
+    for vg, pg in [_c1(), (ts.volume(shape=(20, 24, 28), size=(1, 1.1, 1.2)),
+                          ts.cone(angles=30, shape=(24, 36), size=(1.6, 2.4), cone_angle=0.5))]:
+        g = oracle.oracle_geometry(vg, pg)
+        x = rng.random(g.vol_shape)
+        y = rng.random(g.sino_shape)
+        Ax, ATy = oracle.fp(g, x), oracle.bp(g, y)
+        err = abs(np.vdot(Ax, y) - np.vdot(x, ATy)) / (np.linalg.norm(Ax) * np.linalg.norm(y))
+        assert err < 1e-13
+
+
+def test_additive_mode(rng):
+    vg, pg = _c1()
+    g = oracle.oracle_geometry(vg, pg[::8])
+    x = rng.random(g.vol_shape)
+    y0 = rng.random(g.sino_shape)
+    np.testing.assert_allclose(oracle.fp(g, x, out=y0.copy()), y0 + oracle.fp(g, x), rtol=1e-13)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_transform_equivariance(seed):
+    """op(T vg, pg) == op(vg, T^-1 pg) (tomosipo tests/test_Operator.py:52-83)."""
+    from tomosipo_amd.Operator import to_astra_compatible_operator_geometry
+
+    rng = np.random.default_rng(seed)
+    vg = ts.volume(shape=12, size=1)
+    pg = ts.parallel(angles=16, shape=18, size=1.5).to_vec()
+    T = ts.translate(rng.normal(0, 0.1, 3)) * ts.rotate(pos=0, axis=rng.normal(size=3),
+                                                        angles=rng.normal())
+    vg_a, pg_a = to_astra_compatible_operator_geometry(T * vg.to_vec(), pg)
+    vg_b, pg_b = vg, T.inv * pg
+    x = rng.random(vg.shape)
+    ya = oracle.fp(oracle.oracle_geometry(vg_a, pg_a), x)
+    yb = oracle.fp(oracle.oracle_geometry(vg_b, pg_b), x)
+    np.testing.assert_allclose(ya, yb, rtol=1e-9, atol=1e-12)
+
+
+def test_dominant_axis_rule():
+    """Parallel beam over [0, pi): y for |cos| >= |sin| (ties x first), x otherwise."""
+    vg, pg = _c1()
+    axes = oracle.axes(oracle.oracle_geometry(vg, pg))
+    t = pg.angles
+    ray = np.stack([np.sin(t), -np.cos(t)], axis=1)  # (x, y) components
+    expect = np.where(np.abs(ray[:, 0]) >= np.abs(ray[:, 1]), 0, 1)
+    np.testing.assert_array_equal(axes, expect)
+
+
+def test_oracle_catches_planted_sign_error():
+    """A flipped detector u (a plausible sign mistake) breaks the theta=0 closed form."""
+    vg, pg = _c1()
+    bad = ts.parallel_vec(shape=pg.det_shape, ray_dir=pg.ray_dir, det_pos=pg.det_pos,
+                          det_v=pg.det_v, det_u=-pg.det_u)
+    x = np.random.default_rng(1).random((64, 64, 64))
+    y = oracle.fp(oracle.oracle_geometry(vg, bad), x)
+    assert not np.allclose(y[:, 0, 16:80], x.sum(axis=1) / 64)

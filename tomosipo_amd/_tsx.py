@@ -72,6 +72,8 @@ class TsxPlanInfo(ctypes.Structure):
         ("vol_shape", ctypes.c_int64 * 3),
         ("plan_bytes", ctypes.c_int64),
         ("axis_count", ctypes.c_int32 * 3),
+        ("fp_kernel", ctypes.c_int32),
+        ("bp_kernel", ctypes.c_int32),
     ]
 
 
@@ -179,6 +181,8 @@ class Plan:
             "device": inf.device, "n_angles": inf.n_angles, "det_rows": inf.det_rows,
             "det_cols": inf.det_cols, "vol_shape": tuple(inf.vol_shape),
             "plan_bytes": inf.plan_bytes, "axis_count": tuple(inf.axis_count),
+            "fp_kernel": ("general", "padded")[inf.fp_kernel],
+            "bp_kernel": ("general", "separable")[inf.bp_kernel],
         }
 
     def axes(self):

@@ -1,0 +1,331 @@
+// Backprojection kernels (included by tsx.hip). Both compute the exact
+// transpose of the Joseph forward projector:
+//   x[k] = sum_{a, i, j} l(a,i,j) * L(Q_c1 - k_c1) * L(Q_c2 - k_c2) * y[i, a, j],
+// where Q is the intersection of ray (a, i, j) with slice n = k_m,
+// L(t) = max(0, 1 - |t|) and l is the world length per slice step.
+//
+// bp_general: one thread per voxel; for every angle it maps the voxel's
+//   +-1 square in its slice to the detector (projective map around the
+//   voxel-centre ray) and sums over the pixels in the bounding box. Any
+//   geometry, any detector supersampling.
+//
+// bp_separable: for geometries whose detector v axis is parallel to the
+//   in-slice axis c1 = z and whose stepping axis is x or y (every circular,
+//   helical or jittered-source scan with an upright detector). Then Q_c2 of a
+//   ray depends only on its detector column j, and Q_z is affine in the row i,
+//   so per slice n
+//     x[n, kz, kc2] = sum_j L(Q_c2(j; n) - kc2) * u[n, j, kz],
+//     u[n, j, kz]   = sum_i L(A(j, n) + i B(j, n) - kz) * ly[i, j],
+//   with ly = l * y precomputed. A block owns a 16^3 brick; per angle it
+//   builds u for its 16 slices in LDS (pass 1, gathers from ly) and then
+//   every thread folds u into the 16 voxels of its z-column (pass 2). No
+//   atomics: the result is deterministic.
+#pragma once
+
+#include "tsx_plan.h"
+
+namespace tsx {
+
+constexpr int BP_BX = 64, BP_BY = 4;
+constexpr int BR = 16;  // brick edge of bp_separable
+
+__device__ __forceinline__ int sel3(int axis, int x, int y, int z)
+{
+    return axis == 0 ? x : (axis == 1 ? y : z);
+}
+
+__global__ void __launch_bounds__(BP_BX *BP_BY)
+bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
+           int nx, int ny, int nz, float *__restrict__ vol, int64_t vol_stride,
+           const float *__restrict__ proj, int64_t proj_stride, int additive)
+{
+    const int x = blockIdx.x * BP_BX + threadIdx.x;
+    const int y = blockIdx.y * BP_BY + threadIdx.y;
+    const int z = blockIdx.z % nz;
+    const int b = blockIdx.z / nz;
+    if (x >= nx || y >= ny) return;
+    const float *pr = proj + (int64_t)b * proj_stride;
+    const float inv_ss = 1.0f / (float)ss;
+    const float sub_margin = 0.5f - 0.5f * inv_ss;
+    float acc = 0.0f;
+    for (int a = 0; a < n_angles; ++a) {
+        const AngleParams &A = angles[a];
+        const double km = sel3(A.m, x, y, z), k1 = sel3(A.c1, x, y, z), k2 = sel3(A.c2, x, y, z);
+        double cj, ci, r1, r2, T, Bjc = 0.0, Bic = 0.0, g = 0.0;
+        if (A.cone) {
+            const double W0 = km - A.S[0], W1 = k1 - A.S[1], W2 = k2 - A.S[2];
+            const double dn = W0 * A.N[0] + W1 * A.N[1] + W2 * A.N[2];
+            if (dn == 0.0 || W0 == 0.0) continue;
+            const double lam = A.kappa / dn;
+            const double R0 = A.S[0] + lam * W0 - A.P0[0];
+            const double R1 = A.S[1] + lam * W1 - A.P0[1];
+            const double R2 = A.S[2] + lam * W2 - A.P0[2];
+            cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
+            ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
+            r1 = W1 / W0;
+            r2 = W2 / W0;
+            T = 1.0 / lam;
+            Bjc = A.U[0] / (lam * W0);
+            Bic = A.V[0] / (lam * W0);
+            g = 1.0 / W0;
+        } else {
+            const double t0 = ((A.P0[0] - km) * A.N[0] + (A.P0[1] - k1) * A.N[1] +
+                               (A.P0[2] - k2) * A.N[2]) / A.EN;
+            const double R0 = km + t0 * A.S[0] - A.P0[0];
+            const double R1 = k1 + t0 * A.S[1] - A.P0[1];
+            const double R2 = k2 + t0 * A.S[2] - A.P0[2];
+            cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
+            ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
+            r1 = A.S[1] / A.S[0];
+            r2 = A.S[2] / A.S[0];
+            T = 1.0;
+        }
+        const float A1j = (float)(T * (A.U[1] - r1 * A.U[0]));
+        const float A1i = (float)(T * (A.V[1] - r1 * A.V[0]));
+        const float A2j = (float)(T * (A.U[2] - r2 * A.U[0]));
+        const float A2i = (float)(T * (A.V[2] - r2 * A.V[0]));
+        const float fBj = (float)Bjc, fBi = (float)Bic;
+        const float det = A1j * A2i - A1i * A2j;
+        if (det == 0.0f) continue;
+        const float idet = 1.0f / det;
+        const float M00 = A2i * idet, M01 = -A1i * idet, M10 = -A2j * idet, M11 = A1j * idet;
+        const float cv0 = fBj * M00 + fBi * M10, cv1 = fBj * M01 + fBi * M11;
+        float djmin = 1e30f, djmax = -1e30f, dimin = 1e30f, dimax = -1e30f;
+        bool unbounded = false;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const float s1 = (s & 1) ? 1.0f : -1.0f, s2 = (s & 2) ? 1.0f : -1.0f;
+            const float den = 1.0f - (cv0 * s1 + cv1 * s2);
+            if (!(den > 1e-6f)) unbounded = true;
+            const float iden = 1.0f / den;
+            const float dj = (M00 * s1 + M01 * s2) * iden, di = (M10 * s1 + M11 * s2) * iden;
+            djmin = fminf(djmin, dj);
+            djmax = fmaxf(djmax, dj);
+            dimin = fminf(dimin, di);
+            dimax = fmaxf(dimax, di);
+        }
+        int j0, j1, i0, i1;
+        if (unbounded) {
+            j0 = 0; j1 = cols - 1; i0 = 0; i1 = rows - 1;
+        } else {
+            const double mj = sub_margin + 1e-3, mi = sub_margin + 1e-3;
+            j0 = (int)fmax(0.0, ceil(cj + djmin - mj));
+            j1 = (int)fmin((double)(cols - 1), floor(cj + djmax + mj));
+            i0 = (int)fmax(0.0, ceil(ci + dimin - mi));
+            i1 = (int)fmin((double)(rows - 1), floor(ci + dimax + mi));
+        }
+        if (j0 > j1 || i0 > i1) continue;
+        const float fr1 = (float)r1, fr2 = (float)r2, fg = (float)g;
+        const float h0 = (float)(A.h[0] * A.h[0]), h1 = (float)(A.h[1] * A.h[1]),
+                    h2 = (float)(A.h[2] * A.h[2]);
+        const float len_par = sqrtf(h0 + h1 * fr1 * fr1 + h2 * fr2 * fr2);
+        const float dj0 = (float)((double)j0 - cj), di0 = (float)((double)i0 - ci);
+        float acc_a = 0.0f;
+        for (int i = i0; i <= i1; ++i) {
+            const float *row = pr + ((int64_t)i * n_angles + a) * cols;
+            for (int j = j0; j <= j1; ++j) {
+                const float yv = row[j];
+                float wsum = 0.0f;
+                for (int p = 0; p < ss; ++p) {
+                    const float di = di0 + (float)(i - i0) + ((p + 0.5f) * inv_ss - 0.5f);
+                    for (int q = 0; q < ss; ++q) {
+                        const float dj = dj0 + (float)(j - j0) + ((q + 0.5f) * inv_ss - 0.5f);
+                        const float den = fmaf(dj, fBj, fmaf(di, fBi, 1.0f));
+                        const float iden = 1.0f / den;
+                        const float dq1 = fmaf(dj, A1j, di * A1i) * iden;
+                        const float dq2 = fmaf(dj, A2j, di * A2i) * iden;
+                        const float w = fmaxf(0.0f, 1.0f - fabsf(dq1)) * fmaxf(0.0f, 1.0f - fabsf(dq2));
+                        float len = len_par;
+                        if (A.cone) {
+                            const float s1 = fmaf(dq1, fg, fr1), s2 = fmaf(dq2, fg, fr2);
+                            len = sqrtf(h0 + h1 * s1 * s1 + h2 * s2 * s2);
+                        }
+                        wsum = fmaf(w, len, wsum);
+                    }
+                }
+                acc_a = fmaf(wsum, yv, acc_a);
+            }
+        }
+        acc += acc_a * (inv_ss * inv_ss);
+    }
+    float *out = vol + (int64_t)b * vol_stride + ((int64_t)z * ny + y) * nx + x;
+    *out = additive ? *out + acc : acc;
+}
+
+// ly[i, a, j] = l(a, i, j) * y[i, a, j] (detector supersampling 1).
+__global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_angles, int rows,
+                                int cols, const float *__restrict__ proj, int64_t proj_stride,
+                                float *__restrict__ out, int64_t out_stride)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.y;
+    const int i = blockIdx.z % rows;
+    const int b = blockIdx.z / rows;
+    if (j >= cols) return;
+    const AngleParams &A = angles[a];
+    float D[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float P = (float)(A.P0[c] + (double)j * A.U[c] + (double)i * A.V[c]);
+        D[c] = A.cone ? P - (float)A.S[c] : (float)A.S[c];
+    }
+    const float inv = 1.0f / D[0];
+    const float a1 = D[1] * inv, a2 = D[2] * inv;
+    const float len = (D[0] == 0.0f) ? 0.0f
+        : sqrtf((float)(A.h[0] * A.h[0]) + (float)(A.h[1] * A.h[1]) * a1 * a1 +
+                (float)(A.h[2] * A.h[2]) * a2 * a2);
+    const int64_t o = ((int64_t)i * n_angles + a) * cols + j;
+    out[(int64_t)b * out_stride + o] = len * proj[(int64_t)b * proj_stride + o];
+}
+
+struct BpShared {
+    float u[2][BR][BR][BR];  // [buffer][slice s][column jj][kz]
+    float q2[2][BR][BR];     // Q_c2 - c0 of column jj in slice s (1e30 if unused)
+    int jlo[BR];
+    int cnt[BR];
+};
+
+// Detector column where the ray through in-slice point (n, c) (axes m, c2)
+// meets the detector. Independent of z for an upright detector.
+__device__ __forceinline__ double column_of(const AngleParams &A, double n, double c)
+{
+    if (A.cone) {
+        const double W0 = n - A.S[0], W2 = c - A.S[2];
+        const double lam = A.kappa / (W0 * A.N[0] + W2 * A.N[2]);
+        return (A.S[0] + lam * W0 - A.P0[0]) * A.Bj[0] + (A.S[2] + lam * W2 - A.P0[2]) * A.Bj[2];
+    }
+    const double t0 = ((A.P0[0] - n) * A.N[0] + (A.P0[2] - c) * A.N[2]) / A.EN;
+    return (n + t0 * A.S[0] - A.P0[0]) * A.Bj[0] + (c + t0 * A.S[2] - A.P0[2]) * A.Bj[2];
+}
+
+__global__ void __launch_bounds__(BR *BR)
+bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int nx,
+             int ny, int nz, int bricks_x, int bricks_y, int bricks_z, int n_blocks,
+             float *__restrict__ vol, int64_t vol_stride, const float *__restrict__ ly,
+             int64_t ly_stride, int additive)
+{
+    __shared__ BpShared sh;
+    const int per = (n_blocks + 7) / 8;
+    const int logical = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+    if (logical >= n_blocks) return;
+    const int bx = logical % bricks_x;
+    const int by = (logical / bricks_x) % bricks_y;
+    const int bz = (logical / (bricks_x * bricks_y)) % bricks_z;
+    const int b = logical / (bricks_x * bricks_y * bricks_z);
+    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * BR + tx;
+    const int x = bx * BR + tx, y = by * BR + ty, z0 = bz * BR;
+    const float *yb = ly + (int64_t)b * ly_stride;
+    const int64_t row_stride = (int64_t)n_angles * cols;
+    float acc[BR];
+#pragma unroll
+    for (int k = 0; k < BR; ++k) acc[k] = 0.0f;
+    int buf = 0;
+    for (int a = 0; a < n_angles; ++a) {
+        const AngleParams &A = angles[a];
+        const bool mx = A.m == 0;
+        const int m0 = mx ? bx * BR : by * BR;
+        const int c0 = mx ? by * BR : bx * BR;
+        // slice ranges: columns j whose Q_c2 lies in (c0 - 1, c0 + BR)
+        if (t < BR) {
+            const double n = (double)(m0 + t);
+            const double ja = column_of(A, n, (double)c0 - 1.0);
+            const double jb = column_of(A, n, (double)(c0 + BR));
+            const int lo = max(0, (int)ceil(fmin(ja, jb) - 1e-3));
+            const int hi = min(cols - 1, (int)floor(fmax(ja, jb) + 1e-3));
+            sh.jlo[t] = lo;
+            sh.cnt[t] = max(0, hi - lo + 1);
+        }
+        __syncthreads();
+        int chunks = 0;
+#pragma unroll
+        for (int s = 0; s < BR; ++s) chunks = max(chunks, (sh.cnt[s] + BR - 1) / BR);
+        for (int c = 0; c < chunks; ++c) {
+            // pass 1: thread (s = ty, jj = tx) resamples detector column j onto kz
+            {
+                const int s = ty, jj = tx;
+                const int j = sh.jlo[s] + c * BR + jj;
+                float *u = sh.u[buf][s][jj];
+                if (c * BR + jj < sh.cnt[s]) {
+                    const double n = (double)(m0 + s);
+                    const double Pm = A.P0[0] + j * A.U[0];
+                    const double Pz = A.P0[1] + j * A.U[1];
+                    const double P2 = A.P0[2] + j * A.U[2];
+                    double Az, Bz, Q2;
+                    if (A.cone) {
+                        const double tau = (n - A.S[0]) / (Pm - A.S[0]);
+                        Az = A.S[1] + tau * (Pz - A.S[1]);
+                        Bz = tau * A.V[1];
+                        Q2 = A.S[2] + tau * (P2 - A.S[2]);
+                    } else {
+                        const double tau = (n - Pm) / A.S[0];
+                        Az = Pz + tau * A.S[1];
+                        Bz = A.V[1];
+                        Q2 = P2 + tau * A.S[2];
+                    }
+                    // Q_z(i) - z0 = Ap + (i - iref) * Bz with small operands
+                    const double Arel = Az - (double)z0;
+                    const int iref = (int)rint(fmin(fmax(-Arel / Bz, -1e8), 1e8));
+                    const float Ap = (float)(Arel + (double)iref * Bz);
+                    const float fB = (float)Bz;
+                    const float iB = 1.0f / fB;
+                    sh.q2[buf][s][jj] = (float)(Q2 - (double)c0);
+                    const float *col = yb + (int64_t)a * cols + j;
+#pragma unroll 4
+                    for (int kz = 0; kz < BR; ++kz) {
+                        // rows with |Ap + di * Bz - kz| < 1
+                        const float ta = ((float)kz - 1.0f - Ap) * iB;
+                        const float tb = ((float)kz + 1.0f - Ap) * iB;
+                        const int d0 = max((int)ceilf(fminf(ta, tb)), -iref);
+                        const int d1 = min((int)floorf(fmaxf(ta, tb)), rows - 1 - iref);
+                        float sum = 0.0f;
+                        for (int d = d0; d <= d1; ++d) {
+                            const float w = fmaxf(0.0f, 1.0f - fabsf(fmaf((float)d, fB, Ap) - (float)kz));
+                            sum = fmaf(w, col[(int64_t)(iref + d) * row_stride], sum);
+                        }
+                        u[kz] = sum;
+                    }
+                } else {
+                    sh.q2[buf][s][jj] = 1e30f;
+#pragma unroll
+                    for (int kz = 0; kz < BR; ++kz) u[kz] = 0.0f;
+                }
+            }
+            __syncthreads();
+            // pass 2: fold the columns into this thread's z-column
+            {
+                const int s = mx ? tx : ty;
+                const float kc = (float)(mx ? ty : tx);
+#pragma unroll 1
+                for (int jj = 0; jj < BR; ++jj) {
+                    const float w = 1.0f - fabsf(sh.q2[buf][s][jj] - kc);
+                    if (w > 0.0f) {
+                        const float4 *u4 = reinterpret_cast<const float4 *>(sh.u[buf][s][jj]);
+#pragma unroll
+                        for (int q = 0; q < BR / 4; ++q) {
+                            const float4 v = u4[q];
+                            acc[4 * q + 0] = fmaf(w, v.x, acc[4 * q + 0]);
+                            acc[4 * q + 1] = fmaf(w, v.y, acc[4 * q + 1]);
+                            acc[4 * q + 2] = fmaf(w, v.z, acc[4 * q + 2]);
+                            acc[4 * q + 3] = fmaf(w, v.w, acc[4 * q + 3]);
+                        }
+                    }
+                }
+            }
+            buf ^= 1;
+        }
+        __syncthreads();  // slice ranges of the next angle overwrite jlo/cnt
+    }
+    if (x < nx && y < ny) {
+        float *out = vol + (int64_t)b * vol_stride + (int64_t)y * nx + x;
+#pragma unroll
+        for (int k = 0; k < BR; ++k) {
+            const int z = z0 + k;
+            if (z < nz) {
+                float *o = out + (int64_t)z * ny * nx;
+                *o = additive ? *o + acc[k] : acc[k];
+            }
+        }
+    }
+}
+
+}  // namespace tsx

@@ -1,0 +1,273 @@
+// Forward-projection kernels (included by tsx.hip).
+//
+// fp_general: one thread per (sub-)ray, bounds-checked bilinear reads from the
+//   caller's [z][y][x] volume; any geometry. Kept as the reference
+//   implementation and for volumes whose padded copies would not fit 32-bit
+//   byte offsets.
+// fp_padded: one thread per (sub-)ray, reading from a zero-padded copy of the
+//   volume laid out per stepping axis so that the in-slice axis c2 is
+//   contiguous (see the layout kernels below). No per-sample bounds checks; raw buffer
+//   loads with 32-bit offsets (out-of-range reads return 0 in hardware).
+#pragma once
+
+#include "tsx_plan.h"
+
+namespace tsx {
+
+constexpr int FP_BX = 64, FP_BY = 4;
+constexpr int PAD = 2;  // zero border on each side of c1 and c2 in padded layouts
+
+struct RaySetup {
+    float a1, a2;    // slope of Q_c per slice
+    float R1, R2;    // Q_c at slice nref (padded coordinates in fp_padded)
+    float len;       // world length per slice step
+    int nref, nlo, nhi;
+};
+
+// FP64 setup of the ray through detector point (fj, fi). Returns false when the
+// ray has no sample with non-zero weight. Q_c(n) = R_c + (n - nref) a_c.
+__device__ __forceinline__ bool ray_setup(const AngleParams &A, double fj, double fi, double off,
+                                          RaySetup &r)
+{
+    const double Pm = A.P0[0] + fj * A.U[0] + fi * A.V[0];
+    const double P1 = A.P0[1] + fj * A.U[1] + fi * A.V[1];
+    const double P2 = A.P0[2] + fj * A.U[2] + fi * A.V[2];
+    const double Dm = A.cone ? Pm - A.S[0] : A.S[0];
+    const double D1 = A.cone ? P1 - A.S[1] : A.S[1];
+    const double D2 = A.cone ? P2 - A.S[2] : A.S[2];
+    if (Dm == 0.0) return false;
+    const double inv = 1.0 / Dm;
+    const double a1 = D1 * inv, a2 = D2 * inv;
+    const int nm = A.n[0], n1 = A.n[1], n2 = A.n[2];
+    const int nref = nm / 2;
+    const double R1 = P1 + ((double)nref - Pm) * a1;
+    const double R2 = P2 + ((double)nref - Pm) * a2;
+    // slices n with Q_c(n) in (-1, n_c) for both in-slice axes
+    double lo = -1e30, hi = 1e30;
+    if (a1 != 0.0) {
+        const double t0 = (-1.0 - R1) / a1, t1 = ((double)n1 - R1) / a1;
+        lo = fmax(lo, fmin(t0, t1));
+        hi = fmin(hi, fmax(t0, t1));
+    } else if (!(R1 > -1.0 && R1 < (double)n1)) {
+        return false;
+    }
+    if (a2 != 0.0) {
+        const double t0 = (-1.0 - R2) / a2, t1 = ((double)n2 - R2) / a2;
+        lo = fmax(lo, fmin(t0, t1));
+        hi = fmin(hi, fmax(t0, t1));
+    } else if (!(R2 > -1.0 && R2 < (double)n2)) {
+        return false;
+    }
+    lo = fmax(lo + nref, -1.0);
+    hi = fmin(hi + nref, (double)nm);
+    if (!(lo < hi)) return false;
+    r.nlo = max(0, (int)ceil(lo));
+    r.nhi = min(nm - 1, (int)floor(hi));
+    if (r.nlo > r.nhi) return false;
+    r.a1 = (float)a1;
+    r.a2 = (float)a2;
+    r.R1 = (float)(R1 + off);
+    r.R2 = (float)(R2 + off);
+    r.len = (float)sqrt(A.h[0] * A.h[0] + A.h[1] * A.h[1] * a1 * a1 + A.h[2] * A.h[2] * a2 * a2);
+    r.nref = nref;
+    return true;
+}
+
+__device__ float fp_ray_general(const AngleParams &A, const float *__restrict__ vol, double fj,
+                                double fi)
+{
+    RaySetup r;
+    if (!ray_setup(A, fj, fi, 0.0, r)) return 0.0f;
+    const int n1 = A.n[1], n2 = A.n[2];
+    const int64_t sm = A.st[0], s1 = A.st[1], s2 = A.st[2];
+    float total = 0.0f, part = 0.0f;
+    int cnt = 0;
+    for (int n = r.nlo; n <= r.nhi; ++n) {
+        const float dn = (float)(n - r.nref);
+        const float q1 = fmaf(dn, r.a1, r.R1);
+        const float q2 = fmaf(dn, r.a2, r.R2);
+        const float f1 = floorf(q1), f2 = floorf(q2);
+        const int k1 = (int)f1, k2 = (int)f2;
+        const float w1 = q1 - f1, w2 = q2 - f2;
+        const bool a0 = (unsigned)k1 < (unsigned)n1, a1ok = (unsigned)(k1 + 1) < (unsigned)n1;
+        const bool b0 = (unsigned)k2 < (unsigned)n2, b1ok = (unsigned)(k2 + 1) < (unsigned)n2;
+        const int64_t base = (int64_t)n * sm + (int64_t)k1 * s1 + (int64_t)k2 * s2;
+        const float v00 = (a0 && b0) ? vol[base] : 0.0f;
+        const float v10 = (a1ok && b0) ? vol[base + s1] : 0.0f;
+        const float v01 = (a0 && b1ok) ? vol[base + s2] : 0.0f;
+        const float v11 = (a1ok && b1ok) ? vol[base + s1 + s2] : 0.0f;
+        const float lo2 = fmaf(w1, v10 - v00, v00);
+        const float hi2 = fmaf(w1, v11 - v01, v01);
+        part += fmaf(w2, hi2 - lo2, lo2);
+        if (++cnt == 16) {
+            total += part;
+            part = 0.0f;
+            cnt = 0;
+        }
+    }
+    return r.len * (total + part);
+}
+
+__global__ void __launch_bounds__(FP_BX *FP_BY)
+fp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
+           int tiles_u, const float *__restrict__ vol, int64_t vol_stride,
+           float *__restrict__ proj, int64_t proj_stride, int additive)
+{
+    const int tile = blockIdx.x;
+    const int j = (tile % tiles_u) * FP_BX + threadIdx.x;
+    const int i = (tile / tiles_u) * FP_BY + threadIdx.y;
+    const int a = blockIdx.y;
+    const int b = blockIdx.z;
+    if (j >= cols || i >= rows) return;
+    const AngleParams &A = angles[a];
+    const float *v = vol + (int64_t)b * vol_stride;
+    float acc = 0.0f;
+    const double inv = 1.0 / ss;
+    for (int p = 0; p < ss; ++p)
+        for (int q = 0; q < ss; ++q)
+            acc += fp_ray_general(A, v, j + (q + 0.5) * inv - 0.5, i + (p + 0.5) * inv - 0.5);
+    acc *= (float)(inv * inv);
+    float *out = proj + (int64_t)b * proj_stride + ((int64_t)i * n_angles + a) * cols + j;
+    *out = additive ? *out + acc : acc;
+}
+
+// ---------------------------------------------------------------------------
+// padded per-axis layouts
+// ---------------------------------------------------------------------------
+// Layout for stepping axis x: dims [nz + 2P][nx][ny + 2P], c1 = z, c2 = y.
+// Layout for stepping axis y: dims [nz + 2P][ny][nx + 2P], c1 = z, c2 = x.
+// Layout for stepping axis z: dims [nz][ny + 2P][nx + 2P], c1 = y, c2 = x.
+// Element (m, c1, c2) of an angle sits at m*pst[0] + (c1+P)*pst[1] + (c2+P).
+
+__global__ void pad_copy_yz(const float *__restrict__ src, float *__restrict__ dst, int nx, int ny,
+                            int nz, int axis)
+{
+    // axis 1: dst [nz+2P][ny][nx+2P];  axis 2: dst [nz][ny+2P][nx+2P]
+    const int X = nx + 2 * PAD;
+    const int Y = axis == 1 ? ny : ny + 2 * PAD;
+    const int xp = blockIdx.x * blockDim.x + threadIdx.x;
+    const int yp = blockIdx.y;
+    const int zp = blockIdx.z;
+    if (xp >= X) return;
+    const int x = xp - PAD;
+    const int y = axis == 1 ? yp : yp - PAD;
+    const int z = axis == 1 ? zp - PAD : zp;
+    float v = 0.0f;
+    if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
+        v = src[((int64_t)z * ny + y) * nx + x];
+    dst[((int64_t)zp * Y + yp) * X + xp] = v;
+}
+
+// dst [nz+2P][nx][ny+2P] from src [nz][ny][nx], 32x32 tiles through LDS.
+__global__ void pad_transpose_x(const float *__restrict__ src, float *__restrict__ dst, int nx,
+                                int ny, int nz)
+{
+    __shared__ float tile[32][33];
+    const int Yp = ny + 2 * PAD;
+    const int zp = blockIdx.z;
+    const int z = zp - PAD;
+    const int x0 = blockIdx.x * 32, yp0 = blockIdx.y * 32;
+    for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+        const int yp = yp0 + r, x = x0 + threadIdx.x, y = yp - PAD;
+        float v = 0.0f;
+        if (z >= 0 && z < nz && y >= 0 && y < ny && x < nx) v = src[((int64_t)z * ny + y) * nx + x];
+        tile[r][threadIdx.x] = v;
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+        const int x = x0 + r, yp = yp0 + threadIdx.x;
+        if (x < nx && yp < Yp) dst[((int64_t)zp * nx + x) * Yp + yp] = tile[threadIdx.x][r];
+    }
+}
+
+__device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t byte_off)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
+}
+
+struct Layouts {
+    const float *ptr[3];   // per stepping axis (x, y, z); null if unused
+    uint32_t bytes[3];     // size of each layout in bytes (< 2^32)
+    int64_t stride;        // elements between batch entries of the same layout
+};
+
+// Marches one ray through a padded layout. Q and k are in padded coordinates.
+__device__ __forceinline__ float march_padded(__amdgpu_buffer_rsrc_t rs, const RaySetup &r,
+                                              uint32_t st_m, uint32_t st_1)
+{
+    float dn = (float)(r.nlo - r.nref);
+    uint32_t row = (uint32_t)r.nlo * st_m;
+    float total = 0.0f;
+    int n = r.nlo;
+    const uint32_t rowb = st_1 * 4u;
+    while (n <= r.nhi) {
+        const int stop = min(r.nhi, n + 15);
+        float part = 0.0f;
+        for (; n <= stop; ++n) {
+            const float q1 = fmaf(dn, r.a1, r.R1);
+            const float q2 = fmaf(dn, r.a2, r.R2);
+            // The slice range keeps Q within [-1, n_c] (up to rounding), inside the
+            // zero border; any out-of-range offset is caught by the buffer range check.
+            const int k1 = __float2int_rd(q1);
+            const int k2 = __float2int_rd(q2);
+            const float w1 = __builtin_amdgcn_fractf(q1);
+            const float w2 = __builtin_amdgcn_fractf(q2);
+            const uint32_t off = (row + __umul24((uint32_t)k1, st_1) + (uint32_t)k2) * 4u;
+            const float v00 = bload(rs, off);
+            const float v01 = bload(rs, off + 4u);
+            const float v10 = bload(rs, off + rowb);
+            const float v11 = bload(rs, off + rowb + 4u);
+            const float lo = fmaf(w2, v01 - v00, v00);
+            const float hi = fmaf(w2, v11 - v10, v10);
+            part += fmaf(w1, hi - lo, lo);
+            dn += 1.0f;
+            row += st_m;
+        }
+        total += part;
+    }
+    return r.len * total;
+}
+
+template <int SS>
+__global__ void __launch_bounds__(FP_BX *FP_BY)
+fp_padded(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int tiles_u,
+          int tiles_v, int n_blocks, Layouts L, int batch, float *__restrict__ proj,
+          int64_t proj_stride, int additive)
+{
+    // XCD-aware order: consecutive hardware block ids go round-robin to the 8
+    // XCDs; give each XCD a contiguous range of logical ids. Logical ids run
+    // angle fastest so that blocks resident together share a detector tile and
+    // therefore a slab of the volume.
+    const int per = (n_blocks + 7) / 8;
+    const int logical = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+    if (logical >= n_blocks) return;
+    const int a = logical % n_angles;
+    const int rest = logical / n_angles;
+    const int tiles = tiles_u * tiles_v;
+    const int tile = rest % tiles;
+    const int b = rest / tiles;
+    const int j = (tile % tiles_u) * FP_BX + threadIdx.x;
+    const int i = (tile / tiles_u) * FP_BY + threadIdx.y;
+    if (j >= cols || i >= rows || b >= batch) return;
+    const AngleParams &A = angles[a];
+    const float *lp = L.ptr[A.m] + (int64_t)b * L.stride;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)lp, (short)0, (int)L.bytes[A.m], 0x00020000);
+    const uint32_t st_m = (uint32_t)A.pst[0], st_1 = (uint32_t)A.pst[1];
+    float acc = 0.0f;
+#pragma unroll
+    for (int p = 0; p < SS; ++p) {
+#pragma unroll
+        for (int q = 0; q < SS; ++q) {
+            RaySetup r;
+            const double fj = SS == 1 ? (double)j : j + (q + 0.5) / SS - 0.5;
+            const double fi = SS == 1 ? (double)i : i + (p + 0.5) / SS - 0.5;
+            if (ray_setup(A, fj, fi, (double)PAD, r)) acc += march_padded(rs, r, st_m, st_1);
+        }
+    }
+    if (SS > 1) acc *= 1.0f / (float)(SS * SS);
+    float *out = proj + (int64_t)b * proj_stride + ((int64_t)i * n_angles + a) * cols + j;
+    *out = additive ? *out + acc : acc;
+}
+
+}  // namespace tsx

@@ -16,14 +16,18 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/tsx.h"
 #include "tsx_plan.h"
+#include "fp_kernels.h"
+#include "bp_kernels.h"
 
 using tsx::AngleParams;
+using namespace tsx;
 
 // ---------------------------------------------------------------------------
 // errors
@@ -52,6 +56,11 @@ struct tsx_plan {
     std::vector<AngleParams> host;
     AngleParams *dev = nullptr;
     int32_t axis_count[3] = {0, 0, 0};
+    // padded per-axis layouts for the forward projector (elements per layout)
+    int64_t layout_elems[3] = {0, 0, 0};
+    bool fp_padded = false;   // all used layouts addressable with 32-bit byte offsets
+    bool bp_separable = false;  // upright detector, x/y stepping, ss == 1
+    std::string bp_reason;      // why bp_separable is off
 };
 
 // ---------------------------------------------------------------------------
@@ -174,6 +183,20 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
             A.n[k] = (int32_t)n3[c];
             A.st[k] = stride3[c];
         }
+        {
+            const int64_t P2 = 2 * PAD;
+            if (m == 0) {  // [nz+2P][nx][ny+2P]
+                A.pst[0] = n3[1] + P2;
+                A.pst[1] = n3[0] * (n3[1] + P2);
+            } else if (m == 1) {  // [nz+2P][ny][nx+2P]
+                A.pst[0] = n3[0] + P2;
+                A.pst[1] = n3[1] * (n3[0] + P2);
+            } else {  // [nz][ny+2P][nx+2P]
+                A.pst[0] = (n3[1] + P2) * (n3[0] + P2);
+                A.pst[1] = n3[0] + P2;
+            }
+            A.pst[2] = 1;
+        }
         A.kappa = cone ? dot(P0 - S, N) : 0.0;
         A.EN = cone ? 0.0 : dot(S, N);
         A.m = m;
@@ -182,247 +205,48 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
         A.cone = cone ? 1 : 0;
         p->axis_count[m] += 1;
     }
-    return TSX_OK;
-}
-
-}  // namespace
-
-// ---------------------------------------------------------------------------
-// kernels
-// ---------------------------------------------------------------------------
-namespace {
-
-constexpr int FP_BX = 64, FP_BY = 4;
-constexpr int BP_BX = 64, BP_BY = 4;
-
-__device__ __forceinline__ int sel3(int axis, int x, int y, int z)
-{
-    return axis == 0 ? x : (axis == 1 ? y : z);
-}
-
-// One (sub-)ray of the forward projector. (fj, fi) are detector coordinates in
-// pixel units with pixel centres on integers.
-__device__ float fp_ray(const AngleParams &A, const float *__restrict__ vol, double fj, double fi)
-{
-    const double Pm = A.P0[0] + fj * A.U[0] + fi * A.V[0];
-    const double P1 = A.P0[1] + fj * A.U[1] + fi * A.V[1];
-    const double P2 = A.P0[2] + fj * A.U[2] + fi * A.V[2];
-    const double Dm = A.cone ? Pm - A.S[0] : A.S[0];
-    const double D1 = A.cone ? P1 - A.S[1] : A.S[1];
-    const double D2 = A.cone ? P2 - A.S[2] : A.S[2];
-    if (Dm == 0.0) return 0.0f;
-    const double inv = 1.0 / Dm;
-    const double a1 = D1 * inv, a2 = D2 * inv;
-    const double len = sqrt(A.h[0] * A.h[0] + A.h[1] * A.h[1] * a1 * a1 + A.h[2] * A.h[2] * a2 * a2);
-    const int nm = A.n[0], n1 = A.n[1], n2 = A.n[2];
-    const int nref = nm / 2;
-    const double R1 = P1 + ((double)nref - Pm) * a1;
-    const double R2 = P2 + ((double)nref - Pm) * a2;
-    // slices where (Q1, Q2) can touch the volume: Q_c in (-1, n_c)
-    double lo = -1e30, hi = 1e30;
-    if (a1 != 0.0) {
-        double t0 = (-1.0 - R1) / a1, t1 = ((double)n1 - R1) / a1;
-        lo = fmax(lo, fmin(t0, t1));
-        hi = fmin(hi, fmax(t0, t1));
-    } else if (!(R1 > -1.0 && R1 < (double)n1)) {
-        return 0.0f;
-    }
-    if (a2 != 0.0) {
-        double t0 = (-1.0 - R2) / a2, t1 = ((double)n2 - R2) / a2;
-        lo = fmax(lo, fmin(t0, t1));
-        hi = fmin(hi, fmax(t0, t1));
-    } else if (!(R2 > -1.0 && R2 < (double)n2)) {
-        return 0.0f;
-    }
-    lo = fmax(lo + nref, -1.0);
-    hi = fmin(hi + nref, (double)nm);
-    if (!(lo <= hi)) return 0.0f;
-    const int nlo = max(0, (int)floor(lo));
-    const int nhi = min(nm - 1, (int)ceil(hi));
-    const float fa1 = (float)a1, fa2 = (float)a2, fR1 = (float)R1, fR2 = (float)R2;
-    const int64_t sm = A.st[0], s1 = A.st[1], s2 = A.st[2];
-    float total = 0.0f;
-    float part = 0.0f;
-    int cnt = 0;
-    for (int n = nlo; n <= nhi; ++n) {
-        const float dn = (float)(n - nref);
-        const float q1 = fmaf(dn, fa1, fR1);
-        const float q2 = fmaf(dn, fa2, fR2);
-        const float f1 = floorf(q1), f2 = floorf(q2);
-        const int k1 = (int)f1, k2 = (int)f2;
-        const float w1 = q1 - f1, w2 = q2 - f2;
-        const bool a0 = (unsigned)k1 < (unsigned)n1, a1ok = (unsigned)(k1 + 1) < (unsigned)n1;
-        const bool b0 = (unsigned)k2 < (unsigned)n2, b1ok = (unsigned)(k2 + 1) < (unsigned)n2;
-        const int64_t base = (int64_t)n * sm + (int64_t)k1 * s1 + (int64_t)k2 * s2;
-        const float v00 = (a0 && b0) ? vol[base] : 0.0f;
-        const float v10 = (a1ok && b0) ? vol[base + s1] : 0.0f;
-        const float v01 = (a0 && b1ok) ? vol[base + s2] : 0.0f;
-        const float v11 = (a1ok && b1ok) ? vol[base + s1 + s2] : 0.0f;
-        const float lo2 = fmaf(w1, v10 - v00, v00);
-        const float hi2 = fmaf(w1, v11 - v01, v01);
-        part += fmaf(w2, hi2 - lo2, lo2);
-        if (++cnt == 32) {
-            total += part;
-            part = 0.0f;
-            cnt = 0;
-        }
-    }
-    total += part;
-    return (float)len * total;
-}
-
-__global__ void __launch_bounds__(FP_BX *FP_BY)
-fp_kernel(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
-          int tiles_u, const float *__restrict__ vol, int64_t vol_stride, float *__restrict__ proj,
-          int64_t proj_stride, int additive)
-{
-    const int tile = blockIdx.x;
-    const int j = (tile % tiles_u) * FP_BX + threadIdx.x;
-    const int i = (tile / tiles_u) * FP_BY + threadIdx.y;
-    const int a = blockIdx.y;
-    const int b = blockIdx.z;
-    if (j >= cols || i >= rows) return;
-    const AngleParams &A = angles[a];
-    const float *v = vol + (int64_t)b * vol_stride;
-    float acc;
-    if (ss == 1) {
-        acc = fp_ray(A, v, (double)j, (double)i);
-    } else {
-        acc = 0.0f;
-        const double inv = 1.0 / ss;
-        for (int p = 0; p < ss; ++p)
-            for (int q = 0; q < ss; ++q)
-                acc += fp_ray(A, v, j + (q + 0.5) * inv - 0.5, i + (p + 0.5) * inv - 0.5);
-        acc *= (float)(inv * inv);
-    }
-    float *out = proj + (int64_t)b * proj_stride + ((int64_t)i * n_angles + a) * cols + j;
-    *out = additive ? *out + acc : acc;
-}
-
-// Exact transpose of fp_kernel, voxel-driven. For every angle the voxel's
-// slice n = k_m is intersected by the rays whose samples in that slice lie
-// within the open square |Q - k| < 1. The rays through the voxel centre meet
-// the detector at (cj, ci); around it the in-slice offset of a ray at
-// detector offset (dj, di) is the projective map
-//   dq_c = (dj*A_cj + di*A_ci) / (1 + dj*Bj + di*Bi)   (Bj = Bi = 0 for parallel),
-// whose inverse images of the square corners bound the pixels to visit.
-__global__ void __launch_bounds__(BP_BX *BP_BY)
-bp_kernel(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
-          int nx, int ny, int nz, float *__restrict__ vol, int64_t vol_stride,
-          const float *__restrict__ proj, int64_t proj_stride, int additive)
-{
-    const int x = blockIdx.x * BP_BX + threadIdx.x;
-    const int y = blockIdx.y * BP_BY + threadIdx.y;
-    const int z = blockIdx.z % nz;
-    const int b = blockIdx.z / nz;
-    if (x >= nx || y >= ny) return;
-    const float *pr = proj + (int64_t)b * proj_stride;
-    const float inv_ss = 1.0f / (float)ss;
-    const float sub_margin = 0.5f - 0.5f * inv_ss;
-    float acc = 0.0f;
-    for (int a = 0; a < n_angles; ++a) {
-        const AngleParams &A = angles[a];
-        const double km = sel3(A.m, x, y, z), k1 = sel3(A.c1, x, y, z), k2 = sel3(A.c2, x, y, z);
-        double cj, ci, r1, r2, T, Bjc = 0.0, Bic = 0.0, g = 0.0;
-        if (A.cone) {
-            const double W0 = km - A.S[0], W1 = k1 - A.S[1], W2 = k2 - A.S[2];
-            const double dn = W0 * A.N[0] + W1 * A.N[1] + W2 * A.N[2];
-            if (dn == 0.0 || W0 == 0.0) continue;
-            const double lam = A.kappa / dn;
-            const double R0 = A.S[0] + lam * W0 - A.P0[0];
-            const double R1 = A.S[1] + lam * W1 - A.P0[1];
-            const double R2 = A.S[2] + lam * W2 - A.P0[2];
-            cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
-            ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
-            r1 = W1 / W0;
-            r2 = W2 / W0;
-            T = 1.0 / lam;
-            Bjc = A.U[0] / (lam * W0);
-            Bic = A.V[0] / (lam * W0);
-            g = 1.0 / W0;
-        } else {
-            const double t0 = ((A.P0[0] - km) * A.N[0] + (A.P0[1] - k1) * A.N[1] +
-                               (A.P0[2] - k2) * A.N[2]) / A.EN;
-            const double R0 = km + t0 * A.S[0] - A.P0[0];
-            const double R1 = k1 + t0 * A.S[1] - A.P0[1];
-            const double R2 = k2 + t0 * A.S[2] - A.P0[2];
-            cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
-            ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
-            r1 = A.S[1] / A.S[0];
-            r2 = A.S[2] / A.S[0];
-            T = 1.0;
-        }
-        const float A1j = (float)(T * (A.U[1] - r1 * A.U[0]));
-        const float A1i = (float)(T * (A.V[1] - r1 * A.V[0]));
-        const float A2j = (float)(T * (A.U[2] - r2 * A.U[0]));
-        const float A2i = (float)(T * (A.V[2] - r2 * A.V[0]));
-        const float fBj = (float)Bjc, fBi = (float)Bic;
-        // inverse map of the square corners (dq1, dq2) = (+-1, +-1)
-        const float det = A1j * A2i - A1i * A2j;
-        if (det == 0.0f) continue;
-        const float idet = 1.0f / det;
-        const float M00 = A2i * idet, M01 = -A1i * idet, M10 = -A2j * idet, M11 = A1j * idet;
-        const float cv0 = fBj * M00 + fBi * M10, cv1 = fBj * M01 + fBi * M11;
-        float djmin = 1e30f, djmax = -1e30f, dimin = 1e30f, dimax = -1e30f;
-        bool unbounded = false;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const float s1 = (s & 1) ? 1.0f : -1.0f, s2 = (s & 2) ? 1.0f : -1.0f;
-            const float den = 1.0f - (cv0 * s1 + cv1 * s2);
-            if (!(den > 1e-6f)) unbounded = true;
-            const float iden = 1.0f / den;
-            const float dj = (M00 * s1 + M01 * s2) * iden, di = (M10 * s1 + M11 * s2) * iden;
-            djmin = fminf(djmin, dj);
-            djmax = fmaxf(djmax, dj);
-            dimin = fminf(dimin, di);
-            dimax = fmaxf(dimax, di);
-        }
-        int j0, j1, i0, i1;
-        if (unbounded) {
-            j0 = 0; j1 = cols - 1; i0 = 0; i1 = rows - 1;
-        } else {
-            const double mj = sub_margin + 1e-3, mi = sub_margin + 1e-3;
-            j0 = (int)fmax(0.0, ceil(cj + djmin - mj));
-            j1 = (int)fmin((double)(cols - 1), floor(cj + djmax + mj));
-            i0 = (int)fmax(0.0, ceil(ci + dimin - mi));
-            i1 = (int)fmin((double)(rows - 1), floor(ci + dimax + mi));
-        }
-        if (j0 > j1 || i0 > i1) continue;
-        const float fr1 = (float)r1, fr2 = (float)r2, fg = (float)g;
-        const float h0 = (float)(A.h[0] * A.h[0]), h1 = (float)(A.h[1] * A.h[1]),
-                    h2 = (float)(A.h[2] * A.h[2]);
-        const float len_par = sqrtf(h0 + h1 * fr1 * fr1 + h2 * fr2 * fr2);
-        const float dj0 = (float)((double)j0 - cj), di0 = (float)((double)i0 - ci);
-        float acc_a = 0.0f;
-        for (int i = i0; i <= i1; ++i) {
-            const float* row = pr + ((int64_t)i * n_angles + a) * cols;
-            for (int j = j0; j <= j1; ++j) {
-                const float yv = row[j];
-                float wsum = 0.0f;
-                for (int p = 0; p < ss; ++p) {
-                    const float di = di0 + (float)(i - i0) + ((p + 0.5f) * inv_ss - 0.5f);
-                    for (int q = 0; q < ss; ++q) {
-                        const float dj = dj0 + (float)(j - j0) + ((q + 0.5f) * inv_ss - 0.5f);
-                        const float den = fmaf(dj, fBj, fmaf(di, fBi, 1.0f));
-                        const float iden = 1.0f / den;
-                        const float dq1 = fmaf(dj, A1j, di * A1i) * iden;
-                        const float dq2 = fmaf(dj, A2j, di * A2i) * iden;
-                        const float w = fmaxf(0.0f, 1.0f - fabsf(dq1)) * fmaxf(0.0f, 1.0f - fabsf(dq2));
-                        float len = len_par;
-                        if (A.cone) {
-                            const float s1 = fmaf(dq1, fg, fr1), s2 = fmaf(dq2, fg, fr2);
-                            len = sqrtf(h0 + h1 * s1 * s1 + h2 * s2 * s2);
-                        }
-                        wsum = fmaf(w, len, wsum);
+    // padded layouts of the forward projector
+    const int64_t P2 = 2 * PAD;
+    p->layout_elems[0] = (n3[2] + P2) * n3[0] * (n3[1] + P2);
+    p->layout_elems[1] = (n3[2] + P2) * n3[1] * (n3[0] + P2);
+    p->layout_elems[2] = n3[2] * (n3[1] + P2) * (n3[0] + P2);
+    p->fp_padded = true;
+    for (int c = 0; c < 3; ++c)
+        if (p->axis_count[c] > 0 && p->layout_elems[c] * 4 >= ((int64_t)1 << 32)) p->fp_padded = false;
+    // separable backprojection: upright detector (v along z), stepping along x or y
+    p->bp_separable = g->det_supersampling == 1;
+    if (!p->bp_separable) p->bp_reason = "detector supersampling";
+    for (int a = 0; a < g->n_angles && p->bp_separable; ++a) {
+        const AngleParams &A = p->host[a];
+        const double vn = std::sqrt(A.V[0] * A.V[0] + A.V[1] * A.V[1] + A.V[2] * A.V[2]);
+        if (A.m == 2) {
+            p->bp_separable = false;
+            p->bp_reason = "angle " + std::to_string(a) + " steps along z";
+        } else if (std::fabs(A.V[0]) > 1e-12 * vn || std::fabs(A.V[2]) > 1e-12 * vn) {
+            p->bp_separable = false;
+            p->bp_reason = "detector v is not parallel to z at angle " + std::to_string(a);
+        } else if (A.cone) {
+            // the source must lie outside the slab of slices (with margin), and every
+            // voxel must project with a positive scale onto the detector plane
+            // bricks cover the volume rounded up to whole bricks
+            const double nm = (double)((A.n[0] + BR - 1) / BR * BR);
+            const double nc = (double)((A.n[2] + BR - 1) / BR * BR);
+            if (!(A.S[0] < -2.0 || A.S[0] > nm + 1.0)) {
+                p->bp_separable = false;
+                p->bp_reason = "source within the slice range at angle " + std::to_string(a);
+            }
+            for (int cm = 0; cm < 2 && p->bp_separable; ++cm)
+                for (int cc = 0; cc < 2 && p->bp_separable; ++cc) {
+                    const double X0 = cm ? nm + 1.0 : -2.0, X2 = cc ? nc + 1.0 : -2.0;
+                    const double dn = (X0 - A.S[0]) * A.N[0] + (X2 - A.S[2]) * A.N[2];
+                    if (!(A.kappa / dn > 0.0)) {
+                        p->bp_separable = false;
+                        p->bp_reason = "volume not in front of the source at angle " + std::to_string(a);
                     }
                 }
-                acc_a = fmaf(wsum, yv, acc_a);
-            }
         }
-        acc += acc_a * (inv_ss * inv_ss);
     }
-    float *out = vol + (int64_t)b * vol_stride + ((int64_t)z * ny + y) * nx + x;
-    *out = additive ? *out + acc : acc;
+    return TSX_OK;
 }
 
 }  // namespace
@@ -453,18 +277,126 @@ struct DeviceGuard {
     }
 };
 
+bool force_general()
+{
+    const char *e = std::getenv("TSX_FORCE_GENERAL");
+    return e && e[0] == '1';
+}
+
+// Stream-ordered scratch memory, returned to the device pool when the work
+// on `stream` that uses it has finished.
+struct Scratch {
+    void *ptr = nullptr;
+    hipStream_t stream = nullptr;
+    int alloc(size_t bytes, hipStream_t s)
+    {
+        stream = s;
+        hipError_t e = hipMallocAsync(&ptr, bytes, s);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return fail(TSX_ERR_OUT_OF_MEMORY, std::string("hipMallocAsync of ") +
+                                                   std::to_string(bytes) + " bytes failed: " +
+                                                   hipGetErrorString(e));
+        }
+        return TSX_OK;
+    }
+    ~Scratch()
+    {
+        if (ptr) (void)hipFreeAsync(ptr, stream);
+    }
+};
+
+void keep_pool_memory(int device)
+{
+    static bool done[64] = {false};
+    if (device < 0 || device >= 64 || done[device]) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+        uint64_t thr = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    done[device] = true;
+}
+
 int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int64_t ps,
               int64_t batch, int additive, hipStream_t stream)
 {
     if (batch <= 0) return TSX_OK;
-    if (p->n_angles > 65535 || batch > 65535)
-        return fail(TSX_ERR_UNSUPPORTED, "more than 65535 angles or batch elements per launch");
     const int tiles_u = (p->cols + FP_BX - 1) / FP_BX;
     const int tiles_v = (p->rows + FP_BY - 1) / FP_BY;
-    dim3 grid((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
-    dim3 block(FP_BX, FP_BY, 1);
-    hipLaunchKernelGGL(fp_kernel, grid, block, 0, stream, p->dev, p->n_angles, p->rows, p->cols,
-                       p->ss, tiles_u, vol, vs, proj, ps, additive);
+    if (!p->fp_padded || force_general()) {
+        if (p->n_angles > 65535 || batch > 65535)
+            return fail(TSX_ERR_UNSUPPORTED, "more than 65535 angles or batch elements per launch");
+        dim3 grid((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
+        hipLaunchKernelGGL(fp_general, grid, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
+                           p->rows, p->cols, p->ss, tiles_u, vol, vs, proj, ps, additive);
+        TSX_HIP_CHECK(hipGetLastError());
+        return TSX_OK;
+    }
+    // padded copies of every batch entry for each stepping axis in use
+    int64_t per_entry = 0;
+    int64_t offs[3] = {0, 0, 0};
+    for (int c = 0; c < 3; ++c)
+        if (p->axis_count[c] > 0) {
+            offs[c] = per_entry;
+            per_entry += p->layout_elems[c];
+        }
+    keep_pool_memory(p->device);
+    Scratch ws;
+    int rc = ws.alloc((size_t)(per_entry * batch) * sizeof(float), stream);
+    if (rc) return rc;
+    float *base = (float *)ws.ptr;
+    for (int64_t b = 0; b < batch; ++b) {
+        const float *src = vol + b * vs;
+        float *dst = base + b * per_entry;
+        if (p->axis_count[0] > 0) {
+            dim3 grid((unsigned)((p->nx + 31) / 32), (unsigned)((p->ny + 2 * PAD + 31) / 32),
+                      (unsigned)(p->nz + 2 * PAD));
+            hipLaunchKernelGGL(pad_transpose_x, grid, dim3(32, 8), 0, stream, src, dst + offs[0],
+                               p->nx, p->ny, p->nz);
+        }
+        if (p->axis_count[1] > 0) {
+            dim3 grid((unsigned)((p->nx + 2 * PAD + 255) / 256), (unsigned)p->ny,
+                      (unsigned)(p->nz + 2 * PAD));
+            hipLaunchKernelGGL(pad_copy_yz, grid, dim3(256), 0, stream, src, dst + offs[1], p->nx,
+                               p->ny, p->nz, 1);
+        }
+        if (p->axis_count[2] > 0) {
+            dim3 grid((unsigned)((p->nx + 2 * PAD + 255) / 256), (unsigned)(p->ny + 2 * PAD),
+                      (unsigned)p->nz);
+            hipLaunchKernelGGL(pad_copy_yz, grid, dim3(256), 0, stream, src, dst + offs[2], p->nx,
+                               p->ny, p->nz, 2);
+        }
+    }
+    TSX_HIP_CHECK(hipGetLastError());
+    Layouts L;
+    for (int c = 0; c < 3; ++c) {
+        L.ptr[c] = p->axis_count[c] > 0 ? base + offs[c] : nullptr;
+        L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4);
+    }
+    L.stride = per_entry;
+    const int64_t nb = (int64_t)tiles_u * tiles_v * p->n_angles * batch;
+    if (nb > (int64_t)0x7fffffff - 8) return fail(TSX_ERR_UNSUPPORTED, "forward projection too large for one launch");
+    const int n_blocks = (int)nb;
+    const unsigned grid = (unsigned)(((n_blocks + 7) / 8) * 8);
+    switch (p->ss) {
+#define TSX_FP_CASE(K)                                                                             \
+    case K:                                                                                        \
+        hipLaunchKernelGGL(fp_padded<K>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, p->dev,        \
+                           p->n_angles, p->rows, p->cols, tiles_u, tiles_v, n_blocks, L,           \
+                           (int)batch, proj, ps, additive);                                        \
+        break;
+        TSX_FP_CASE(1)
+        TSX_FP_CASE(2)
+        TSX_FP_CASE(3)
+        TSX_FP_CASE(4)
+#undef TSX_FP_CASE
+    default: {
+        dim3 g2((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
+        hipLaunchKernelGGL(fp_general, g2, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
+                           p->rows, p->cols, p->ss, tiles_u, vol, vs, proj, ps, additive);
+    }
+    }
     TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
 }
@@ -473,13 +405,37 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
               int64_t batch, int additive, hipStream_t stream)
 {
     if (batch <= 0) return TSX_OK;
-    if ((int64_t)p->nz * batch > 65535)
-        return fail(TSX_ERR_UNSUPPORTED, "volume depth times batch exceeds 65535 per launch");
-    dim3 grid((unsigned)((p->nx + BP_BX - 1) / BP_BX), (unsigned)((p->ny + BP_BY - 1) / BP_BY),
-              (unsigned)(p->nz * batch));
-    dim3 block(BP_BX, BP_BY, 1);
-    hipLaunchKernelGGL(bp_kernel, grid, block, 0, stream, p->dev, p->n_angles, p->rows, p->cols,
-                       p->ss, p->nx, p->ny, p->nz, vol, vs, proj, ps, additive);
+    if (!p->bp_separable || force_general()) {
+        if ((int64_t)p->nz * batch > 65535)
+            return fail(TSX_ERR_UNSUPPORTED, "volume depth times batch exceeds 65535 per launch");
+        dim3 grid((unsigned)((p->nx + BP_BX - 1) / BP_BX), (unsigned)((p->ny + BP_BY - 1) / BP_BY),
+                  (unsigned)(p->nz * batch));
+        hipLaunchKernelGGL(bp_general, grid, dim3(BP_BX, BP_BY), 0, stream, p->dev, p->n_angles,
+                           p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, proj, ps, additive);
+        TSX_HIP_CHECK(hipGetLastError());
+        return TSX_OK;
+    }
+    const int64_t sino = (int64_t)p->rows * p->n_angles * p->cols;
+    if ((int64_t)p->rows * batch > 65535)
+        return fail(TSX_ERR_UNSUPPORTED, "detector rows times batch exceeds 65535 per launch");
+    keep_pool_memory(p->device);
+    Scratch ws;
+    int rc = ws.alloc((size_t)(sino * batch) * sizeof(float), stream);
+    if (rc) return rc;
+    float *ly = (float *)ws.ptr;
+    {
+        dim3 grid((unsigned)((p->cols + 255) / 256), (unsigned)p->n_angles, (unsigned)(p->rows * batch));
+        hipLaunchKernelGGL(scale_by_length, grid, dim3(256), 0, stream, p->dev, p->n_angles, p->rows,
+                           p->cols, proj, ps, ly, sino);
+    }
+    const int bx = (p->nx + BR - 1) / BR, by = (p->ny + BR - 1) / BR, bz = (p->nz + BR - 1) / BR;
+    const int64_t nb = (int64_t)bx * by * bz * batch;
+    if (nb > (int64_t)0x7fffffff - 8) return fail(TSX_ERR_UNSUPPORTED, "backprojection too large for one launch");
+    const int n_blocks = (int)nb;
+    const unsigned grid = (unsigned)(((n_blocks + 7) / 8) * 8);
+    hipLaunchKernelGGL(bp_separable, dim3(grid), dim3(BR, BR), 0, stream, p->dev, p->n_angles,
+                       p->rows, p->cols, p->nx, p->ny, p->nz, bx, by, bz, n_blocks, vol, vs, ly,
+                       sino, additive);
     TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
 }
@@ -582,6 +538,8 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->vol_shape[2] = p->nx;
     info->plan_bytes = p->dev ? (int64_t)(sizeof(AngleParams) * p->host.size()) : 0;
     for (int c = 0; c < 3; ++c) info->axis_count[c] = p->axis_count[c];
+    info->fp_kernel = p->fp_padded && !force_general() ? 1 : 0;
+    info->bp_kernel = p->bp_separable && !force_general() ? 1 : 0;
     return TSX_OK;
 }
 

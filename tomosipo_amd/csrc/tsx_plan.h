@@ -23,7 +23,8 @@ struct AngleParams {
     int32_t cone;
     int32_t n[3];         // volume extent along (m, c1, c2)
     int32_t pad;
-    int64_t st[3];        // element strides along (m, c1, c2)
+    int64_t st[3];        // element strides along (m, c1, c2) in the [z][y][x] volume
+    int64_t pst[3];       // element strides along (m, c1, c2) in the padded layout of axis m
 };
 
 }  // namespace tsx

@@ -10,7 +10,7 @@ from .vec import (ParallelVectorGeometry, ConeVectorGeometry, parallel_vec, cone
 from .parametric import (ParallelGeometry, ConeGeometry, parallel, cone, random_parallel,
                          random_cone)
 from .concatenate import concatenate
-from . import volume as _volume_mod
+from .volume import from_astra as _volume_from_astra
 
 
 def from_astra_projection_geometry(astra_pg):
@@ -28,7 +28,7 @@ def from_astra(astra_geom):
     if not isinstance(astra_geom, dict):
         raise TypeError(f"Expected an ASTRA geometry dictionary. Got {type(astra_geom)}.")
     if "GridSliceCount" in astra_geom:
-        return _volume_mod.from_astra(astra_geom)
+        return _volume_from_astra(astra_geom)
     return from_astra_projection_geometry(astra_geom)
 
 
