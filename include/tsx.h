@@ -125,6 +125,17 @@ int tsx_forward_batched(const tsx_plan *plan, const float *vol, int64_t vol_stri
 int tsx_backward_batched(const tsx_plan *plan, float *vol, int64_t vol_stride, const float *proj,
                          int64_t proj_stride, int64_t batch, int additive, void *hip_stream);
 
+/* FP64 forward and backprojection (general kernels, any geometry). No
+ * counterpart in the reference: ASTRA projects float32 only. For callers that
+ * need the operator and its transpose to FP64 rounding, such as
+ * finite-difference gradient checks of the autograd wrapper. */
+int tsx_forward_batched_f64(const tsx_plan *plan, const double *vol, int64_t vol_stride,
+                            double *proj, int64_t proj_stride, int64_t batch, int additive,
+                            void *hip_stream);
+int tsx_backward_batched_f64(const tsx_plan *plan, double *vol, int64_t vol_stride,
+                             const double *proj, int64_t proj_stride, int64_t batch, int additive,
+                             void *hip_stream);
+
 /* Message describing the last failure on this thread ("" if none). */
 const char *tsx_last_error(void);
 

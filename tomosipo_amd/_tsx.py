@@ -36,7 +36,8 @@ _STATUS = {
 EXPORTED = (
     "tsx_plan_create", "tsx_plan_destroy", "tsx_plan_info_get", "tsx_plan_axes",
     "tsx_direct_fpbp3d", "tsx_forward", "tsx_backward", "tsx_forward_batched",
-    "tsx_backward_batched", "tsx_last_error", "tsx_version",
+    "tsx_backward_batched", "tsx_forward_batched_f64", "tsx_backward_batched_f64",
+    "tsx_last_error", "tsx_version",
 )
 
 
@@ -105,6 +106,8 @@ def load():
         lib.tsx_backward.argtypes = [P, P, P, i32, P]
         lib.tsx_forward_batched.argtypes = [P, P, i64, P, i64, i64, i32, P]
         lib.tsx_backward_batched.argtypes = [P, P, i64, P, i64, i64, i32, P]
+        lib.tsx_forward_batched_f64.argtypes = [P, P, i64, P, i64, i64, i32, P]
+        lib.tsx_backward_batched_f64.argtypes = [P, P, i64, P, i64, i64, i32, P]
         for name in EXPORTED:
             if name not in ("tsx_last_error", "tsx_version"):
                 getattr(lib, name).restype = ctypes.c_int
@@ -198,6 +201,14 @@ class Plan:
     def backward(self, vol_ptr, proj_ptr, additive, stream, batch=1, vol_stride=0, proj_stride=0):
         _check(self._lib.tsx_backward_batched(self._h, vol_ptr, vol_stride, proj_ptr, proj_stride,
                                               batch, int(bool(additive)), stream))
+
+    def forward_f64(self, vol_ptr, proj_ptr, additive, stream, batch=1, vol_stride=0, proj_stride=0):
+        _check(self._lib.tsx_forward_batched_f64(self._h, vol_ptr, vol_stride, proj_ptr,
+                                                 proj_stride, batch, int(bool(additive)), stream))
+
+    def backward_f64(self, vol_ptr, proj_ptr, additive, stream, batch=1, vol_stride=0, proj_stride=0):
+        _check(self._lib.tsx_backward_batched_f64(self._h, vol_ptr, vol_stride, proj_ptr,
+                                                  proj_stride, batch, int(bool(additive)), stream))
 
 
 def version():

@@ -23,6 +23,7 @@
 #pragma once
 
 #include "tsx_plan.h"
+#include "fp_kernels.h"
 
 namespace tsx {
 
@@ -34,24 +35,24 @@ __device__ __forceinline__ int sel3(int axis, int x, int y, int z)
     return axis == 0 ? x : (axis == 1 ? y : z);
 }
 
+template <typename T>
 __global__ void __launch_bounds__(BP_BX *BP_BY)
 bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
-           int nx, int ny, int nz, float *__restrict__ vol, int64_t vol_stride,
-           const float *__restrict__ proj, int64_t proj_stride, int additive)
+           int nx, int ny, int nz, T *__restrict__ vol, int64_t vol_stride,
+           const T *__restrict__ proj, int64_t proj_stride, int additive)
 {
     const int x = blockIdx.x * BP_BX + threadIdx.x;
     const int y = blockIdx.y * BP_BY + threadIdx.y;
     const int z = blockIdx.z % nz;
     const int b = blockIdx.z / nz;
     if (x >= nx || y >= ny) return;
-    const float *pr = proj + (int64_t)b * proj_stride;
-    const float inv_ss = 1.0f / (float)ss;
-    const float sub_margin = 0.5f - 0.5f * inv_ss;
-    float acc = 0.0f;
+    const T *pr = proj + (int64_t)b * proj_stride;
+    const double sub_margin = 0.5 - 0.5 / ss;
+    T acc = T(0);
     for (int a = 0; a < n_angles; ++a) {
         const AngleParams &A = angles[a];
         const double km = sel3(A.m, x, y, z), k1 = sel3(A.c1, x, y, z), k2 = sel3(A.c2, x, y, z);
-        double cj, ci, r1, r2, T, Bjc = 0.0, Bic = 0.0, g = 0.0;
+        double cj, ci, r1, r2, Tc, Bjc = 0.0, Bic = 0.0, g = 0.0;
         if (A.cone) {
             const double W0 = km - A.S[0], W1 = k1 - A.S[1], W2 = k2 - A.S[2];
             const double dn = W0 * A.N[0] + W1 * A.N[1] + W2 * A.N[2];
@@ -64,7 +65,7 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
             ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
             r1 = W1 / W0;
             r2 = W2 / W0;
-            T = 1.0 / lam;
+            Tc = 1.0 / lam;
             Bjc = A.U[0] / (lam * W0);
             Bic = A.V[0] / (lam * W0);
             g = 1.0 / W0;
@@ -78,31 +79,31 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
             ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
             r1 = A.S[1] / A.S[0];
             r2 = A.S[2] / A.S[0];
-            T = 1.0;
+            Tc = 1.0;
         }
-        const float A1j = (float)(T * (A.U[1] - r1 * A.U[0]));
-        const float A1i = (float)(T * (A.V[1] - r1 * A.V[0]));
-        const float A2j = (float)(T * (A.U[2] - r2 * A.U[0]));
-        const float A2i = (float)(T * (A.V[2] - r2 * A.V[0]));
-        const float fBj = (float)Bjc, fBi = (float)Bic;
-        const float det = A1j * A2i - A1i * A2j;
-        if (det == 0.0f) continue;
-        const float idet = 1.0f / det;
-        const float M00 = A2i * idet, M01 = -A1i * idet, M10 = -A2j * idet, M11 = A1j * idet;
-        const float cv0 = fBj * M00 + fBi * M10, cv1 = fBj * M01 + fBi * M11;
-        float djmin = 1e30f, djmax = -1e30f, dimin = 1e30f, dimax = -1e30f;
+        const T A1j = (T)(Tc * (A.U[1] - r1 * A.U[0]));
+        const T A1i = (T)(Tc * (A.V[1] - r1 * A.V[0]));
+        const T A2j = (T)(Tc * (A.U[2] - r2 * A.U[0]));
+        const T A2i = (T)(Tc * (A.V[2] - r2 * A.V[0]));
+        const T fBj = (T)Bjc, fBi = (T)Bic;
+        const double det = (double)A1j * A2i - (double)A1i * A2j;
+        if (det == 0.0) continue;
+        const double idet = 1.0 / det;
+        const double M00 = A2i * idet, M01 = -A1i * idet, M10 = -A2j * idet, M11 = A1j * idet;
+        const double cv0 = fBj * M00 + fBi * M10, cv1 = fBj * M01 + fBi * M11;
+        double djmin = 1e30, djmax = -1e30, dimin = 1e30, dimax = -1e30;
         bool unbounded = false;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const float s1 = (s & 1) ? 1.0f : -1.0f, s2 = (s & 2) ? 1.0f : -1.0f;
-            const float den = 1.0f - (cv0 * s1 + cv1 * s2);
-            if (!(den > 1e-6f)) unbounded = true;
-            const float iden = 1.0f / den;
-            const float dj = (M00 * s1 + M01 * s2) * iden, di = (M10 * s1 + M11 * s2) * iden;
-            djmin = fminf(djmin, dj);
-            djmax = fmaxf(djmax, dj);
-            dimin = fminf(dimin, di);
-            dimax = fmaxf(dimax, di);
+            const double s1 = (s & 1) ? 1.0 : -1.0, s2 = (s & 2) ? 1.0 : -1.0;
+            const double den = 1.0 - (cv0 * s1 + cv1 * s2);
+            if (!(den > 1e-6)) unbounded = true;
+            const double iden = 1.0 / den;
+            const double dj = (M00 * s1 + M01 * s2) * iden, di = (M10 * s1 + M11 * s2) * iden;
+            djmin = fmin(djmin, dj);
+            djmax = fmax(djmax, dj);
+            dimin = fmin(dimin, di);
+            dimax = fmax(dimax, di);
         }
         int j0, j1, i0, i1;
         if (unbounded) {
@@ -115,40 +116,41 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
             i1 = (int)fmin((double)(rows - 1), floor(ci + dimax + mi));
         }
         if (j0 > j1 || i0 > i1) continue;
-        const float fr1 = (float)r1, fr2 = (float)r2, fg = (float)g;
-        const float h0 = (float)(A.h[0] * A.h[0]), h1 = (float)(A.h[1] * A.h[1]),
-                    h2 = (float)(A.h[2] * A.h[2]);
-        const float len_par = sqrtf(h0 + h1 * fr1 * fr1 + h2 * fr2 * fr2);
-        const float dj0 = (float)((double)j0 - cj), di0 = (float)((double)i0 - ci);
-        float acc_a = 0.0f;
+        const T fr1 = (T)r1, fr2 = (T)r2, fg = (T)g;
+        const T h0 = (T)(A.h[0] * A.h[0]), h1 = (T)(A.h[1] * A.h[1]), h2 = (T)(A.h[2] * A.h[2]);
+        const T len_par = sqrt(h0 + h1 * fr1 * fr1 + h2 * fr2 * fr2);
+        const T dj0 = (T)((double)j0 - cj), di0 = (T)((double)i0 - ci);
+        const T tinv = (T)1 / (T)ss;
+        T acc_a = T(0);
         for (int i = i0; i <= i1; ++i) {
-            const float *row = pr + ((int64_t)i * n_angles + a) * cols;
+            const T *row = pr + ((int64_t)i * n_angles + a) * cols;
             for (int j = j0; j <= j1; ++j) {
-                const float yv = row[j];
-                float wsum = 0.0f;
+                const T yv = row[j];
+                T wsum = T(0);
                 for (int p = 0; p < ss; ++p) {
-                    const float di = di0 + (float)(i - i0) + ((p + 0.5f) * inv_ss - 0.5f);
+                    const T di = di0 + (T)(i - i0) + ((T)(p + 0.5) * tinv - (T)0.5);
                     for (int q = 0; q < ss; ++q) {
-                        const float dj = dj0 + (float)(j - j0) + ((q + 0.5f) * inv_ss - 0.5f);
-                        const float den = fmaf(dj, fBj, fmaf(di, fBi, 1.0f));
-                        const float iden = 1.0f / den;
-                        const float dq1 = fmaf(dj, A1j, di * A1i) * iden;
-                        const float dq2 = fmaf(dj, A2j, di * A2i) * iden;
-                        const float w = fmaxf(0.0f, 1.0f - fabsf(dq1)) * fmaxf(0.0f, 1.0f - fabsf(dq2));
-                        float len = len_par;
+                        const T dj = dj0 + (T)(j - j0) + ((T)(q + 0.5) * tinv - (T)0.5);
+                        const T den = dj * fBj + (di * fBi + T(1));
+                        const T iden = T(1) / den;
+                        const T dq1 = (dj * A1j + di * A1i) * iden;
+                        const T dq2 = (dj * A2j + di * A2i) * iden;
+                        const T w1 = T(1) - fabs(dq1), w2 = T(1) - fabs(dq2);
+                        const T w = (w1 > T(0) ? w1 : T(0)) * (w2 > T(0) ? w2 : T(0));
+                        T len = len_par;
                         if (A.cone) {
-                            const float s1 = fmaf(dq1, fg, fr1), s2 = fmaf(dq2, fg, fr2);
-                            len = sqrtf(h0 + h1 * s1 * s1 + h2 * s2 * s2);
+                            const T s1 = dq1 * fg + fr1, s2 = dq2 * fg + fr2;
+                            len = sqrt(h0 + h1 * s1 * s1 + h2 * s2 * s2);
                         }
-                        wsum = fmaf(w, len, wsum);
+                        wsum += w * len;
                     }
                 }
-                acc_a = fmaf(wsum, yv, acc_a);
+                acc_a += wsum * yv;
             }
         }
-        acc += acc_a * (inv_ss * inv_ss);
+        acc += acc_a * (tinv * tinv);
     }
-    float *out = vol + (int64_t)b * vol_stride + ((int64_t)z * ny + y) * nx + x;
+    T *out = vol + (int64_t)b * vol_stride + ((int64_t)z * ny + y) * nx + x;
     *out = additive ? *out + acc : acc;
 }
 
@@ -178,9 +180,14 @@ __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_an
     out[(int64_t)b * out_stride + o] = len * proj[(int64_t)b * proj_stride + o];
 }
 
+// Column j of slice s keeps u at UCOL-float stride; kz = -1 .. 16 live at
+// UOFF - 1 .. UOFF + 16 so that the splat needs no range test and the 16
+// voxel entries are 16-byte aligned.
+constexpr int UOFF = 4, UCOL = 24;
+
 struct BpShared {
-    float u[2][BR][BR][BR];  // [buffer][slice s][column jj][kz]
-    float q2[2][BR][BR];     // Q_c2 - c0 of column jj in slice s (1e30 if unused)
+    float u[BR][BR][UCOL];  // [slice s][column jj][UOFF + kz]
+    float q2[BR][BR];       // Q_c2 - c0 of column jj in slice s (1e30 if unused)
     int jlo[BR];
     int cnt[BR];
 };
@@ -202,7 +209,7 @@ __global__ void __launch_bounds__(BR *BR)
 bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int nx,
              int ny, int nz, int bricks_x, int bricks_y, int bricks_z, int n_blocks,
              float *__restrict__ vol, int64_t vol_stride, const float *__restrict__ ly,
-             int64_t ly_stride, int additive)
+             uint32_t ly_bytes, int additive)
 {
     __shared__ BpShared sh;
     const int per = (n_blocks + 7) / 8;
@@ -214,12 +221,12 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
     const int b = logical / (bricks_x * bricks_y * bricks_z);
     const int tx = threadIdx.x, ty = threadIdx.y, t = ty * BR + tx;
     const int x = bx * BR + tx, y = by * BR + ty, z0 = bz * BR;
-    const float *yb = ly + (int64_t)b * ly_stride;
-    const int64_t row_stride = (int64_t)n_angles * cols;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(ly + (int64_t)b * ((int64_t)ly_bytes / 4)), (short)0, (int)ly_bytes, 0x00020000);
+    const uint32_t row_bytes = (uint32_t)n_angles * (uint32_t)cols * 4u;
     float acc[BR];
 #pragma unroll
     for (int k = 0; k < BR; ++k) acc[k] = 0.0f;
-    int buf = 0;
     for (int a = 0; a < n_angles; ++a) {
         const AngleParams &A = angles[a];
         const bool mx = A.m == 0;
@@ -240,11 +247,15 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
 #pragma unroll
         for (int s = 0; s < BR; ++s) chunks = max(chunks, (sh.cnt[s] + BR - 1) / BR);
         for (int c = 0; c < chunks; ++c) {
-            // pass 1: thread (s = ty, jj = tx) resamples detector column j onto kz
+            // pass 1: thread (s = ty, jj = tx) splats detector column j onto kz:
+            // row i lands at z-offset t_i = Q_z(i) - z0 and adds (1 - frac) to
+            // floor(t_i) and frac to floor(t_i) + 1 of its own LDS column.
             {
                 const int s = ty, jj = tx;
                 const int j = sh.jlo[s] + c * BR + jj;
-                float *u = sh.u[buf][s][jj];
+                float *u = &sh.u[s][jj][UOFF];
+#pragma unroll
+                for (int k = -UOFF; k < UCOL - UOFF; ++k) u[k] = 0.0f;
                 if (c * BR + jj < sh.cnt[s]) {
                     const double n = (double)(m0 + s);
                     const double Pm = A.P0[0] + j * A.U[0];
@@ -262,32 +273,37 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
                         Bz = A.V[1];
                         Q2 = P2 + tau * A.S[2];
                     }
-                    // Q_z(i) - z0 = Ap + (i - iref) * Bz with small operands
+                    sh.q2[s][jj] = (float)(Q2 - (double)c0);
+                    // rows with t_i = Arel + i * Bz in (-1, BR)
                     const double Arel = Az - (double)z0;
-                    const int iref = (int)rint(fmin(fmax(-Arel / Bz, -1e8), 1e8));
-                    const float Ap = (float)(Arel + (double)iref * Bz);
+                    const double ra = (-1.0 - Arel) / Bz, rb = ((double)BR - Arel) / Bz;
+                    const int i0 = max(0, (int)ceil(fmax(fmin(ra, rb), -1e9)));
+                    const int i1 = min(rows - 1, (int)floor(fmin(fmax(ra, rb), 1e9)));
+                    const float t0 = (float)(Arel + (double)i0 * Bz);
                     const float fB = (float)Bz;
-                    const float iB = 1.0f / fB;
-                    sh.q2[buf][s][jj] = (float)(Q2 - (double)c0);
-                    const float *col = yb + (int64_t)a * cols + j;
-#pragma unroll 4
-                    for (int kz = 0; kz < BR; ++kz) {
-                        // rows with |Ap + di * Bz - kz| < 1
-                        const float ta = ((float)kz - 1.0f - Ap) * iB;
-                        const float tb = ((float)kz + 1.0f - Ap) * iB;
-                        const int d0 = max((int)ceilf(fminf(ta, tb)), -iref);
-                        const int d1 = min((int)floorf(fmaxf(ta, tb)), rows - 1 - iref);
-                        float sum = 0.0f;
-                        for (int d = d0; d <= d1; ++d) {
-                            const float w = fmaxf(0.0f, 1.0f - fabsf(fmaf((float)d, fB, Ap) - (float)kz));
-                            sum = fmaf(w, col[(int64_t)(iref + d) * row_stride], sum);
+                    uint32_t off = (uint32_t)i0 * row_bytes + ((uint32_t)a * (uint32_t)cols + (uint32_t)j) * 4u;
+                    // four rows per step: their loads are in flight together; rows past
+                    // i1 are read (inside the buffer or zero by the range check) and
+                    // given weight zero
+                    for (int i = i0; i <= i1; i += 4) {
+                        float yv[4];
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                                  rs, off + (uint32_t)r * row_bytes, 0, 0));
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const float ti = fmaf((float)(i + r - i0), fB, t0);
+                            const int kf = min(max(floor_to_int(ti), -1), BR - 1);
+                            const float w = ti - (float)kf;
+                            const float yy = (i + r <= i1) ? yv[r] : 0.0f;
+                            u[kf] = fmaf(1.0f - w, yy, u[kf]);
+                            u[kf + 1] = fmaf(w, yy, u[kf + 1]);
                         }
-                        u[kz] = sum;
+                        off += 4u * row_bytes;
                     }
                 } else {
-                    sh.q2[buf][s][jj] = 1e30f;
-#pragma unroll
-                    for (int kz = 0; kz < BR; ++kz) u[kz] = 0.0f;
+                    sh.q2[s][jj] = 1e30f;
                 }
             }
             __syncthreads();
@@ -297,9 +313,9 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
                 const float kc = (float)(mx ? ty : tx);
 #pragma unroll 1
                 for (int jj = 0; jj < BR; ++jj) {
-                    const float w = 1.0f - fabsf(sh.q2[buf][s][jj] - kc);
+                    const float w = 1.0f - fabsf(sh.q2[s][jj] - kc);
                     if (w > 0.0f) {
-                        const float4 *u4 = reinterpret_cast<const float4 *>(sh.u[buf][s][jj]);
+                        const float4 *u4 = reinterpret_cast<const float4 *>(&sh.u[s][jj][UOFF]);
 #pragma unroll
                         for (int q = 0; q < BR / 4; ++q) {
                             const float4 v = u4[q];
@@ -311,9 +327,8 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
                     }
                 }
             }
-            buf ^= 1;
+            __syncthreads();  // u is rewritten by the next chunk or angle
         }
-        __syncthreads();  // slice ranges of the next angle overwrite jlo/cnt
     }
     if (x < nx && y < ny) {
         float *out = vol + (int64_t)b * vol_stride + (int64_t)y * nx + x;

@@ -73,45 +73,60 @@ __device__ __forceinline__ bool ray_setup(const AngleParams &A, double fj, doubl
     return true;
 }
 
-__device__ float fp_ray_general(const AngleParams &A, const float *__restrict__ vol, double fj,
-                                double fi)
+// General forward projector, templated on the data type (float for the
+// projector, double for exact gradient checks). Ray setup is FP64 in both.
+template <typename T>
+__device__ T fp_ray_general(const AngleParams &A, const T *__restrict__ vol, double fj, double fi)
 {
+    const double Pm = A.P0[0] + fj * A.U[0] + fi * A.V[0];
+    const double P1 = A.P0[1] + fj * A.U[1] + fi * A.V[1];
+    const double P2 = A.P0[2] + fj * A.U[2] + fi * A.V[2];
+    const double Dm = A.cone ? Pm - A.S[0] : A.S[0];
+    if (Dm == 0.0) return T(0);
     RaySetup r;
-    if (!ray_setup(A, fj, fi, 0.0, r)) return 0.0f;
+    if (!ray_setup(A, fj, fi, 0.0, r)) return T(0);
+    const double inv = 1.0 / Dm;
+    const T a1 = (T)(((A.cone ? P1 - A.S[1] : A.S[1])) * inv);
+    const T a2 = (T)(((A.cone ? P2 - A.S[2] : A.S[2])) * inv);
+    const T R1 = (T)(P1 + ((double)r.nref - Pm) * (double)a1);
+    const T R2 = (T)(P2 + ((double)r.nref - Pm) * (double)a2);
+    const T len = (T)sqrt(A.h[0] * A.h[0] + A.h[1] * A.h[1] * (double)a1 * (double)a1 +
+                          A.h[2] * A.h[2] * (double)a2 * (double)a2);
     const int n1 = A.n[1], n2 = A.n[2];
     const int64_t sm = A.st[0], s1 = A.st[1], s2 = A.st[2];
-    float total = 0.0f, part = 0.0f;
+    T total = T(0), part = T(0);
     int cnt = 0;
     for (int n = r.nlo; n <= r.nhi; ++n) {
-        const float dn = (float)(n - r.nref);
-        const float q1 = fmaf(dn, r.a1, r.R1);
-        const float q2 = fmaf(dn, r.a2, r.R2);
-        const float f1 = floorf(q1), f2 = floorf(q2);
+        const T dn = (T)(n - r.nref);
+        const T q1 = dn * a1 + R1;
+        const T q2 = dn * a2 + R2;
+        const T f1 = floor(q1), f2 = floor(q2);
         const int k1 = (int)f1, k2 = (int)f2;
-        const float w1 = q1 - f1, w2 = q2 - f2;
+        const T w1 = q1 - f1, w2 = q2 - f2;
         const bool a0 = (unsigned)k1 < (unsigned)n1, a1ok = (unsigned)(k1 + 1) < (unsigned)n1;
         const bool b0 = (unsigned)k2 < (unsigned)n2, b1ok = (unsigned)(k2 + 1) < (unsigned)n2;
         const int64_t base = (int64_t)n * sm + (int64_t)k1 * s1 + (int64_t)k2 * s2;
-        const float v00 = (a0 && b0) ? vol[base] : 0.0f;
-        const float v10 = (a1ok && b0) ? vol[base + s1] : 0.0f;
-        const float v01 = (a0 && b1ok) ? vol[base + s2] : 0.0f;
-        const float v11 = (a1ok && b1ok) ? vol[base + s1 + s2] : 0.0f;
-        const float lo2 = fmaf(w1, v10 - v00, v00);
-        const float hi2 = fmaf(w1, v11 - v01, v01);
-        part += fmaf(w2, hi2 - lo2, lo2);
+        const T v00 = (a0 && b0) ? vol[base] : T(0);
+        const T v10 = (a1ok && b0) ? vol[base + s1] : T(0);
+        const T v01 = (a0 && b1ok) ? vol[base + s2] : T(0);
+        const T v11 = (a1ok && b1ok) ? vol[base + s1 + s2] : T(0);
+        const T lo2 = w1 * (v10 - v00) + v00;
+        const T hi2 = w1 * (v11 - v01) + v01;
+        part += w2 * (hi2 - lo2) + lo2;
         if (++cnt == 16) {
             total += part;
-            part = 0.0f;
+            part = T(0);
             cnt = 0;
         }
     }
-    return r.len * (total + part);
+    return len * (total + part);
 }
 
+template <typename T>
 __global__ void __launch_bounds__(FP_BX *FP_BY)
 fp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
-           int tiles_u, const float *__restrict__ vol, int64_t vol_stride,
-           float *__restrict__ proj, int64_t proj_stride, int additive)
+           int tiles_u, const T *__restrict__ vol, int64_t vol_stride, T *__restrict__ proj,
+           int64_t proj_stride, int additive)
 {
     const int tile = blockIdx.x;
     const int j = (tile % tiles_u) * FP_BX + threadIdx.x;
@@ -120,14 +135,14 @@ fp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
     const int b = blockIdx.z;
     if (j >= cols || i >= rows) return;
     const AngleParams &A = angles[a];
-    const float *v = vol + (int64_t)b * vol_stride;
-    float acc = 0.0f;
+    const T *v = vol + (int64_t)b * vol_stride;
+    T acc = T(0);
     const double inv = 1.0 / ss;
     for (int p = 0; p < ss; ++p)
         for (int q = 0; q < ss; ++q)
-            acc += fp_ray_general(A, v, j + (q + 0.5) * inv - 0.5, i + (p + 0.5) * inv - 0.5);
-    acc *= (float)(inv * inv);
-    float *out = proj + (int64_t)b * proj_stride + ((int64_t)i * n_angles + a) * cols + j;
+            acc += fp_ray_general<T>(A, v, j + (q + 0.5) * inv - 0.5, i + (p + 0.5) * inv - 0.5);
+    acc *= (T)(inv * inv);
+    T *out = proj + (int64_t)b * proj_stride + ((int64_t)i * n_angles + a) * cols + j;
     *out = additive ? *out + acc : acc;
 }
 
@@ -228,7 +243,68 @@ __device__ __forceinline__ float march_padded(__amdgpu_buffer_rsrc_t rs, const R
     return r.len * total;
 }
 
-template <int SS>
+__device__ __forceinline__ int floor_to_int(float q)
+{
+    int k;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(k) : "v"(q));
+    return k;
+}
+
+// Same sampling as march_padded, software-pipelined and unrolled by two with
+// two register sets: the corners of slice n + 1 are requested before slice n is
+// interpolated, so two samples' loads are in flight per lane. The request past
+// the last slice of the ray reads at most one slice beyond its range, which is
+// inside the layout or returns 0 through the buffer range check, and is not
+// used.
+struct Corners {
+    float v00, v01, v10, v11, w1, w2;
+};
+
+__device__ __forceinline__ void request(__amdgpu_buffer_rsrc_t rs, const RaySetup &r, float dn,
+                                        uint32_t rowoff, uint32_t rowb, Corners &c)
+{
+    const float q1 = fmaf(dn, r.a1, r.R1), q2 = fmaf(dn, r.a2, r.R2);
+    const uint32_t off = rowoff + __umul24((uint32_t)floor_to_int(q1), rowb) +
+                         ((uint32_t)floor_to_int(q2) << 2);
+    c.v00 = bload(rs, off);
+    c.v01 = bload(rs, off + 4u);
+    c.v10 = bload(rs, off + rowb);
+    c.v11 = bload(rs, off + rowb + 4u);
+    c.w1 = __builtin_amdgcn_fractf(q1);
+    c.w2 = __builtin_amdgcn_fractf(q2);
+}
+
+__device__ __forceinline__ float interp(const Corners &c)
+{
+    const float lo = fmaf(c.w2, c.v01 - c.v00, c.v00);
+    const float hi = fmaf(c.w2, c.v11 - c.v10, c.v10);
+    return fmaf(c.w1, hi - lo, lo);
+}
+
+__device__ __forceinline__ float march_padded_pipe(__amdgpu_buffer_rsrc_t rs, const RaySetup &r,
+                                                   uint32_t st_m, uint32_t st_1)
+{
+    const uint32_t rowb = st_1 * 4u, stepb = st_m * 4u;
+    float dn = (float)(r.nlo - r.nref);
+    uint32_t rowoff = (uint32_t)r.nlo * stepb;
+    Corners A, B;
+    request(rs, r, dn, rowoff, rowb, A);
+    float sa = 0.0f, sb = 0.0f;
+    const int cnt = r.nhi - r.nlo + 1;
+    const int pairs = (cnt + 1) >> 1;
+    for (int p = 0; p < pairs; ++p) {
+        request(rs, r, dn + 1.0f, rowoff + stepb, rowb, B);
+        sa += interp(A);
+        request(rs, r, dn + 2.0f, rowoff + 2u * stepb, rowb, A);
+        const float vb = interp(B);
+        sb += (2 * p + 1 < cnt) ? vb : 0.0f;
+        dn += 2.0f;
+        rowoff += 2u * stepb;
+    }
+    return r.len * (sa + sb);
+}
+
+template <int SS, int PIPE>
 __global__ void __launch_bounds__(FP_BX *FP_BY)
 fp_padded(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int tiles_u,
           int tiles_v, int n_blocks, Layouts L, int batch, float *__restrict__ proj,
@@ -262,7 +338,8 @@ fp_padded(const AngleParams *__restrict__ angles, int n_angles, int rows, int co
             RaySetup r;
             const double fj = SS == 1 ? (double)j : j + (q + 0.5) / SS - 0.5;
             const double fi = SS == 1 ? (double)i : i + (p + 0.5) / SS - 0.5;
-            if (ray_setup(A, fj, fi, (double)PAD, r)) acc += march_padded(rs, r, st_m, st_1);
+            if (ray_setup(A, fj, fi, (double)PAD, r))
+                acc += PIPE ? march_padded_pipe(rs, r, st_m, st_1) : march_padded(rs, r, st_m, st_1);
         }
     }
     if (SS > 1) acc *= 1.0f / (float)(SS * SS);

@@ -216,6 +216,10 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
     // separable backprojection: upright detector (v along z), stepping along x or y
     p->bp_separable = g->det_supersampling == 1;
     if (!p->bp_separable) p->bp_reason = "detector supersampling";
+    if ((int64_t)g->det_row_count * g->n_angles * g->det_col_count * 4 >= ((int64_t)1 << 32)) {
+        p->bp_separable = false;
+        p->bp_reason = "projection data of 4 GiB or more";
+    }
     for (int a = 0; a < g->n_angles && p->bp_separable; ++a) {
         const AngleParams &A = p->host[a];
         const double vn = std::sqrt(A.V[0] * A.V[0] + A.V[1] * A.V[1] + A.V[2] * A.V[2]);
@@ -277,6 +281,14 @@ struct DeviceGuard {
     }
 };
 
+// Kernel variant of the padded forward projector: 1 = software-pipelined
+// (default), 0 = plain loop. TSX_FP_VARIANT overrides.
+int fp_variant()
+{
+    const char *e = std::getenv("TSX_FP_VARIANT");
+    return e ? std::atoi(e) : 1;
+}
+
 bool force_general()
 {
     const char *e = std::getenv("TSX_FORCE_GENERAL");
@@ -328,7 +340,7 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
         if (p->n_angles > 65535 || batch > 65535)
             return fail(TSX_ERR_UNSUPPORTED, "more than 65535 angles or batch elements per launch");
         dim3 grid((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
-        hipLaunchKernelGGL(fp_general, grid, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
+        hipLaunchKernelGGL(fp_general<float>, grid, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
                            p->rows, p->cols, p->ss, tiles_u, vol, vs, proj, ps, additive);
         TSX_HIP_CHECK(hipGetLastError());
         return TSX_OK;
@@ -379,12 +391,18 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     if (nb > (int64_t)0x7fffffff - 8) return fail(TSX_ERR_UNSUPPORTED, "forward projection too large for one launch");
     const int n_blocks = (int)nb;
     const unsigned grid = (unsigned)(((n_blocks + 7) / 8) * 8);
+    const bool pipe = fp_variant() == 1;
     switch (p->ss) {
 #define TSX_FP_CASE(K)                                                                             \
     case K:                                                                                        \
-        hipLaunchKernelGGL(fp_padded<K>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, p->dev,        \
-                           p->n_angles, p->rows, p->cols, tiles_u, tiles_v, n_blocks, L,           \
-                           (int)batch, proj, ps, additive);                                        \
+        if (pipe)                                                                                  \
+            hipLaunchKernelGGL((fp_padded<K, 1>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream,      \
+                               p->dev, p->n_angles, p->rows, p->cols, tiles_u, tiles_v, n_blocks,  \
+                               L, (int)batch, proj, ps, additive);                                 \
+        else                                                                                       \
+            hipLaunchKernelGGL((fp_padded<K, 0>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream,      \
+                               p->dev, p->n_angles, p->rows, p->cols, tiles_u, tiles_v, n_blocks,  \
+                               L, (int)batch, proj, ps, additive);                                 \
         break;
         TSX_FP_CASE(1)
         TSX_FP_CASE(2)
@@ -393,7 +411,7 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
 #undef TSX_FP_CASE
     default: {
         dim3 g2((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
-        hipLaunchKernelGGL(fp_general, g2, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
+        hipLaunchKernelGGL(fp_general<float>, g2, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
                            p->rows, p->cols, p->ss, tiles_u, vol, vs, proj, ps, additive);
     }
     }
@@ -410,12 +428,14 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
             return fail(TSX_ERR_UNSUPPORTED, "volume depth times batch exceeds 65535 per launch");
         dim3 grid((unsigned)((p->nx + BP_BX - 1) / BP_BX), (unsigned)((p->ny + BP_BY - 1) / BP_BY),
                   (unsigned)(p->nz * batch));
-        hipLaunchKernelGGL(bp_general, grid, dim3(BP_BX, BP_BY), 0, stream, p->dev, p->n_angles,
+        hipLaunchKernelGGL(bp_general<float>, grid, dim3(BP_BX, BP_BY), 0, stream, p->dev, p->n_angles,
                            p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, proj, ps, additive);
         TSX_HIP_CHECK(hipGetLastError());
         return TSX_OK;
     }
     const int64_t sino = (int64_t)p->rows * p->n_angles * p->cols;
+    if (sino * 4 >= ((int64_t)1 << 32))
+        return fail(TSX_ERR_UNSUPPORTED, "projection data of 4 GiB or more per entry");
     if ((int64_t)p->rows * batch > 65535)
         return fail(TSX_ERR_UNSUPPORTED, "detector rows times batch exceeds 65535 per launch");
     keep_pool_memory(p->device);
@@ -435,7 +455,33 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
     const unsigned grid = (unsigned)(((n_blocks + 7) / 8) * 8);
     hipLaunchKernelGGL(bp_separable, dim3(grid), dim3(BR, BR), 0, stream, p->dev, p->n_angles,
                        p->rows, p->cols, p->nx, p->ny, p->nz, bx, by, bz, n_blocks, vol, vs, ly,
-                       sino, additive);
+                       (uint32_t)(sino * 4), additive);
+    TSX_HIP_CHECK(hipGetLastError());
+    return TSX_OK;
+}
+
+template <typename T>
+int launch_general(const tsx_plan *p, bool forward, T *vol, int64_t vs, T *proj, int64_t ps,
+                   int64_t batch, int additive, hipStream_t stream)
+{
+    if (batch <= 0) return TSX_OK;
+    if (forward) {
+        if (p->n_angles > 65535 || batch > 65535)
+            return fail(TSX_ERR_UNSUPPORTED, "more than 65535 angles or batch elements per launch");
+        const int tiles_u = (p->cols + FP_BX - 1) / FP_BX;
+        const int tiles_v = (p->rows + FP_BY - 1) / FP_BY;
+        dim3 grid((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
+        hipLaunchKernelGGL(fp_general<T>, grid, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
+                           p->rows, p->cols, p->ss, tiles_u, (const T *)vol, vs, proj, ps, additive);
+    } else {
+        if ((int64_t)p->nz * batch > 65535)
+            return fail(TSX_ERR_UNSUPPORTED, "volume depth times batch exceeds 65535 per launch");
+        dim3 grid((unsigned)((p->nx + BP_BX - 1) / BP_BX), (unsigned)((p->ny + BP_BY - 1) / BP_BY),
+                  (unsigned)(p->nz * batch));
+        hipLaunchKernelGGL(bp_general<T>, grid, dim3(BP_BX, BP_BY), 0, stream, p->dev, p->n_angles,
+                           p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, (const T *)proj,
+                           ps, additive);
+    }
     TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
 }
@@ -568,6 +614,28 @@ int tsx_backward_batched(const tsx_plan *p, float *vol, int64_t vs, const float 
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
     DeviceGuard guard(p->device);
     return launch_bp(p, vol, vs, proj, ps, batch, additive, (hipStream_t)stream);
+}
+
+int tsx_forward_batched_f64(const tsx_plan *p, const double *vol, int64_t vs, double *proj,
+                            int64_t ps, int64_t batch, int additive, void *stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+    DeviceGuard guard(p->device);
+    return launch_general<double>(p, true, (double *)vol, vs, proj, ps, batch, additive,
+                                  (hipStream_t)stream);
+}
+
+int tsx_backward_batched_f64(const tsx_plan *p, double *vol, int64_t vs, const double *proj,
+                             int64_t ps, int64_t batch, int additive, void *stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+    DeviceGuard guard(p->device);
+    return launch_general<double>(p, false, vol, vs, (double *)proj, ps, batch, additive,
+                                  (hipStream_t)stream);
 }
 
 int tsx_forward(const tsx_plan *p, const float *vol, float *proj, int additive, void *stream)

@@ -85,11 +85,18 @@ def _require_device(torch):
 
 
 def _run(projector, vol_t, proj_t, forward, additive, batch=1, vol_stride=0, proj_stride=0):
-    """Launch on device tensors (float32, contiguous, same device)."""
+    """Launch on device tensors (contiguous, same device; float32, or float64 for the
+    FP64 kernels)."""
     torch = _torch()
     dev = vol_t.device.index if vol_t.device.index is not None else torch.cuda.current_device()
     plan = projector.plan(dev)
     stream = torch.cuda.current_stream(dev).cuda_stream
+    if vol_t.dtype == torch.float64 and proj_t.dtype == torch.float64:
+        fn = plan.forward_f64 if forward else plan.backward_f64
+        fn(vol_t.data_ptr(), proj_t.data_ptr(), additive, stream, batch, vol_stride, proj_stride)
+        return
+    if vol_t.dtype != torch.float32 or proj_t.dtype != torch.float32:
+        raise TypeError("projection kernels take float32 (or float64 on both sides)")
     if forward:
         plan.forward(vol_t.data_ptr(), proj_t.data_ptr(), additive, stream, batch, vol_stride,
                      proj_stride)

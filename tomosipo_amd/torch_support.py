@@ -33,7 +33,7 @@ def _base_and_dir(operator):
 
 
 def _apply(operator, inp, out, n_extra):
-    """out[...] = operator(inp[...]) over the leading ``n_extra`` dims."""
+    """out[...] = operator(inp[...]) over the leading ``n_extra`` dims, in the dtype of ``out``."""
     base, forward = _base_and_dir(operator)
     extra = tuple(inp.shape[:n_extra])
     batch = 1
@@ -41,8 +41,11 @@ def _apply(operator, inp, out, n_extra):
         batch *= int(d)
     if batch == 0:
         return
+    if out.dtype == torch.float64:
+        if not (inp.is_cuda and out.is_cuda):
+            raise RuntimeError("the float64 projector runs on a HIP device only")
     if inp.is_cuda and out.is_cuda and inp.device == out.device and out.is_contiguous():
-        x = inp.detach().to(torch.float32).contiguous()
+        x = inp.detach().to(out.dtype).contiguous()
         in_numel = x[(0,) * n_extra].numel() if n_extra else x.numel()
         out_numel = out[(0,) * n_extra].numel() if n_extra else out.numel()
         with torch.cuda.device_of(x):
@@ -62,18 +65,19 @@ def _apply(operator, inp, out, n_extra):
 
 class OperatorFunction(Function):
     @staticmethod
-    def forward(ctx, input, operator, num_extra_dims=0, is_2d=False):
+    def forward(ctx, input, operator, num_extra_dims=0, is_2d=False, dtype=torch.float32):
         extra_dims = input.size()[:num_extra_dims]
         if input.requires_grad:
             ctx.operator = operator
             ctx.extra_dims = extra_dims
             ctx.is_2d = is_2d
+            ctx.dtype = dtype
         expected_ndim = (2 if is_2d else 3) + num_extra_dims
         assert input.ndim == expected_ndim, (
             f"Tomosipo autograd operator expected {expected_ndim} dimensions but got {input.ndim}.\n"
             "To add batch and channel dimensions set argument num_extra_dims=2\n"
             "To remove the first operator dimension set argument is_2d=True\n")
-        output = input.new_empty(tuple(extra_dims) + tuple(operator.range_shape), dtype=torch.float32)
+        output = input.new_empty(tuple(extra_dims) + tuple(operator.range_shape), dtype=dtype)
         if is_2d:
             input = torch.unsqueeze(input, dim=-3)
         _apply(operator, input, output, num_extra_dims)
@@ -87,18 +91,29 @@ class OperatorFunction(Function):
         extra_dims = ctx.extra_dims
         is_2d = ctx.is_2d
         grad_input = grad_output.new_empty(tuple(extra_dims) + tuple(operator.domain_shape),
-                                           dtype=torch.float32)
+                                           dtype=ctx.dtype)
         if is_2d:
             grad_output = torch.unsqueeze(grad_output, dim=-3)
         _apply(operator.T, grad_output, grad_input, len(extra_dims))
         if is_2d:
             grad_input = torch.squeeze(grad_input, dim=-3)
-        return grad_input, None, None, None
+        return grad_input, None, None, None, None
 
 
-def to_autograd(operator, num_extra_dims=0, is_2d=False):
+def to_autograd(operator, num_extra_dims=0, is_2d=False, dtype=None):
+    """Autograd function of ``operator`` (forward ``A``, backward ``A.T``).
+
+    ``dtype`` defaults to float32, as in tomosipo (outputs are always float32).
+    ``dtype=torch.float64`` selects the FP64 kernels on a HIP device; forward and
+    backward are then exact transposes to FP64 rounding, which finite-difference
+    gradient checks need. This option has no counterpart in the reference.
+    """
+    dtype = torch.float32 if dtype is None else dtype
+    if dtype not in (torch.float32, torch.float64):
+        raise ValueError(f"dtype must be torch.float32 or torch.float64, got {dtype}")
+
     def f(x):
-        return OperatorFunction.apply(x, operator, num_extra_dims, is_2d)
+        return OperatorFunction.apply(x, operator, num_extra_dims, is_2d, dtype)
 
     return f
 
