@@ -117,13 +117,38 @@ int tsx_forward(const tsx_plan *plan, const float *vol, float *proj, int additiv
 int tsx_backward(const tsx_plan *plan, float *vol, const float *proj, int additive,
                  void *hip_stream);
 
+/* Device scratch bytes that one launch of `op` over `batch` entries needs
+ * (padded volume copies for FP, length-weighted projections for BP; 0 when
+ * the general kernels run). */
+int tsx_workspace_size(const tsx_plan *plan, int op, int64_t batch, int64_t *bytes);
+
 /* Batched variants: `batch` independent volumes / projection sets with the
  * given element strides, in one launch. Replaces the per-slice loop of
- * tomosipo's autograd wrapper (torch_support.py:49-53, 70-74). */
+ * tomosipo's autograd wrapper (torch_support.py:49-53, 70-74). `workspace`
+ * (device memory of at least tsx_workspace_size bytes) may be NULL, in which
+ * case the library takes stream-ordered memory for the call; passing it keeps
+ * all allocation with the caller, e.g. under HIP graph capture. */
 int tsx_forward_batched(const tsx_plan *plan, const float *vol, int64_t vol_stride, float *proj,
-                        int64_t proj_stride, int64_t batch, int additive, void *hip_stream);
+                        int64_t proj_stride, int64_t batch, int additive, void *workspace,
+                        int64_t workspace_bytes, void *hip_stream);
 int tsx_backward_batched(const tsx_plan *plan, float *vol, int64_t vol_stride, const float *proj,
-                         int64_t proj_stride, int64_t batch, int additive, void *hip_stream);
+                         int64_t proj_stride, int64_t batch, int additive, void *workspace,
+                         int64_t workspace_bytes, void *hip_stream);
+
+/* The two halves of one SIRT iteration x += C * A^T(R * (y - A x)) with the
+ * elementwise work fused into the projection kernels. They replace the
+ * projector calls plus elementwise operations of tomosipo's SIRT loops
+ * (README.md:150-164, doc/intro/fast_reconstruction.rst:92-104,
+ * notebooks/sirt_benchmark.py:131-136).
+ *   tsx_forward_residual:    proj = w * (y - A vol)      (w = R, same layout as proj)
+ *   tsx_backward_scaled_add: vol += w * (A^T proj)       (w = C, same layout as vol)
+ * y and w use the batch stride of the output. */
+int tsx_forward_residual(const tsx_plan *plan, const float *vol, int64_t vol_stride, const float *y,
+                         const float *w, float *proj, int64_t proj_stride, int64_t batch,
+                         void *workspace, int64_t workspace_bytes, void *hip_stream);
+int tsx_backward_scaled_add(const tsx_plan *plan, float *vol, int64_t vol_stride,
+                            const float *proj, int64_t proj_stride, const float *w, int64_t batch,
+                            void *workspace, int64_t workspace_bytes, void *hip_stream);
 
 /* FP64 forward and backprojection (general kernels, any geometry). No
  * counterpart in the reference: ASTRA projects float32 only. For callers that

@@ -35,8 +35,10 @@ _STATUS = {
 
 EXPORTED = (
     "tsx_plan_create", "tsx_plan_destroy", "tsx_plan_info_get", "tsx_plan_axes",
+    "tsx_workspace_size",
     "tsx_direct_fpbp3d", "tsx_forward", "tsx_backward", "tsx_forward_batched",
-    "tsx_backward_batched", "tsx_forward_batched_f64", "tsx_backward_batched_f64",
+    "tsx_backward_batched", "tsx_forward_residual", "tsx_backward_scaled_add",
+    "tsx_forward_batched_f64", "tsx_backward_batched_f64",
     "tsx_last_error", "tsx_version",
 )
 
@@ -104,8 +106,11 @@ def load():
         lib.tsx_direct_fpbp3d.argtypes = [P, P, P, i32, i32, P]
         lib.tsx_forward.argtypes = [P, P, P, i32, P]
         lib.tsx_backward.argtypes = [P, P, P, i32, P]
-        lib.tsx_forward_batched.argtypes = [P, P, i64, P, i64, i64, i32, P]
-        lib.tsx_backward_batched.argtypes = [P, P, i64, P, i64, i64, i32, P]
+        lib.tsx_workspace_size.argtypes = [P, i32, i64, ctypes.POINTER(i64)]
+        lib.tsx_forward_batched.argtypes = [P, P, i64, P, i64, i64, i32, P, i64, P]
+        lib.tsx_backward_batched.argtypes = [P, P, i64, P, i64, i64, i32, P, i64, P]
+        lib.tsx_forward_residual.argtypes = [P, P, i64, P, P, P, i64, i64, P, i64, P]
+        lib.tsx_backward_scaled_add.argtypes = [P, P, i64, P, i64, P, i64, P, i64, P]
         lib.tsx_forward_batched_f64.argtypes = [P, P, i64, P, i64, i64, i32, P]
         lib.tsx_backward_batched_f64.argtypes = [P, P, i64, P, i64, i64, i32, P]
         for name in EXPORTED:
@@ -194,13 +199,34 @@ class Plan:
         _check(self._lib.tsx_plan_axes(self._h, out.ctypes.data))
         return out
 
-    def forward(self, vol_ptr, proj_ptr, additive, stream, batch=1, vol_stride=0, proj_stride=0):
-        _check(self._lib.tsx_forward_batched(self._h, vol_ptr, vol_stride, proj_ptr, proj_stride,
-                                             batch, int(bool(additive)), stream))
+    def workspace_size(self, op, batch=1):
+        out = ctypes.c_int64()
+        _check(self._lib.tsx_workspace_size(self._h, int(op), int(batch), ctypes.byref(out)))
+        return int(out.value)
 
-    def backward(self, vol_ptr, proj_ptr, additive, stream, batch=1, vol_stride=0, proj_stride=0):
+    def forward(self, vol_ptr, proj_ptr, additive, stream, batch=1, vol_stride=0, proj_stride=0,
+                workspace=None, workspace_bytes=0):
+        _check(self._lib.tsx_forward_batched(self._h, vol_ptr, vol_stride, proj_ptr, proj_stride,
+                                             batch, int(bool(additive)), workspace,
+                                             int(workspace_bytes), stream))
+
+    def backward(self, vol_ptr, proj_ptr, additive, stream, batch=1, vol_stride=0, proj_stride=0,
+                 workspace=None, workspace_bytes=0):
         _check(self._lib.tsx_backward_batched(self._h, vol_ptr, vol_stride, proj_ptr, proj_stride,
-                                              batch, int(bool(additive)), stream))
+                                              batch, int(bool(additive)), workspace,
+                                              int(workspace_bytes), stream))
+
+    def forward_residual(self, vol_ptr, y_ptr, w_ptr, proj_ptr, stream, batch=1, vol_stride=0,
+                         proj_stride=0, workspace=None, workspace_bytes=0):
+        _check(self._lib.tsx_forward_residual(self._h, vol_ptr, vol_stride, y_ptr, w_ptr, proj_ptr,
+                                              proj_stride, batch, workspace, int(workspace_bytes),
+                                              stream))
+
+    def backward_scaled_add(self, vol_ptr, proj_ptr, w_ptr, stream, batch=1, vol_stride=0,
+                            proj_stride=0, workspace=None, workspace_bytes=0):
+        _check(self._lib.tsx_backward_scaled_add(self._h, vol_ptr, vol_stride, proj_ptr,
+                                                 proj_stride, w_ptr, batch, workspace,
+                                                 int(workspace_bytes), stream))
 
     def forward_f64(self, vol_ptr, proj_ptr, additive, stream, batch=1, vol_stride=0, proj_stride=0):
         _check(self._lib.tsx_forward_batched_f64(self._h, vol_ptr, vol_stride, proj_ptr,

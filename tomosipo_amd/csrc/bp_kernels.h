@@ -39,7 +39,7 @@ template <typename T>
 __global__ void __launch_bounds__(BP_BX *BP_BY)
 bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
            int nx, int ny, int nz, T *__restrict__ vol, int64_t vol_stride,
-           const T *__restrict__ proj, int64_t proj_stride, int additive)
+           const T *__restrict__ proj, int64_t proj_stride, Epilogue ep)
 {
     const int x = blockIdx.x * BP_BX + threadIdx.x;
     const int y = blockIdx.y * BP_BY + threadIdx.y;
@@ -150,8 +150,7 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
         }
         acc += acc_a * (tinv * tinv);
     }
-    T *out = vol + (int64_t)b * vol_stride + ((int64_t)z * ny + y) * nx + x;
-    *out = additive ? *out + acc : acc;
+    emit(vol + (int64_t)b * vol_stride, ((int64_t)z * ny + y) * nx + x, b, acc, ep);
 }
 
 // ly[i, a, j] = l(a, i, j) * y[i, a, j] (detector supersampling 1).
@@ -180,12 +179,14 @@ __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_an
     out[(int64_t)b * out_stride + o] = len * proj[(int64_t)b * proj_stride + o];
 }
 
-// Column j of slice s keeps u at UCOL-float stride; kz = -1 .. 16 live at
-// UOFF - 1 .. UOFF + 16 so that the splat needs no range test and the 16
-// voxel entries are 16-byte aligned.
-constexpr int UOFF = 4, UCOL = 24;
+// Column j of slice s keeps u at (BZ + 8)-float stride; kz = -1 .. BZ live at
+// UOFF - 1 .. UOFF + BZ so that the splat needs no range test beyond a clamp
+// and the BZ voxel entries are 16-byte aligned.
+constexpr int UOFF = 4;
 
+template <int BZ>
 struct BpShared {
+    static constexpr int UCOL = BZ + 8;
     float u[BR][BR][UCOL];  // [slice s][column jj][UOFF + kz]
     float q2[BR][BR];       // Q_c2 - c0 of column jj in slice s (1e30 if unused)
     int jlo[BR];
@@ -205,13 +206,15 @@ __device__ __forceinline__ double column_of(const AngleParams &A, double n, doub
     return (n + t0 * A.S[0] - A.P0[0]) * A.Bj[0] + (c + t0 * A.S[2] - A.P0[2]) * A.Bj[2];
 }
 
+// Brick of BR x BR columns (m, c2 plane) by BZ voxels along z.
+template <int BZ>
 __global__ void __launch_bounds__(BR *BR)
 bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int nx,
              int ny, int nz, int bricks_x, int bricks_y, int bricks_z, int n_blocks,
              float *__restrict__ vol, int64_t vol_stride, const float *__restrict__ ly,
-             uint32_t ly_bytes, int additive)
+             uint32_t ly_bytes, Epilogue ep)
 {
-    __shared__ BpShared sh;
+    __shared__ BpShared<BZ> sh;
     const int per = (n_blocks + 7) / 8;
     const int logical = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
     if (logical >= n_blocks) return;
@@ -220,13 +223,13 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
     const int bz = (logical / (bricks_x * bricks_y)) % bricks_z;
     const int b = logical / (bricks_x * bricks_y * bricks_z);
     const int tx = threadIdx.x, ty = threadIdx.y, t = ty * BR + tx;
-    const int x = bx * BR + tx, y = by * BR + ty, z0 = bz * BR;
+    const int x = bx * BR + tx, y = by * BR + ty, z0 = bz * BZ;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(ly + (int64_t)b * ((int64_t)ly_bytes / 4)), (short)0, (int)ly_bytes, 0x00020000);
     const uint32_t row_bytes = (uint32_t)n_angles * (uint32_t)cols * 4u;
-    float acc[BR];
+    float acc[BZ];
 #pragma unroll
-    for (int k = 0; k < BR; ++k) acc[k] = 0.0f;
+    for (int k = 0; k < BZ; ++k) acc[k] = 0.0f;
     for (int a = 0; a < n_angles; ++a) {
         const AngleParams &A = angles[a];
         const bool mx = A.m == 0;
@@ -254,8 +257,9 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
                 const int s = ty, jj = tx;
                 const int j = sh.jlo[s] + c * BR + jj;
                 float *u = &sh.u[s][jj][UOFF];
+                float4 *u4 = reinterpret_cast<float4 *>(&sh.u[s][jj][0]);
 #pragma unroll
-                for (int k = -UOFF; k < UCOL - UOFF; ++k) u[k] = 0.0f;
+                for (int k = 0; k < BpShared<BZ>::UCOL / 4; ++k) u4[k] = make_float4(0.f, 0.f, 0.f, 0.f);
                 if (c * BR + jj < sh.cnt[s]) {
                     const double n = (double)(m0 + s);
                     const double Pm = A.P0[0] + j * A.U[0];
@@ -274,9 +278,10 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
                         Q2 = P2 + tau * A.S[2];
                     }
                     sh.q2[s][jj] = (float)(Q2 - (double)c0);
-                    // rows with t_i = Arel + i * Bz in (-1, BR)
+                    // rows with t_i = Arel + i * Bz in (-1, BZ)
                     const double Arel = Az - (double)z0;
-                    const double ra = (-1.0 - Arel) / Bz, rb = ((double)BR - Arel) / Bz;
+                    const double iBz = 1.0 / Bz;
+                    const double ra = (-1.0 - Arel) * iBz, rb = ((double)BZ - Arel) * iBz;
                     const int i0 = max(0, (int)ceil(fmax(fmin(ra, rb), -1e9)));
                     const int i1 = min(rows - 1, (int)floor(fmin(fmax(ra, rb), 1e9)));
                     const float t0 = (float)(Arel + (double)i0 * Bz);
@@ -285,6 +290,7 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
                     // four rows per step: their loads are in flight together; rows past
                     // i1 are read (inside the buffer or zero by the range check) and
                     // given weight zero
+                    float tb = t0;
                     for (int i = i0; i <= i1; i += 4) {
                         float yv[4];
 #pragma unroll
@@ -293,14 +299,15 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
                                                                   rs, off + (uint32_t)r * row_bytes, 0, 0));
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
-                            const float ti = fmaf((float)(i + r - i0), fB, t0);
-                            const int kf = min(max(floor_to_int(ti), -1), BR - 1);
+                            const float ti = fmaf((float)r, fB, tb);
+                            const int kf = min(max(floor_to_int(ti), -1), BZ - 1);
                             const float w = ti - (float)kf;
                             const float yy = (i + r <= i1) ? yv[r] : 0.0f;
                             u[kf] = fmaf(1.0f - w, yy, u[kf]);
                             u[kf + 1] = fmaf(w, yy, u[kf + 1]);
                         }
                         off += 4u * row_bytes;
+                        tb = fmaf(4.0f, fB, tb);
                     }
                 } else {
                     sh.q2[s][jj] = 1e30f;
@@ -317,7 +324,7 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
                     if (w > 0.0f) {
                         const float4 *u4 = reinterpret_cast<const float4 *>(&sh.u[s][jj][UOFF]);
 #pragma unroll
-                        for (int q = 0; q < BR / 4; ++q) {
+                        for (int q = 0; q < BZ / 4; ++q) {
                             const float4 v = u4[q];
                             acc[4 * q + 0] = fmaf(w, v.x, acc[4 * q + 0]);
                             acc[4 * q + 1] = fmaf(w, v.y, acc[4 * q + 1]);
@@ -331,14 +338,11 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
         }
     }
     if (x < nx && y < ny) {
-        float *out = vol + (int64_t)b * vol_stride + (int64_t)y * nx + x;
+        float *out = vol + (int64_t)b * vol_stride;
 #pragma unroll
-        for (int k = 0; k < BR; ++k) {
+        for (int k = 0; k < BZ; ++k) {
             const int z = z0 + k;
-            if (z < nz) {
-                float *o = out + (int64_t)z * ny * nx;
-                *o = additive ? *o + acc[k] : acc[k];
-            }
+            if (z < nz) emit(out, ((int64_t)z * ny + y) * nx + x, b, acc[k], ep);
         }
     }
 }

@@ -17,6 +17,21 @@ namespace tsx {
 constexpr int FP_BX = 64, FP_BY = 4;
 constexpr int PAD = 2;  // zero border on each side of c1 and c2 in padded layouts
 
+template <typename T>
+__device__ __forceinline__ void emit(T *out, int64_t o, int64_t b, T acc, const Epilogue &e)
+{
+    if (e.mode == EP_SET) {
+        out[o] = acc;
+    } else if (e.mode == EP_ADD) {
+        out[o] += acc;
+    } else if (e.mode == EP_RESIDUAL) {
+        const int64_t q = b * e.stride + o;
+        out[o] = (T)(e.w[q] * (e.y[q] - (float)acc));
+    } else {
+        out[o] += (T)(e.w[b * e.stride + o] * (float)acc);
+    }
+}
+
 struct RaySetup {
     float a1, a2;    // slope of Q_c per slice
     float R1, R2;    // Q_c at slice nref (padded coordinates in fp_padded)
@@ -126,7 +141,7 @@ template <typename T>
 __global__ void __launch_bounds__(FP_BX *FP_BY)
 fp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
            int tiles_u, const T *__restrict__ vol, int64_t vol_stride, T *__restrict__ proj,
-           int64_t proj_stride, int additive)
+           int64_t proj_stride, Epilogue ep)
 {
     const int tile = blockIdx.x;
     const int j = (tile % tiles_u) * FP_BX + threadIdx.x;
@@ -142,8 +157,7 @@ fp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
         for (int q = 0; q < ss; ++q)
             acc += fp_ray_general<T>(A, v, j + (q + 0.5) * inv - 0.5, i + (p + 0.5) * inv - 0.5);
     acc *= (T)(inv * inv);
-    T *out = proj + (int64_t)b * proj_stride + ((int64_t)i * n_angles + a) * cols + j;
-    *out = additive ? *out + acc : acc;
+    emit(proj + (int64_t)b * proj_stride, ((int64_t)i * n_angles + a) * cols + j, b, acc, ep);
 }
 
 // ---------------------------------------------------------------------------
@@ -304,11 +318,45 @@ __device__ __forceinline__ float march_padded_pipe(__amdgpu_buffer_rsrc_t rs, co
     return r.len * (sa + sb);
 }
 
+// Four samples in flight per lane: ring of four corner sets, unrolled so
+// that no register copies are needed. Requests run up to three slices past
+// the ray's range (discarded; in-layout or zero by the buffer range check).
+__device__ __forceinline__ float march_padded_pipe4(__amdgpu_buffer_rsrc_t rs, const RaySetup &r,
+                                                    uint32_t st_m, uint32_t st_1)
+{
+    const uint32_t rowb = st_1 * 4u, stepb = st_m * 4u;
+    float dn = (float)(r.nlo - r.nref);
+    uint32_t rowoff = (uint32_t)r.nlo * stepb;
+    Corners A0, A1, A2, A3;
+    request(rs, r, dn, rowoff, rowb, A0);
+    request(rs, r, dn + 1.0f, rowoff + stepb, rowb, A1);
+    request(rs, r, dn + 2.0f, rowoff + 2u * stepb, rowb, A2);
+    request(rs, r, dn + 3.0f, rowoff + 3u * stepb, rowb, A3);
+    float s0 = 0.0f, s1 = 0.0f;
+    const int cnt = r.nhi - r.nlo + 1;
+    for (int k = 0; k < cnt; k += 4) {
+        s0 += interp(A0);
+        request(rs, r, dn + 4.0f, rowoff + 4u * stepb, rowb, A0);
+        const float v1 = interp(A1);
+        s1 += (k + 1 < cnt) ? v1 : 0.0f;
+        request(rs, r, dn + 5.0f, rowoff + 5u * stepb, rowb, A1);
+        const float v2 = interp(A2);
+        s0 += (k + 2 < cnt) ? v2 : 0.0f;
+        request(rs, r, dn + 6.0f, rowoff + 6u * stepb, rowb, A2);
+        const float v3 = interp(A3);
+        s1 += (k + 3 < cnt) ? v3 : 0.0f;
+        request(rs, r, dn + 7.0f, rowoff + 7u * stepb, rowb, A3);
+        dn += 4.0f;
+        rowoff += 4u * stepb;
+    }
+    return r.len * (s0 + s1);
+}
+
 template <int SS, int PIPE>
 __global__ void __launch_bounds__(FP_BX *FP_BY)
 fp_padded(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int tiles_u,
           int tiles_v, int n_blocks, Layouts L, int batch, float *__restrict__ proj,
-          int64_t proj_stride, int additive)
+          int64_t proj_stride, Epilogue ep)
 {
     // XCD-aware order: consecutive hardware block ids go round-robin to the 8
     // XCDs; give each XCD a contiguous range of logical ids. Logical ids run
@@ -339,12 +387,13 @@ fp_padded(const AngleParams *__restrict__ angles, int n_angles, int rows, int co
             const double fj = SS == 1 ? (double)j : j + (q + 0.5) / SS - 0.5;
             const double fi = SS == 1 ? (double)i : i + (p + 0.5) / SS - 0.5;
             if (ray_setup(A, fj, fi, (double)PAD, r))
-                acc += PIPE ? march_padded_pipe(rs, r, st_m, st_1) : march_padded(rs, r, st_m, st_1);
+                acc += PIPE == 2   ? march_padded_pipe4(rs, r, st_m, st_1)
+                       : PIPE == 1 ? march_padded_pipe(rs, r, st_m, st_1)
+                                   : march_padded(rs, r, st_m, st_1);
         }
     }
     if (SS > 1) acc *= 1.0f / (float)(SS * SS);
-    float *out = proj + (int64_t)b * proj_stride + ((int64_t)i * n_angles + a) * cols + j;
-    *out = additive ? *out + acc : acc;
+    emit(proj + (int64_t)b * proj_stride, ((int64_t)i * n_angles + a) * cols + j, b, acc, ep);
 }
 
 }  // namespace tsx

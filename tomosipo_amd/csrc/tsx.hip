@@ -281,12 +281,21 @@ struct DeviceGuard {
     }
 };
 
-// Kernel variant of the padded forward projector: 1 = software-pipelined
-// (default), 0 = plain loop. TSX_FP_VARIANT overrides.
+// Kernel variant of the padded forward projector: 0 = plain loop,
+// 1 = two samples in flight, 2 = four samples in flight (default).
+// TSX_FP_VARIANT overrides.
 int fp_variant()
 {
     const char *e = std::getenv("TSX_FP_VARIANT");
-    return e ? std::atoi(e) : 1;
+    return e ? std::atoi(e) : 2;
+}
+
+// z-extent of the separable backprojector's bricks: 32 (default) or 16.
+// TSX_BP_BRICK_Z overrides.
+int bp_brick_z()
+{
+    const char *e = std::getenv("TSX_BP_BRICK_Z");
+    return (e && std::atoi(e) == 16) ? 16 : 32;
 }
 
 bool force_general()
@@ -330,8 +339,48 @@ void keep_pool_memory(int device)
     done[device] = true;
 }
 
+// Scratch bytes one launch of the fast forward / back projector needs.
+int64_t fp_workspace_elems(const tsx_plan *p)
+{
+    int64_t per_entry = 0;
+    for (int c = 0; c < 3; ++c)
+        if (p->axis_count[c] > 0) per_entry += p->layout_elems[c];
+    return per_entry;
+}
+
+int64_t bp_workspace_elems(const tsx_plan *p)
+{
+    return (int64_t)p->rows * p->n_angles * p->cols;
+}
+
+int64_t workspace_bytes(const tsx_plan *p, int op, int64_t batch)
+{
+    if (force_general()) return 0;
+    if (op == TSX_OP_FP) return p->fp_padded ? fp_workspace_elems(p) * batch * 4 : 0;
+    return p->bp_separable ? bp_workspace_elems(p) * batch * 4 : 0;
+}
+
+// Either the caller's workspace (checked for size) or stream-ordered memory.
+int get_workspace(const tsx_plan *p, int64_t bytes, void *ws, int64_t ws_bytes, hipStream_t stream,
+                  Scratch &owned, void **out)
+{
+    if (ws) {
+        if (ws_bytes < bytes)
+            return fail(TSX_ERR_INVALID_ARGUMENT, "workspace of " + std::to_string(ws_bytes) +
+                                                      " bytes is smaller than the " +
+                                                      std::to_string(bytes) + " bytes required");
+        *out = ws;
+        return TSX_OK;
+    }
+    keep_pool_memory(p->device);
+    int rc = owned.alloc((size_t)bytes, stream);
+    *out = owned.ptr;
+    return rc;
+}
+
 int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int64_t ps,
-              int64_t batch, int additive, hipStream_t stream)
+              int64_t batch, Epilogue ep, hipStream_t stream, void *wsp = nullptr,
+              int64_t ws_bytes = 0)
 {
     if (batch <= 0) return TSX_OK;
     const int tiles_u = (p->cols + FP_BX - 1) / FP_BX;
@@ -341,7 +390,7 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
             return fail(TSX_ERR_UNSUPPORTED, "more than 65535 angles or batch elements per launch");
         dim3 grid((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
         hipLaunchKernelGGL(fp_general<float>, grid, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
-                           p->rows, p->cols, p->ss, tiles_u, vol, vs, proj, ps, additive);
+                           p->rows, p->cols, p->ss, tiles_u, vol, vs, proj, ps, ep);
         TSX_HIP_CHECK(hipGetLastError());
         return TSX_OK;
     }
@@ -353,11 +402,11 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
             offs[c] = per_entry;
             per_entry += p->layout_elems[c];
         }
-    keep_pool_memory(p->device);
-    Scratch ws;
-    int rc = ws.alloc((size_t)(per_entry * batch) * sizeof(float), stream);
+    Scratch owned;
+    void *wptr = nullptr;
+    int rc = get_workspace(p, per_entry * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
     if (rc) return rc;
-    float *base = (float *)ws.ptr;
+    float *base = (float *)wptr;
     for (int64_t b = 0; b < batch; ++b) {
         const float *src = vol + b * vs;
         float *dst = base + b * per_entry;
@@ -391,28 +440,31 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     if (nb > (int64_t)0x7fffffff - 8) return fail(TSX_ERR_UNSUPPORTED, "forward projection too large for one launch");
     const int n_blocks = (int)nb;
     const unsigned grid = (unsigned)(((n_blocks + 7) / 8) * 8);
-    const bool pipe = fp_variant() == 1;
+    const int variant = fp_variant();
     switch (p->ss) {
+#define TSX_FP_LAUNCH(K, V)                                                                        \
+    hipLaunchKernelGGL((fp_padded<K, V>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream, p->dev,      \
+                       p->n_angles, p->rows, p->cols, tiles_u, tiles_v, n_blocks, L, (int)batch,   \
+                       proj, ps, ep)
 #define TSX_FP_CASE(K)                                                                             \
     case K:                                                                                        \
-        if (pipe)                                                                                  \
-            hipLaunchKernelGGL((fp_padded<K, 1>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream,      \
-                               p->dev, p->n_angles, p->rows, p->cols, tiles_u, tiles_v, n_blocks,  \
-                               L, (int)batch, proj, ps, additive);                                 \
+        if (variant == 2)                                                                          \
+            TSX_FP_LAUNCH(K, 2);                                                                   \
+        else if (variant == 1)                                                                     \
+            TSX_FP_LAUNCH(K, 1);                                                                   \
         else                                                                                       \
-            hipLaunchKernelGGL((fp_padded<K, 0>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream,      \
-                               p->dev, p->n_angles, p->rows, p->cols, tiles_u, tiles_v, n_blocks,  \
-                               L, (int)batch, proj, ps, additive);                                 \
+            TSX_FP_LAUNCH(K, 0);                                                                   \
         break;
         TSX_FP_CASE(1)
         TSX_FP_CASE(2)
         TSX_FP_CASE(3)
         TSX_FP_CASE(4)
 #undef TSX_FP_CASE
+#undef TSX_FP_LAUNCH
     default: {
         dim3 g2((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
         hipLaunchKernelGGL(fp_general<float>, g2, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
-                           p->rows, p->cols, p->ss, tiles_u, vol, vs, proj, ps, additive);
+                           p->rows, p->cols, p->ss, tiles_u, vol, vs, proj, ps, ep);
     }
     }
     TSX_HIP_CHECK(hipGetLastError());
@@ -420,7 +472,8 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
 }
 
 int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int64_t ps,
-              int64_t batch, int additive, hipStream_t stream)
+              int64_t batch, Epilogue ep, hipStream_t stream, void *wsp = nullptr,
+              int64_t ws_bytes = 0)
 {
     if (batch <= 0) return TSX_OK;
     if (!p->bp_separable || force_general()) {
@@ -429,7 +482,7 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
         dim3 grid((unsigned)((p->nx + BP_BX - 1) / BP_BX), (unsigned)((p->ny + BP_BY - 1) / BP_BY),
                   (unsigned)(p->nz * batch));
         hipLaunchKernelGGL(bp_general<float>, grid, dim3(BP_BX, BP_BY), 0, stream, p->dev, p->n_angles,
-                           p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, proj, ps, additive);
+                           p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, proj, ps, ep);
         TSX_HIP_CHECK(hipGetLastError());
         return TSX_OK;
     }
@@ -438,31 +491,47 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
         return fail(TSX_ERR_UNSUPPORTED, "projection data of 4 GiB or more per entry");
     if ((int64_t)p->rows * batch > 65535)
         return fail(TSX_ERR_UNSUPPORTED, "detector rows times batch exceeds 65535 per launch");
-    keep_pool_memory(p->device);
-    Scratch ws;
-    int rc = ws.alloc((size_t)(sino * batch) * sizeof(float), stream);
+    Scratch owned;
+    void *wptr = nullptr;
+    int rc = get_workspace(p, sino * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
     if (rc) return rc;
-    float *ly = (float *)ws.ptr;
+    float *ly = (float *)wptr;
     {
         dim3 grid((unsigned)((p->cols + 255) / 256), (unsigned)p->n_angles, (unsigned)(p->rows * batch));
         hipLaunchKernelGGL(scale_by_length, grid, dim3(256), 0, stream, p->dev, p->n_angles, p->rows,
                            p->cols, proj, ps, ly, sino);
     }
-    const int bx = (p->nx + BR - 1) / BR, by = (p->ny + BR - 1) / BR, bz = (p->nz + BR - 1) / BR;
+    const int BZ = bp_brick_z();
+    const int bx = (p->nx + BR - 1) / BR, by = (p->ny + BR - 1) / BR, bz = (p->nz + BZ - 1) / BZ;
     const int64_t nb = (int64_t)bx * by * bz * batch;
     if (nb > (int64_t)0x7fffffff - 8) return fail(TSX_ERR_UNSUPPORTED, "backprojection too large for one launch");
     const int n_blocks = (int)nb;
     const unsigned grid = (unsigned)(((n_blocks + 7) / 8) * 8);
-    hipLaunchKernelGGL(bp_separable, dim3(grid), dim3(BR, BR), 0, stream, p->dev, p->n_angles,
-                       p->rows, p->cols, p->nx, p->ny, p->nz, bx, by, bz, n_blocks, vol, vs, ly,
-                       (uint32_t)(sino * 4), additive);
+    if (BZ == 32)
+        hipLaunchKernelGGL(bp_separable<32>, dim3(grid), dim3(BR, BR), 0, stream, p->dev,
+                           p->n_angles, p->rows, p->cols, p->nx, p->ny, p->nz, bx, by, bz, n_blocks,
+                           vol, vs, ly, (uint32_t)(sino * 4), ep);
+    else
+        hipLaunchKernelGGL(bp_separable<16>, dim3(grid), dim3(BR, BR), 0, stream, p->dev,
+                           p->n_angles, p->rows, p->cols, p->nx, p->ny, p->nz, bx, by, bz, n_blocks,
+                           vol, vs, ly, (uint32_t)(sino * 4), ep);
     TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
 }
 
+Epilogue plain(int additive)
+{
+    Epilogue e;
+    e.mode = additive ? EP_ADD : EP_SET;
+    e.y = nullptr;
+    e.w = nullptr;
+    e.stride = 0;
+    return e;
+}
+
 template <typename T>
 int launch_general(const tsx_plan *p, bool forward, T *vol, int64_t vs, T *proj, int64_t ps,
-                   int64_t batch, int additive, hipStream_t stream)
+                   int64_t batch, Epilogue ep, hipStream_t stream)
 {
     if (batch <= 0) return TSX_OK;
     if (forward) {
@@ -472,7 +541,7 @@ int launch_general(const tsx_plan *p, bool forward, T *vol, int64_t vs, T *proj,
         const int tiles_v = (p->rows + FP_BY - 1) / FP_BY;
         dim3 grid((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
         hipLaunchKernelGGL(fp_general<T>, grid, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
-                           p->rows, p->cols, p->ss, tiles_u, (const T *)vol, vs, proj, ps, additive);
+                           p->rows, p->cols, p->ss, tiles_u, (const T *)vol, vs, proj, ps, ep);
     } else {
         if ((int64_t)p->nz * batch > 65535)
             return fail(TSX_ERR_UNSUPPORTED, "volume depth times batch exceeds 65535 per launch");
@@ -480,7 +549,7 @@ int launch_general(const tsx_plan *p, bool forward, T *vol, int64_t vs, T *proj,
                   (unsigned)(p->nz * batch));
         hipLaunchKernelGGL(bp_general<T>, grid, dim3(BP_BX, BP_BY), 0, stream, p->dev, p->n_angles,
                            p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, (const T *)proj,
-                           ps, additive);
+                           ps, ep);
     }
     TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
@@ -596,24 +665,37 @@ int tsx_plan_axes(const tsx_plan *p, int32_t *axes)
     return TSX_OK;
 }
 
+int tsx_workspace_size(const tsx_plan *p, int op, int64_t batch, int64_t *bytes)
+{
+    if (!p || !bytes) return fail(TSX_ERR_INVALID_ARGUMENT, "plan and bytes must not be NULL");
+    if (op != TSX_OP_FP && op != TSX_OP_BP)
+        return fail(TSX_ERR_INVALID_ARGUMENT, "op must be TSX_OP_FP (0) or TSX_OP_BP (1)");
+    *bytes = workspace_bytes(p, op, batch < 1 ? 1 : batch);
+    return TSX_OK;
+}
+
 int tsx_forward_batched(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int64_t ps,
-                        int64_t batch, int additive, void *stream)
+                        int64_t batch, int additive, void *workspace, int64_t workspace_bytes,
+                        void *stream)
 {
     int rc = check_plan(p);
     if (rc) return rc;
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
     DeviceGuard guard(p->device);
-    return launch_fp(p, vol, vs, proj, ps, batch, additive, (hipStream_t)stream);
+    return launch_fp(p, vol, vs, proj, ps, batch, plain(additive), (hipStream_t)stream, workspace,
+                     workspace_bytes);
 }
 
 int tsx_backward_batched(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int64_t ps,
-                         int64_t batch, int additive, void *stream)
+                         int64_t batch, int additive, void *workspace, int64_t workspace_bytes,
+                         void *stream)
 {
     int rc = check_plan(p);
     if (rc) return rc;
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
     DeviceGuard guard(p->device);
-    return launch_bp(p, vol, vs, proj, ps, batch, additive, (hipStream_t)stream);
+    return launch_bp(p, vol, vs, proj, ps, batch, plain(additive), (hipStream_t)stream, workspace,
+                     workspace_bytes);
 }
 
 int tsx_forward_batched_f64(const tsx_plan *p, const double *vol, int64_t vs, double *proj,
@@ -623,7 +705,7 @@ int tsx_forward_batched_f64(const tsx_plan *p, const double *vol, int64_t vs, do
     if (rc) return rc;
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
     DeviceGuard guard(p->device);
-    return launch_general<double>(p, true, (double *)vol, vs, proj, ps, batch, additive,
+    return launch_general<double>(p, true, (double *)vol, vs, proj, ps, batch, plain(additive),
                                   (hipStream_t)stream);
 }
 
@@ -634,18 +716,53 @@ int tsx_backward_batched_f64(const tsx_plan *p, double *vol, int64_t vs, const d
     if (rc) return rc;
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
     DeviceGuard guard(p->device);
-    return launch_general<double>(p, false, vol, vs, (double *)proj, ps, batch, additive,
+    return launch_general<double>(p, false, vol, vs, (double *)proj, ps, batch, plain(additive),
                                   (hipStream_t)stream);
+}
+
+int tsx_forward_residual(const tsx_plan *p, const float *vol, int64_t vs, const float *y,
+                         const float *w, float *proj, int64_t ps, int64_t batch, void *workspace,
+                         int64_t workspace_bytes, void *stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!vol || !proj || !y || !w)
+        return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+    Epilogue e;
+    e.mode = EP_RESIDUAL;
+    e.y = y;
+    e.w = w;
+    e.stride = ps;
+    DeviceGuard guard(p->device);
+    return launch_fp(p, vol, vs, proj, ps, batch, e, (hipStream_t)stream, workspace,
+                     workspace_bytes);
+}
+
+int tsx_backward_scaled_add(const tsx_plan *p, float *vol, int64_t vs, const float *proj,
+                            int64_t ps, const float *w, int64_t batch, void *workspace,
+                            int64_t workspace_bytes, void *stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!vol || !proj || !w) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+    Epilogue e;
+    e.mode = EP_SCALED_ADD;
+    e.y = nullptr;
+    e.w = w;
+    e.stride = vs;
+    DeviceGuard guard(p->device);
+    return launch_bp(p, vol, vs, proj, ps, batch, e, (hipStream_t)stream, workspace,
+                     workspace_bytes);
 }
 
 int tsx_forward(const tsx_plan *p, const float *vol, float *proj, int additive, void *stream)
 {
-    return tsx_forward_batched(p, vol, 0, proj, 0, 1, additive, stream);
+    return tsx_forward_batched(p, vol, 0, proj, 0, 1, additive, nullptr, 0, stream);
 }
 
 int tsx_backward(const tsx_plan *p, float *vol, const float *proj, int additive, void *stream)
 {
-    return tsx_backward_batched(p, vol, 0, proj, 0, 1, additive, stream);
+    return tsx_backward_batched(p, vol, 0, proj, 0, 1, additive, nullptr, 0, stream);
 }
 
 int tsx_direct_fpbp3d(const tsx_plan *p, float *vol, float *proj, int mode, int op, void *stream)

@@ -97,12 +97,16 @@ def _run(projector, vol_t, proj_t, forward, additive, batch=1, vol_stride=0, pro
         return
     if vol_t.dtype != torch.float32 or proj_t.dtype != torch.float32:
         raise TypeError("projection kernels take float32 (or float64 on both sides)")
+    # Scratch memory comes from torch's caching allocator on the launch stream, so
+    # that torch owns every allocation (and HIP graph capture sees none).
+    nbytes = plan.workspace_size(_tsx.TSX_OP_FP if forward else _tsx.TSX_OP_BP, batch)
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=vol_t.device)
     if forward:
         plan.forward(vol_t.data_ptr(), proj_t.data_ptr(), additive, stream, batch, vol_stride,
-                     proj_stride)
+                     proj_stride, ws.data_ptr(), nbytes)
     else:
         plan.backward(vol_t.data_ptr(), proj_t.data_ptr(), additive, stream, batch, vol_stride,
-                      proj_stride)
+                      proj_stride, ws.data_ptr(), nbytes)
 
 
 def direct_project(projector, vol_link, proj_link, forward=None, additive=False):
