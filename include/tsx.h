@@ -135,6 +135,24 @@ int tsx_backward_batched(const tsx_plan *plan, float *vol, int64_t vol_stride, c
                          int64_t proj_stride, int64_t batch, int additive, void *workspace,
                          int64_t workspace_bytes, void *hip_stream);
 
+/* Fused launches over n independent instances (each its own plan, volume
+ * and projections; shapes may differ; all plans on one device). Instances on
+ * the fast kernels share one launch per MAX 16 instances, so the device never
+ * idles between instances; the others run one launch each. Results equal
+ * those of n separate tsx_forward / tsx_backward calls bit for bit. No
+ * counterpart in the reference (tomosipo makes one synchronous projector call
+ * per operator application, tomosipo/astra.py:147-153); for callers that
+ * project many volumes at once, such as multi-volume throughput jobs and the
+ * request server. `workspace` as for the batched calls, sized by
+ * tsx_multi_workspace_size. */
+int tsx_multi_workspace_size(const tsx_plan *const *plans, int n, int op, int64_t *bytes);
+int tsx_multi_forward(const tsx_plan *const *plans, const float *const *vols, float *const *projs,
+                      int n, int additive, void *workspace, int64_t workspace_bytes,
+                      void *hip_stream);
+int tsx_multi_backward(const tsx_plan *const *plans, float *const *vols, const float *const *projs,
+                       int n, int additive, void *workspace, int64_t workspace_bytes,
+                       void *hip_stream);
+
 /* The two halves of one SIRT iteration x += C * A^T(R * (y - A x)) with the
  * elementwise work fused into the projection kernels. They replace the
  * projector calls plus elementwise operations of tomosipo's SIRT loops

@@ -37,7 +37,8 @@ EXPORTED = (
     "tsx_plan_create", "tsx_plan_destroy", "tsx_plan_info_get", "tsx_plan_axes",
     "tsx_workspace_size",
     "tsx_direct_fpbp3d", "tsx_forward", "tsx_backward", "tsx_forward_batched",
-    "tsx_backward_batched", "tsx_forward_residual", "tsx_backward_scaled_add",
+    "tsx_backward_batched", "tsx_multi_workspace_size", "tsx_multi_forward",
+    "tsx_multi_backward", "tsx_forward_residual", "tsx_backward_scaled_add",
     "tsx_forward_batched_f64", "tsx_backward_batched_f64",
     "tsx_last_error", "tsx_version",
 )
@@ -109,6 +110,9 @@ def load():
         lib.tsx_workspace_size.argtypes = [P, i32, i64, ctypes.POINTER(i64)]
         lib.tsx_forward_batched.argtypes = [P, P, i64, P, i64, i64, i32, P, i64, P]
         lib.tsx_backward_batched.argtypes = [P, P, i64, P, i64, i64, i32, P, i64, P]
+        lib.tsx_multi_workspace_size.argtypes = [P, i32, i32, ctypes.POINTER(i64)]
+        lib.tsx_multi_forward.argtypes = [P, P, P, i32, i32, P, i64, P]
+        lib.tsx_multi_backward.argtypes = [P, P, P, i32, i32, P, i64, P]
         lib.tsx_forward_residual.argtypes = [P, P, i64, P, P, P, i64, i64, P, i64, P]
         lib.tsx_backward_scaled_add.argtypes = [P, P, i64, P, i64, P, i64, P, i64, P]
         lib.tsx_forward_batched_f64.argtypes = [P, P, i64, P, i64, i64, i32, P]
@@ -239,3 +243,29 @@ class Plan:
 
 def version():
     return load().tsx_version().decode()
+
+
+def _ptr_array(values):
+    arr = (ctypes.c_void_p * max(len(values), 1))()
+    for k, v in enumerate(values):
+        arr[k] = v
+    return arr
+
+
+def multi_workspace_size(plans, op):
+    lib = load()
+    out = ctypes.c_int64()
+    hs = _ptr_array([p.handle.value for p in plans])
+    _check(lib.tsx_multi_workspace_size(hs, len(plans), int(op), ctypes.byref(out)))
+    return int(out.value)
+
+
+def multi_project(plans, vol_ptrs, proj_ptrs, forward, additive, stream, workspace=None,
+                  workspace_bytes=0):
+    """One fused launch over many instances (tsx_multi_forward / tsx_multi_backward)."""
+    lib = load()
+    hs = _ptr_array([p.handle.value for p in plans])
+    vs = _ptr_array(vol_ptrs)
+    ps = _ptr_array(proj_ptrs)
+    fn = lib.tsx_multi_forward if forward else lib.tsx_multi_backward
+    _check(fn(hs, vs, ps, len(plans), int(bool(additive)), workspace, int(workspace_bytes), stream))

@@ -206,26 +206,34 @@ __device__ __forceinline__ double column_of(const AngleParams &A, double n, doub
     return (n + t0 * A.S[0] - A.P0[0]) * A.Bj[0] + (c + t0 * A.S[2] - A.P0[2]) * A.Bj[2];
 }
 
-// Brick of BR x BR columns (m, c2 plane) by BZ voxels along z.
+// One instance of the separable backprojector as seen by a block.
+struct BpView {
+    const AngleParams *angles;
+    int n_angles, rows, cols, nx, ny, nz, bricks_x, bricks_y, bricks_z, batch;
+    float *vol;
+    int64_t vol_stride;
+    const float *ly;    // length-weighted projections, batch entries ly_bytes apart
+    uint32_t ly_bytes;
+    Epilogue ep;
+};
+
+// Work item `logical` of one instance: (batch entry, brick), a brick being
+// BR x BR columns (x, y) by BZ voxels along z.
 template <int BZ>
-__global__ void __launch_bounds__(BR *BR)
-bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int nx,
-             int ny, int nz, int bricks_x, int bricks_y, int bricks_z, int n_blocks,
-             float *__restrict__ vol, int64_t vol_stride, const float *__restrict__ ly,
-             uint32_t ly_bytes, Epilogue ep)
+__device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, BpShared<BZ> &sh)
 {
-    __shared__ BpShared<BZ> sh;
-    const int per = (n_blocks + 7) / 8;
-    const int logical = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
-    if (logical >= n_blocks) return;
-    const int bx = logical % bricks_x;
-    const int by = (logical / bricks_x) % bricks_y;
-    const int bz = (logical / (bricks_x * bricks_y)) % bricks_z;
-    const int b = logical / (bricks_x * bricks_y * bricks_z);
+    const int n_angles = v.n_angles, rows = v.rows, cols = v.cols;
+    const int bx = logical % v.bricks_x;
+    const int by = (logical / v.bricks_x) % v.bricks_y;
+    const int bz = (logical / (v.bricks_x * v.bricks_y)) % v.bricks_z;
+    const int b = logical / (v.bricks_x * v.bricks_y * v.bricks_z);
+    if (b >= v.batch) return;
+    const AngleParams *__restrict__ angles = v.angles;
     const int tx = threadIdx.x, ty = threadIdx.y, t = ty * BR + tx;
     const int x = bx * BR + tx, y = by * BR + ty, z0 = bz * BZ;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(ly + (int64_t)b * ((int64_t)ly_bytes / 4)), (short)0, (int)ly_bytes, 0x00020000);
+        (void *)(v.ly + (int64_t)b * ((int64_t)v.ly_bytes / 4)), (short)0, (int)v.ly_bytes,
+        0x00020000);
     const uint32_t row_bytes = (uint32_t)n_angles * (uint32_t)cols * 4u;
     float acc[BZ];
 #pragma unroll
@@ -337,14 +345,41 @@ bp_separable(const AngleParams *__restrict__ angles, int n_angles, int rows, int
             __syncthreads();  // u is rewritten by the next chunk or angle
         }
     }
-    if (x < nx && y < ny) {
-        float *out = vol + (int64_t)b * vol_stride;
+    if (x < v.nx && y < v.ny) {
+        float *out = v.vol + (int64_t)b * v.vol_stride;
 #pragma unroll
         for (int k = 0; k < BZ; ++k) {
             const int z = z0 + k;
-            if (z < nz) emit(out, ((int64_t)z * ny + y) * nx + x, b, acc[k], ep);
+            if (z < v.nz) emit(out, ((int64_t)z * v.ny + y) * v.nx + x, b, acc[k], v.ep);
         }
     }
+}
+
+template <int BZ>
+__global__ void __launch_bounds__(BR *BR) bp_separable(BpView v, int n_blocks)
+{
+    __shared__ BpShared<BZ> sh;
+    const int logical = xcd_logical(n_blocks);
+    if (logical >= n_blocks) return;
+    bp_separable_work<BZ>(v, logical, sh);
+}
+
+struct BpMulti {
+    BpView v[MAX_FUSED];
+    int begin[MAX_FUSED + 1];
+    int n;
+};
+
+template <int BZ>
+__global__ void __launch_bounds__(BR *BR) bp_separable_multi(BpMulti m)
+{
+    __shared__ BpShared<BZ> sh;
+    const int total = m.begin[m.n];
+    const int logical = xcd_logical(total);
+    if (logical >= total) return;
+    int k = 0;
+    while (k + 1 < m.n && logical >= m.begin[k + 1]) ++k;
+    bp_separable_work<BZ>(m.v[k], logical - m.begin[k], sh);
 }
 
 }  // namespace tsx

@@ -352,31 +352,34 @@ __device__ __forceinline__ float march_padded_pipe4(__amdgpu_buffer_rsrc_t rs, c
     return r.len * (s0 + s1);
 }
 
+// One instance of the padded forward projector as seen by a block.
+struct FpView {
+    const AngleParams *angles;
+    int n_angles, rows, cols, tiles_u, tiles_v, batch;
+    Layouts L;
+    float *proj;
+    int64_t proj_stride;
+    Epilogue ep;
+};
+
+// Work item `logical` of one instance: (batch entry, detector tile, angle),
+// angle fastest so that blocks resident together share a detector tile and
+// therefore a slab of the volume.
 template <int SS, int PIPE>
-__global__ void __launch_bounds__(FP_BX *FP_BY)
-fp_padded(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int tiles_u,
-          int tiles_v, int n_blocks, Layouts L, int batch, float *__restrict__ proj,
-          int64_t proj_stride, Epilogue ep)
+__device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
 {
-    // XCD-aware order: consecutive hardware block ids go round-robin to the 8
-    // XCDs; give each XCD a contiguous range of logical ids. Logical ids run
-    // angle fastest so that blocks resident together share a detector tile and
-    // therefore a slab of the volume.
-    const int per = (n_blocks + 7) / 8;
-    const int logical = (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
-    if (logical >= n_blocks) return;
-    const int a = logical % n_angles;
-    const int rest = logical / n_angles;
-    const int tiles = tiles_u * tiles_v;
+    const int a = logical % v.n_angles;
+    const int rest = logical / v.n_angles;
+    const int tiles = v.tiles_u * v.tiles_v;
     const int tile = rest % tiles;
     const int b = rest / tiles;
-    const int j = (tile % tiles_u) * FP_BX + threadIdx.x;
-    const int i = (tile / tiles_u) * FP_BY + threadIdx.y;
-    if (j >= cols || i >= rows || b >= batch) return;
-    const AngleParams &A = angles[a];
-    const float *lp = L.ptr[A.m] + (int64_t)b * L.stride;
+    const int j = (tile % v.tiles_u) * FP_BX + threadIdx.x;
+    const int i = (tile / v.tiles_u) * FP_BY + threadIdx.y;
+    if (j >= v.cols || i >= v.rows || b >= v.batch) return;
+    const AngleParams &A = v.angles[a];
+    const float *lp = v.L.ptr[A.m] + (int64_t)b * v.L.stride;
     const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void *)lp, (short)0, (int)L.bytes[A.m], 0x00020000);
+        __builtin_amdgcn_make_buffer_rsrc((void *)lp, (short)0, (int)v.L.bytes[A.m], 0x00020000);
     const uint32_t st_m = (uint32_t)A.pst[0], st_1 = (uint32_t)A.pst[1];
     float acc = 0.0f;
 #pragma unroll
@@ -393,7 +396,45 @@ fp_padded(const AngleParams *__restrict__ angles, int n_angles, int rows, int co
         }
     }
     if (SS > 1) acc *= 1.0f / (float)(SS * SS);
-    emit(proj + (int64_t)b * proj_stride, ((int64_t)i * n_angles + a) * cols + j, b, acc, ep);
+    emit(v.proj + (int64_t)b * v.proj_stride, ((int64_t)i * v.n_angles + a) * v.cols + j, b, acc,
+         v.ep);
+}
+
+// XCD-aware order: consecutive hardware block ids go round-robin to the 8
+// XCDs; each XCD gets a contiguous range of logical ids.
+__device__ __forceinline__ int xcd_logical(int n_blocks)
+{
+    const int per = (n_blocks + 7) / 8;
+    return (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+}
+
+template <int SS, int PIPE>
+__global__ void __launch_bounds__(FP_BX *FP_BY) fp_padded(FpView v, int n_blocks)
+{
+    const int logical = xcd_logical(n_blocks);
+    if (logical >= n_blocks) return;
+    fp_padded_work<SS, PIPE>(v, logical);
+}
+
+// Fused launch over up to MAX_FUSED instances (any shapes): instance k owns the
+// logical ids [begin[k], begin[k + 1]).
+constexpr int MAX_FUSED = 16;
+
+struct FpMulti {
+    FpView v[MAX_FUSED];
+    int begin[MAX_FUSED + 1];
+    int n;
+};
+
+template <int PIPE>
+__global__ void __launch_bounds__(FP_BX *FP_BY) fp_padded_multi(FpMulti m)
+{
+    const int total = m.begin[m.n];
+    const int logical = xcd_logical(total);
+    if (logical >= total) return;
+    int k = 0;
+    while (k + 1 < m.n && logical >= m.begin[k + 1]) ++k;
+    fp_padded_work<1, PIPE>(m.v[k], logical - m.begin[k]);
 }
 
 }  // namespace tsx

@@ -378,6 +378,40 @@ int get_workspace(const tsx_plan *p, int64_t bytes, void *ws, int64_t ws_bytes, 
     return rc;
 }
 
+// Padded copies of one volume for every stepping axis the plan uses.
+void build_layouts(const tsx_plan *p, const float *src, float *dst, const int64_t offs[3],
+                   hipStream_t stream)
+{
+    if (p->axis_count[0] > 0) {
+        dim3 grid((unsigned)((p->nx + 31) / 32), (unsigned)((p->ny + 2 * PAD + 31) / 32),
+                  (unsigned)(p->nz + 2 * PAD));
+        hipLaunchKernelGGL(pad_transpose_x, grid, dim3(32, 8), 0, stream, src, dst + offs[0], p->nx,
+                           p->ny, p->nz);
+    }
+    if (p->axis_count[1] > 0) {
+        dim3 grid((unsigned)((p->nx + 2 * PAD + 255) / 256), (unsigned)p->ny,
+                  (unsigned)(p->nz + 2 * PAD));
+        hipLaunchKernelGGL(pad_copy_yz, grid, dim3(256), 0, stream, src, dst + offs[1], p->nx, p->ny,
+                           p->nz, 1);
+    }
+    if (p->axis_count[2] > 0) {
+        dim3 grid((unsigned)((p->nx + 2 * PAD + 255) / 256), (unsigned)(p->ny + 2 * PAD),
+                  (unsigned)p->nz);
+        hipLaunchKernelGGL(pad_copy_yz, grid, dim3(256), 0, stream, src, dst + offs[2], p->nx, p->ny,
+                           p->nz, 2);
+    }
+}
+
+void layout_offsets(const tsx_plan *p, int64_t offs[3], int64_t *per_entry)
+{
+    int64_t e = 0;
+    for (int c = 0; c < 3; ++c) {
+        offs[c] = e;
+        if (p->axis_count[c] > 0) e += p->layout_elems[c];
+    }
+    *per_entry = e;
+}
+
 int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int64_t ps,
               int64_t batch, Epilogue ep, hipStream_t stream, void *wsp = nullptr,
               int64_t ws_bytes = 0)
@@ -407,35 +441,24 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     int rc = get_workspace(p, per_entry * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
     if (rc) return rc;
     float *base = (float *)wptr;
-    for (int64_t b = 0; b < batch; ++b) {
-        const float *src = vol + b * vs;
-        float *dst = base + b * per_entry;
-        if (p->axis_count[0] > 0) {
-            dim3 grid((unsigned)((p->nx + 31) / 32), (unsigned)((p->ny + 2 * PAD + 31) / 32),
-                      (unsigned)(p->nz + 2 * PAD));
-            hipLaunchKernelGGL(pad_transpose_x, grid, dim3(32, 8), 0, stream, src, dst + offs[0],
-                               p->nx, p->ny, p->nz);
-        }
-        if (p->axis_count[1] > 0) {
-            dim3 grid((unsigned)((p->nx + 2 * PAD + 255) / 256), (unsigned)p->ny,
-                      (unsigned)(p->nz + 2 * PAD));
-            hipLaunchKernelGGL(pad_copy_yz, grid, dim3(256), 0, stream, src, dst + offs[1], p->nx,
-                               p->ny, p->nz, 1);
-        }
-        if (p->axis_count[2] > 0) {
-            dim3 grid((unsigned)((p->nx + 2 * PAD + 255) / 256), (unsigned)(p->ny + 2 * PAD),
-                      (unsigned)p->nz);
-            hipLaunchKernelGGL(pad_copy_yz, grid, dim3(256), 0, stream, src, dst + offs[2], p->nx,
-                               p->ny, p->nz, 2);
-        }
-    }
+    for (int64_t b = 0; b < batch; ++b) build_layouts(p, vol + b * vs, base + b * per_entry, offs, stream);
     TSX_HIP_CHECK(hipGetLastError());
-    Layouts L;
+    FpView v;
+    v.angles = p->dev;
+    v.n_angles = p->n_angles;
+    v.rows = p->rows;
+    v.cols = p->cols;
+    v.tiles_u = tiles_u;
+    v.tiles_v = tiles_v;
+    v.batch = (int)batch;
     for (int c = 0; c < 3; ++c) {
-        L.ptr[c] = p->axis_count[c] > 0 ? base + offs[c] : nullptr;
-        L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4);
+        v.L.ptr[c] = p->axis_count[c] > 0 ? base + offs[c] : nullptr;
+        v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4);
     }
-    L.stride = per_entry;
+    v.L.stride = per_entry;
+    v.proj = proj;
+    v.proj_stride = ps;
+    v.ep = ep;
     const int64_t nb = (int64_t)tiles_u * tiles_v * p->n_angles * batch;
     if (nb > (int64_t)0x7fffffff - 8) return fail(TSX_ERR_UNSUPPORTED, "forward projection too large for one launch");
     const int n_blocks = (int)nb;
@@ -443,9 +466,7 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     const int variant = fp_variant();
     switch (p->ss) {
 #define TSX_FP_LAUNCH(K, V)                                                                        \
-    hipLaunchKernelGGL((fp_padded<K, V>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream, p->dev,      \
-                       p->n_angles, p->rows, p->cols, tiles_u, tiles_v, n_blocks, L, (int)batch,   \
-                       proj, ps, ep)
+    hipLaunchKernelGGL((fp_padded<K, V>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream, v, n_blocks)
 #define TSX_FP_CASE(K)                                                                             \
     case K:                                                                                        \
         if (variant == 2)                                                                          \
@@ -469,6 +490,29 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     }
     TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
+}
+
+BpView bp_view(const tsx_plan *p, int BZ, float *vol, int64_t vs, const float *ly, uint32_t ly_bytes,
+               int batch, Epilogue ep)
+{
+    BpView v;
+    v.angles = p->dev;
+    v.n_angles = p->n_angles;
+    v.rows = p->rows;
+    v.cols = p->cols;
+    v.nx = p->nx;
+    v.ny = p->ny;
+    v.nz = p->nz;
+    v.bricks_x = (p->nx + BR - 1) / BR;
+    v.bricks_y = (p->ny + BR - 1) / BR;
+    v.bricks_z = (p->nz + BZ - 1) / BZ;
+    v.batch = batch;
+    v.vol = vol;
+    v.vol_stride = vs;
+    v.ly = ly;
+    v.ly_bytes = ly_bytes;
+    v.ep = ep;
+    return v;
 }
 
 int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int64_t ps,
@@ -502,19 +546,15 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
                            p->cols, proj, ps, ly, sino);
     }
     const int BZ = bp_brick_z();
-    const int bx = (p->nx + BR - 1) / BR, by = (p->ny + BR - 1) / BR, bz = (p->nz + BZ - 1) / BZ;
-    const int64_t nb = (int64_t)bx * by * bz * batch;
+    BpView v = bp_view(p, BZ, vol, vs, ly, (uint32_t)(sino * 4), (int)batch, ep);
+    const int64_t nb = (int64_t)v.bricks_x * v.bricks_y * v.bricks_z * batch;
     if (nb > (int64_t)0x7fffffff - 8) return fail(TSX_ERR_UNSUPPORTED, "backprojection too large for one launch");
     const int n_blocks = (int)nb;
     const unsigned grid = (unsigned)(((n_blocks + 7) / 8) * 8);
     if (BZ == 32)
-        hipLaunchKernelGGL(bp_separable<32>, dim3(grid), dim3(BR, BR), 0, stream, p->dev,
-                           p->n_angles, p->rows, p->cols, p->nx, p->ny, p->nz, bx, by, bz, n_blocks,
-                           vol, vs, ly, (uint32_t)(sino * 4), ep);
+        hipLaunchKernelGGL(bp_separable<32>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks);
     else
-        hipLaunchKernelGGL(bp_separable<16>, dim3(grid), dim3(BR, BR), 0, stream, p->dev,
-                           p->n_angles, p->rows, p->cols, p->nx, p->ny, p->nz, bx, by, bz, n_blocks,
-                           vol, vs, ly, (uint32_t)(sino * 4), ep);
+        hipLaunchKernelGGL(bp_separable<16>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks);
     TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
 }
@@ -553,6 +593,174 @@ int launch_general(const tsx_plan *p, bool forward, T *vol, int64_t vs, T *proj,
     }
     TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fused launches over many instances
+// ---------------------------------------------------------------------------
+constexpr int64_t WS_ALIGN = 256;
+
+int64_t align_up(int64_t v) { return (v + WS_ALIGN - 1) / WS_ALIGN * WS_ALIGN; }
+
+int check_multi(const tsx_plan *const *plans, int n)
+{
+    if (!plans || n < 0) return fail(TSX_ERR_INVALID_ARGUMENT, "plans must not be NULL");
+    for (int k = 0; k < n; ++k) {
+        int rc = check_plan(plans[k]);
+        if (rc) return rc;
+        if (plans[k]->device != plans[0]->device)
+            return fail(TSX_ERR_INVALID_ARGUMENT, "all plans of a fused launch must be on one device");
+    }
+    return TSX_OK;
+}
+
+int64_t multi_workspace_bytes(const tsx_plan *const *plans, int n, int op)
+{
+    int64_t total = 0;
+    for (int k = 0; k < n; ++k) total += align_up(workspace_bytes(plans[k], op, 1));
+    return total;
+}
+
+int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *const *projs, int n,
+             int additive, void *wsp, int64_t ws_bytes, hipStream_t stream)
+{
+    const int64_t need = multi_workspace_bytes(plans, n, TSX_OP_FP);
+    Scratch owned;
+    void *wptr = nullptr;
+    if (need > 0) {
+        int rc = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
+        if (rc) return rc;
+    }
+    const Epilogue ep = plain(additive);
+    const bool fast = !force_general();
+    FpMulti m;
+    m.n = 0;
+    m.begin[0] = 0;
+    int64_t woff = 0;
+    const int variant = fp_variant();
+    auto flush = [&]() -> int {
+        if (m.n == 0) return TSX_OK;
+        const unsigned grid = (unsigned)(((m.begin[m.n] + 7) / 8) * 8);
+        if (variant == 1)
+            hipLaunchKernelGGL(fp_padded_multi<1>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
+        else if (variant == 0)
+            hipLaunchKernelGGL(fp_padded_multi<0>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
+        else
+            hipLaunchKernelGGL(fp_padded_multi<2>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
+        TSX_HIP_CHECK(hipGetLastError());
+        m.n = 0;
+        return TSX_OK;
+    };
+    for (int k = 0; k < n; ++k) {
+        const tsx_plan *p = plans[k];
+        if (!vols[k] || !projs[k]) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+        if (!(fast && p->fp_padded && p->ss == 1)) {
+            // this instance takes its own launch (general kernel or supersampling)
+            int rc = launch_fp(p, vols[k], 0, projs[k], 0, 1, ep, stream,
+                               wptr ? (char *)wptr + woff : nullptr, align_up(workspace_bytes(p, TSX_OP_FP, 1)));
+            if (rc) return rc;
+            woff += align_up(workspace_bytes(p, TSX_OP_FP, 1));
+            continue;
+        }
+        int64_t offs[3], per_entry;
+        layout_offsets(p, offs, &per_entry);
+        float *base = (float *)((char *)wptr + woff);
+        woff += align_up(per_entry * 4);
+        build_layouts(p, vols[k], base, offs, stream);
+        FpView &v = m.v[m.n];
+        v.angles = p->dev;
+        v.n_angles = p->n_angles;
+        v.rows = p->rows;
+        v.cols = p->cols;
+        v.tiles_u = (p->cols + FP_BX - 1) / FP_BX;
+        v.tiles_v = (p->rows + FP_BY - 1) / FP_BY;
+        v.batch = 1;
+        for (int c = 0; c < 3; ++c) {
+            v.L.ptr[c] = p->axis_count[c] > 0 ? base + offs[c] : nullptr;
+            v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4);
+        }
+        v.L.stride = 0;
+        v.proj = projs[k];
+        v.proj_stride = 0;
+        v.ep = ep;
+        const int64_t blocks = (int64_t)v.tiles_u * v.tiles_v * v.n_angles;
+        if ((int64_t)m.begin[m.n] + blocks > (int64_t)0x7fffffff - 8) {
+            int rc = flush();
+            if (rc) return rc;
+            m.v[0] = v;
+        }
+        m.begin[m.n + 1] = m.begin[m.n] + (int)blocks;
+        m.n += 1;
+        if (m.n == MAX_FUSED) {
+            int rc = flush();
+            if (rc) return rc;
+        }
+    }
+    TSX_HIP_CHECK(hipGetLastError());
+    return flush();
+}
+
+int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *const *projs, int n,
+             int additive, void *wsp, int64_t ws_bytes, hipStream_t stream)
+{
+    const int64_t need = multi_workspace_bytes(plans, n, TSX_OP_BP);
+    Scratch owned;
+    void *wptr = nullptr;
+    if (need > 0) {
+        int rc = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
+        if (rc) return rc;
+    }
+    const Epilogue ep = plain(additive);
+    const bool fast = !force_general();
+    const int BZ = bp_brick_z();
+    BpMulti m;
+    m.n = 0;
+    m.begin[0] = 0;
+    int64_t woff = 0;
+    auto flush = [&]() -> int {
+        if (m.n == 0) return TSX_OK;
+        const unsigned grid = (unsigned)(((m.begin[m.n] + 7) / 8) * 8);
+        if (BZ == 32)
+            hipLaunchKernelGGL(bp_separable_multi<32>, dim3(grid), dim3(BR, BR), 0, stream, m);
+        else
+            hipLaunchKernelGGL(bp_separable_multi<16>, dim3(grid), dim3(BR, BR), 0, stream, m);
+        TSX_HIP_CHECK(hipGetLastError());
+        m.n = 0;
+        return TSX_OK;
+    };
+    for (int k = 0; k < n; ++k) {
+        const tsx_plan *p = plans[k];
+        if (!vols[k] || !projs[k]) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+        const int64_t wb = align_up(workspace_bytes(p, TSX_OP_BP, 1));
+        if (!(fast && p->bp_separable)) {
+            int rc = launch_bp(p, vols[k], 0, projs[k], 0, 1, ep, stream,
+                               wptr ? (char *)wptr + woff : nullptr, wb);
+            if (rc) return rc;
+            woff += wb;
+            continue;
+        }
+        float *ly = (float *)((char *)wptr + woff);
+        woff += wb;
+        const int64_t sino = bp_workspace_elems(p);
+        dim3 sgrid((unsigned)((p->cols + 255) / 256), (unsigned)p->n_angles, (unsigned)p->rows);
+        hipLaunchKernelGGL(scale_by_length, sgrid, dim3(256), 0, stream, p->dev, p->n_angles, p->rows,
+                           p->cols, projs[k], (int64_t)0, ly, sino);
+        BpView v = bp_view(p, BZ, vols[k], 0, ly, (uint32_t)(sino * 4), 1, ep);
+        const int64_t blocks = (int64_t)v.bricks_x * v.bricks_y * v.bricks_z;
+        if ((int64_t)m.begin[m.n] + blocks > (int64_t)0x7fffffff - 8) {
+            int rc = flush();
+            if (rc) return rc;
+        }
+        m.v[m.n] = v;
+        m.begin[m.n + 1] = m.begin[m.n] + (int)blocks;
+        m.n += 1;
+        if (m.n == MAX_FUSED) {
+            int rc = flush();
+            if (rc) return rc;
+        }
+    }
+    TSX_HIP_CHECK(hipGetLastError());
+    return flush();
 }
 
 }  // namespace
@@ -753,6 +961,39 @@ int tsx_backward_scaled_add(const tsx_plan *p, float *vol, int64_t vs, const flo
     DeviceGuard guard(p->device);
     return launch_bp(p, vol, vs, proj, ps, batch, e, (hipStream_t)stream, workspace,
                      workspace_bytes);
+}
+
+int tsx_multi_workspace_size(const tsx_plan *const *plans, int n, int op, int64_t *bytes)
+{
+    if (!plans || !bytes || n < 0) return fail(TSX_ERR_INVALID_ARGUMENT, "plans and bytes must not be NULL");
+    if (op != TSX_OP_FP && op != TSX_OP_BP)
+        return fail(TSX_ERR_INVALID_ARGUMENT, "op must be TSX_OP_FP (0) or TSX_OP_BP (1)");
+    for (int k = 0; k < n; ++k)
+        if (!plans[k]) return fail(TSX_ERR_INVALID_ARGUMENT, "plan is NULL");
+    *bytes = multi_workspace_bytes(plans, n, op);
+    return TSX_OK;
+}
+
+int tsx_multi_forward(const tsx_plan *const *plans, const float *const *vols, float *const *projs,
+                      int n, int additive, void *workspace, int64_t workspace_bytes, void *stream)
+{
+    int rc = check_multi(plans, n);
+    if (rc) return rc;
+    if (n == 0) return TSX_OK;
+    if (!vols || !projs) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointer arrays must not be NULL");
+    DeviceGuard guard(plans[0]->device);
+    return multi_fp(plans, vols, projs, n, additive, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int tsx_multi_backward(const tsx_plan *const *plans, float *const *vols, const float *const *projs,
+                       int n, int additive, void *workspace, int64_t workspace_bytes, void *stream)
+{
+    int rc = check_multi(plans, n);
+    if (rc) return rc;
+    if (n == 0) return TSX_OK;
+    if (!vols || !projs) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointer arrays must not be NULL");
+    DeviceGuard guard(plans[0]->device);
+    return multi_bp(plans, vols, projs, n, additive, workspace, workspace_bytes, (hipStream_t)stream);
 }
 
 int tsx_forward(const tsx_plan *p, const float *vol, float *proj, int additive, void *stream)
