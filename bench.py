@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--instances", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="finish with a 1 GiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -128,15 +130,16 @@ def main():
         bs.append(torch.empty(A.domain_shape, device=dev))
     torch.cuda.synchronize()
 
+    from tomosipo_amd.fused import fused_forward, fused_backward
+
     def step(ev=None):
+        # one fused launch for the 16 forward projections, one for the 16 backprojections
         if ev is not None:
             ev[0].record()
-        for A, x, y in zip(ops, xs, ys):
-            A(x, out=y)
+        fused_forward(ops, xs, out=ys)
         if ev is not None:
             ev[1].record()
-        for A, y, b in zip(ops, ys, bs):
-            A.T(y, out=b)
+        fused_backward(ops, ys, out=bs)
         if ev is not None:
             ev[2].record()
 
@@ -156,9 +159,10 @@ def main():
     va = float(N_VOX) ** 3 * N_ANGLES * args.instances
     gups = 2 * va / (ms_per_step * 1e-3) / 1e9
 
-    # roofline of the dominant kernel (per-instance launch)
-    samples = float(DET[0]) * DET[1] * N_VOX * N_ANGLES
-    dom, dom_ms = ("backprojection", bp_ms) if bp_ms >= fp_ms else ("forward", fp_ms)
+    # roofline of the dominant kernel; one fused launch covers all instances
+    samples = float(DET[0]) * DET[1] * N_VOX * N_ANGLES * args.instances
+    dom, dom_ms = ("backprojection", bp_ms * args.instances) if bp_ms >= fp_ms else \
+        ("forward", fp_ms * args.instances)
     achieved_tf = samples * FLOP_PER_SAMPLE / (dom_ms * 1e-3) / 1e12
     gather_tbs = samples * GATHER_B_PER_SAMPLE / (dom_ms * 1e-3) / 1e12
     roofline = {
@@ -168,7 +172,7 @@ def main():
         "unit": "TFLOP/s",
         "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4),
         "traffic": None,
-        "kernel": dom,
+        "kernel": f"fused {dom} over {args.instances} instances (incl. its pre-pass)",
         "kernel_ms": round(dom_ms, 3),
         "basis": "FP32 vector flops, 14 per bilinear sample (no MFMA: projection is a gather)",
     }
@@ -186,7 +190,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded box + N(0, 0.01) phantoms, helical cone_vec with seeded jitter)",
         "config": {
-            "workload": f"{args.instances} x cone_vec 512^3, 720 angles, 512x768 detector, FP then BP",
+            "workload": f"{args.instances} x cone_vec 512^3, 720 angles, 512x768 detector, "
+                        "fused FP launch then fused BP launch",
             "instances": args.instances,
             "fp_ms_per_instance": round(fp_ms, 3),
             "bp_ms_per_instance": round(bp_ms, 3),
@@ -200,6 +205,21 @@ def main():
             "basis": "16 B of bilinear corners per sample against aggregate LDS bandwidth",
         },
     }
+    # HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter
+    # summary (FETCH_SIZE + WRITE_SIZE per instance, copy-calibrated), times instances
+    tpath = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            t = json.load(f)
+        key = "bp_separable_bytes_per_instance" if dom == "backprojection" else "fp_padded_bytes_per_instance"
+        if key in t:
+            roofline["traffic"] = float(t[key]) * args.instances
+            roofline["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
+    if args.calibrate_copy:
+        src = torch.empty(256 * 1024 * 1024, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
     if not args.no_cpu:
         rec["cpu_baseline"] = cpu_baseline(ts, args.cpu_seconds)
         rec["cpu_baseline"]["gpu_over_cpu"] = round(gups / rec["cpu_baseline"]["value"], 1)
