@@ -97,6 +97,28 @@ def cpu_baseline(ts, target_s=12.0):
     }
 
 
+def serving_benchmark(n_requests):
+    """Mixed request stream (Poisson 200 req/s; sides 64-256; 60-720 angles; cone or
+    parallel; FP, BP or 20-iteration SIRT): continuous batching vs one request per launch."""
+    from tomosipo_amd import serving
+
+    reqs = serving.make_stream(n_requests, rate=200.0, seed=7)
+    # warm both servers' plans and kernels on a few requests first
+    warm = serving.make_stream(6, rate=1000.0, seed=8, sirt_iterations=1)
+    serving.Server().run(warm)
+    serving.serve_one_at_a_time(warm)
+    batched = serving.Server(hbm_budget_bytes=24 << 30).run(reqs)
+    single = serving.serve_one_at_a_time(reqs)
+    keys = ("req_per_s", "p50_latency_s", "p99_latency_s", "makespan_s")
+    return {
+        "stream": f"{n_requests} requests, Poisson 200 req/s, seed 7",
+        "continuous_batching": {k: round(batched[k], 4) for k in keys} |
+        {"deferred_admissions": batched["deferred_admissions"]},
+        "one_request_per_launch": {k: round(single[k], 4) for k in keys},
+        "req_per_s_speedup": round(batched["req_per_s"] / single["req_per_s"], 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +127,8 @@ def main():
     ap.add_argument("--instances", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--serving-requests", type=int, default=120,
+                    help="requests in the mixed serving stream (0 skips the serving run)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="finish with a 1 GiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -215,6 +239,8 @@ def main():
         if key in t:
             roofline["traffic"] = float(t[key]) * args.instances
             roofline["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
+    if args.serving_requests > 0:
+        rec["serving"] = serving_benchmark(args.serving_requests)
     if args.calibrate_copy:
         src = torch.empty(256 * 1024 * 1024, device=dev)
         dst = torch.empty_like(src)

@@ -1,0 +1,304 @@
+"""Continuous-batching projection server.
+
+Requests (forward projection, backprojection or an n-iteration SIRT
+reconstruction, each on its own volume and geometry) arrive over time. The
+server admits a request as soon as it has arrived and its device memory fits
+the HBM budget, keeps its volume and projections resident, and advances every
+admitted request by one projection per tick: all requests whose next step is
+a forward projection go into one fused launch, all whose next step is a
+backprojection into another (``tomosipo_amd.fused``). Requests that arrive
+while others are on the device join at the next tick, so their work overlaps
+on the device. Results are bit-identical to running each request alone,
+because the fused kernels are deterministic and compute exactly what the
+per-operator kernels compute.
+
+``serve_one_at_a_time`` is the reference point: the same requests in arrival
+order, one operator call per launch, one request at a time.
+
+The reference has no server; its operators are applied one synchronous call
+at a time (tomosipo/Operator.py:176-233, tomosipo/astra.py:147-153).
+"""
+import collections
+import dataclasses
+import time
+
+import numpy as np
+
+from . import epsilon
+
+
+@dataclasses.dataclass
+class Request:
+    """One job. ``op`` is "fp", "bp" or "sirt"; ``kind`` "cone" or "parallel"."""
+    rid: int
+    arrival: float          # seconds after the stream starts
+    kind: str
+    side: int               # volume side (voxels) of a unit-cube volume
+    n_angles: int
+    op: str
+    iterations: int = 20    # SIRT only
+    seed: int = 0
+
+
+def make_stream(n_requests, rate=200.0, seed=0, sides=(64, 128, 192, 256), angles=(60, 720),
+                ops=("fp", "bp", "sirt"), sirt_iterations=20):
+    """Seeded Poisson stream: exponential inter-arrival times at ``rate`` per second."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n_requests))
+    out = []
+    for k in range(n_requests):
+        out.append(Request(rid=k, arrival=float(t[k]),
+                           kind=("cone", "parallel")[int(rng.integers(2))],
+                           side=int(sides[int(rng.integers(len(sides)))]),
+                           n_angles=int(rng.integers(angles[0], angles[1] + 1)),
+                           op=ops[int(rng.integers(len(ops)))], iterations=sirt_iterations,
+                           seed=int(rng.integers(1 << 30))))
+    return out
+
+
+def request_geometry(req):
+    """Unit-cube volume; detector (s, 1.5 s) of size (sqrt 2, 1.5 sqrt 2); cone
+    source and detector distances from cone_angle 0.5 (SOD = SDD = 2 sqrt 2)."""
+    from . import volume, cone, parallel
+
+    s = req.side
+    vg = volume(shape=s, size=1)
+    size = (np.sqrt(2), 1.5 * np.sqrt(2))
+    if req.kind == "cone":
+        pg = cone(angles=req.n_angles, shape=(s, 3 * s // 2), size=size, cone_angle=0.5).to_vec()
+    else:
+        pg = parallel(angles=req.n_angles, shape=(s, 3 * s // 2), size=size).to_vec()
+    return vg, pg
+
+
+def request_bytes(req):
+    """Device bytes a request holds while admitted (data, SIRT state, workspaces)."""
+    s = req.side
+    vol = 4 * s ** 3
+    sino = 4 * s * req.n_angles * (3 * s // 2)
+    pad = 4 * (s + 4) ** 2 * s * 2       # padded volume copies of the forward projector
+    if req.op == "fp":
+        return vol + sino + pad
+    if req.op == "bp":
+        return vol + 2 * sino
+    return 4 * vol + 4 * sino + pad      # x, tmp_x, C, ones | y, tmp_y, R, l*y
+
+
+def _input_data(req, torch, dev):
+    g = torch.Generator(device=dev)
+    g.manual_seed(req.seed)
+    vg_shape = (req.side,) * 3
+    sino_shape = (req.side, req.n_angles, 3 * req.side // 2)
+    shape = sino_shape if req.op == "bp" else vg_shape
+    x = torch.rand(shape, generator=g, device=dev)
+    return x
+
+
+class _Job:
+    """Device state of an admitted request and its remaining steps."""
+
+    def __init__(self, req, A, torch, dev):
+        self.req = req
+        self.A = A
+        self.steps = collections.deque()
+        data = _input_data(req, torch, dev)
+        if req.op == "fp":
+            self.inp, self.out = data, torch.empty(A.range_shape, device=dev)
+            self.steps.append("fp")
+        elif req.op == "bp":
+            self.inp, self.out = data, torch.empty(A.domain_shape, device=dev)
+            self.steps.append("bp")
+        else:
+            # SIRT on the projections of the request's phantom
+            y = A(data)
+            R = A(torch.ones(A.domain_shape, device=dev))
+            self.R = torch.clamp(1.0 / R, max=1.0 / epsilon)
+            C = A.T(torch.ones(A.range_shape, device=dev))
+            self.C = torch.clamp(1.0 / C, max=1.0 / epsilon)
+            self.y = y
+            self.x = torch.zeros(A.domain_shape, device=dev)
+            self.tmp_y = torch.empty(A.range_shape, device=dev)
+            self.tmp_x = torch.empty(A.domain_shape, device=dev)
+            for _ in range(req.iterations):
+                self.steps.append("fp")
+                self.steps.append("bp")
+        self.done_event = None
+
+    # what the next fused launch of each kind reads and writes
+    def fp_io(self):
+        return (self.inp, self.out) if self.req.op == "fp" else (self.x, self.tmp_y)
+
+    def bp_io(self):
+        return (self.inp, self.out) if self.req.op == "bp" else (self.tmp_y, self.tmp_x)
+
+    def after_fp(self, torch):
+        if self.req.op == "sirt":
+            torch.sub(self.y, self.tmp_y, out=self.tmp_y)
+            self.tmp_y.mul_(self.R)
+
+    def after_bp(self, torch):
+        if self.req.op == "sirt":
+            self.x.addcmul_(self.C, self.tmp_x)
+
+    def result(self):
+        return self.out if self.req.op in ("fp", "bp") else self.x
+
+
+class Server:
+    """Continuous batching with HBM-budget admission on one HIP device."""
+
+    def __init__(self, hbm_budget_bytes=16 << 30, device=0, max_active=64, keep_results=False):
+        import torch
+
+        self.torch = torch
+        self.dev = torch.device("cuda", device)
+        self.budget = int(hbm_budget_bytes)
+        self.max_active = max_active
+        self.keep_results = keep_results
+        self._ops = {}
+
+    def _operator(self, req):
+        from .Operator import operator
+
+        key = (req.kind, req.side, req.n_angles)
+        A = self._ops.get(key)
+        if A is None:
+            vg, pg = request_geometry(req)
+            A = operator(vg, pg)
+            self._ops[key] = A
+        return A
+
+    def run(self, requests):
+        """Serve ``requests`` in real time; returns per-request records and a summary."""
+        from .fused import fused_forward, fused_backward
+
+        torch = self.torch
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        for r in pending:
+            if request_bytes(r) > self.budget:
+                raise ValueError(f"request {r.rid} needs {request_bytes(r)} bytes, more than the "
+                                 f"HBM budget of {self.budget} bytes")
+        active, finishing, records, results = [], [], {}, {}
+        ticks = collections.deque()  # events of ticks queued on the device
+        used = 0
+        deferred = 0
+        with torch.cuda.device(self.dev):
+            t0 = time.perf_counter()
+            while pending or active or finishing:
+                now = time.perf_counter() - t0
+                # admit everything that has arrived and fits
+                while pending and pending[0].arrival <= now and len(active) < self.max_active:
+                    r = pending[0]
+                    need = request_bytes(r)
+                    if used + need > self.budget:
+                        deferred += 1
+                        break
+                    pending.popleft()
+                    used += need
+                    active.append(_Job(r, self._operator(r), torch, self.dev))
+                    records[r.rid] = {"arrival": r.arrival, "admitted": now}
+                # retire finished jobs whose work has completed on the device
+                still = []
+                for job in finishing:
+                    if job.done_event.query():
+                        t = time.perf_counter() - t0
+                        records[job.req.rid]["done"] = t
+                        used -= request_bytes(job.req)
+                        if self.keep_results:
+                            results[job.req.rid] = job.result()
+                    else:
+                        still.append(job)
+                finishing = still
+                if not active:
+                    if finishing:
+                        time.sleep(0.0002)
+                    elif pending:
+                        time.sleep(max(0.0, min(0.002, pending[0].arrival - now)))
+                    continue
+                # keep at most two ticks queued ahead of the device so that new
+                # arrivals join the next launch instead of queueing behind many
+                while len(ticks) >= 2:
+                    ticks.popleft().synchronize()
+                # one tick: one fused forward launch and one fused backprojection launch
+                fps = [j for j in active if j.steps[0] == "fp"]
+                bps = [j for j in active if j.steps[0] == "bp"]
+                if fps:
+                    io = [j.fp_io() for j in fps]
+                    fused_forward([j.A for j in fps], [a for a, _ in io], out=[b for _, b in io])
+                    for j in fps:
+                        j.after_fp(torch)
+                        j.steps.popleft()
+                if bps:
+                    io = [j.bp_io() for j in bps]
+                    fused_backward([j.A for j in bps], [a for a, _ in io], out=[b for _, b in io])
+                    for j in bps:
+                        j.after_bp(torch)
+                        j.steps.popleft()
+                ev = torch.cuda.Event()
+                ev.record()
+                ticks.append(ev)
+                keep = []
+                for j in active:
+                    if j.steps:
+                        keep.append(j)
+                    else:
+                        j.done_event = ev
+                        finishing.append(j)
+                active = keep
+            torch.cuda.synchronize()
+        return _summary(records, results, deferred)
+
+
+def serve_one_at_a_time(requests, device=0, keep_results=False):
+    """Reference server: arrival order, one request at a time, one launch per operator call."""
+    import torch
+
+    from .Operator import operator
+
+    ops, records, results = {}, {}, {}
+    dev = torch.device("cuda", device)
+    with torch.cuda.device(dev):
+        t0 = time.perf_counter()
+        for r in sorted(requests, key=lambda r: r.arrival):
+            now = time.perf_counter() - t0
+            if r.arrival > now:
+                time.sleep(r.arrival - now)
+            start = time.perf_counter() - t0
+            key = (r.kind, r.side, r.n_angles)
+            if key not in ops:
+                vg, pg = request_geometry(r)
+                ops[key] = operator(vg, pg)
+            job = _Job(r, ops[key], torch, dev)
+            A = job.A
+            while job.steps:
+                step = job.steps.popleft()
+                if step == "fp":
+                    a, b = job.fp_io()
+                    A(a, out=b)
+                    job.after_fp(torch)
+                else:
+                    a, b = job.bp_io()
+                    A.T(a, out=b)
+                    job.after_bp(torch)
+            torch.cuda.synchronize()
+            records[r.rid] = {"arrival": r.arrival, "admitted": start,
+                              "done": time.perf_counter() - t0}
+            if keep_results:
+                results[r.rid] = job.result()
+    return _summary(records, results, 0)
+
+
+def _summary(records, results, deferred):
+    lat = np.array([v["done"] - v["arrival"] for v in records.values()])
+    first = min(v["arrival"] for v in records.values())
+    last = max(v["done"] for v in records.values())
+    return {
+        "completed": len(records),
+        "req_per_s": len(records) / (last - first),
+        "p50_latency_s": float(np.percentile(lat, 50)),
+        "p99_latency_s": float(np.percentile(lat, 99)),
+        "makespan_s": last - first,
+        "deferred_admissions": deferred,
+        "records": records,
+        "results": results,
+    }
