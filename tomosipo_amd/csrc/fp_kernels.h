@@ -352,6 +352,94 @@ __device__ __forceinline__ float march_padded_pipe4(__amdgpu_buffer_rsrc_t rs, c
     return r.len * (s0 + s1);
 }
 
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Wave-synchronous march: every lane of the wave samples the same slice n at
+// the same time, over the union of the lanes' slice ranges, so the 64 lanes
+// of one detector row read neighbouring voxels of one slice (coalesced).
+// Sample positions are clamped into [lo, hi] = [PAD - 1.5, n_c + PAD + 0.5]:
+// positions outside a ray's support land in the zero border and add exactly
+// 0, positions inside are unchanged. Four samples in flight per lane.
+struct SyncRay {
+    float a1, a2, R1, R2, lo1, hi1, lo2, hi2;
+};
+
+__device__ __forceinline__ void request_sync(__amdgpu_buffer_rsrc_t rs, const SyncRay &r, float dn,
+                                             uint32_t rowoff, uint32_t rowb, Corners &c)
+{
+    const float q1 = __builtin_amdgcn_fmed3f(fmaf(dn, r.a1, r.R1), r.lo1, r.hi1);
+    const float q2 = __builtin_amdgcn_fmed3f(fmaf(dn, r.a2, r.R2), r.lo2, r.hi2);
+    const uint32_t off = rowoff + __umul24((uint32_t)floor_to_int(q1), rowb) +
+                         ((uint32_t)floor_to_int(q2) << 2);
+    c.v00 = bload(rs, off);
+    c.v01 = bload(rs, off + 4u);
+    c.v10 = bload(rs, off + rowb);
+    c.v11 = bload(rs, off + rowb + 4u);
+    c.w1 = __builtin_amdgcn_fractf(q1);
+    c.w2 = __builtin_amdgcn_fractf(q2);
+}
+
+__device__ __forceinline__ float march_padded_sync(__amdgpu_buffer_rsrc_t rs, const RaySetup &r,
+                                                   bool valid, int nref, int n1, int n2,
+                                                   uint32_t st_m, uint32_t st_1)
+{
+    const int wlo = wave_min(valid ? r.nlo : 0x7fffffff);
+    const int whi = wave_max(valid ? r.nhi : -1);
+    if (wlo > whi) return 0.0f;
+    SyncRay q;
+    q.lo1 = (float)PAD - 1.5f;
+    q.lo2 = q.lo1;
+    q.hi1 = (float)(n1 + PAD) + 0.5f;
+    q.hi2 = (float)(n2 + PAD) + 0.5f;
+    if (valid) {
+        q.a1 = r.a1;
+        q.a2 = r.a2;
+        q.R1 = r.R1;
+        q.R2 = r.R2;
+    } else {  // a ray without support samples the zero border
+        q.a1 = q.a2 = 0.0f;
+        q.R1 = q.R2 = q.lo1;
+    }
+    const uint32_t rowb = st_1 * 4u, stepb = st_m * 4u;
+    float dn = (float)(wlo - nref);
+    uint32_t rowoff = (uint32_t)wlo * stepb;
+    Corners A0, A1, A2, A3;
+    request_sync(rs, q, dn, rowoff, rowb, A0);
+    request_sync(rs, q, dn + 1.0f, rowoff + stepb, rowb, A1);
+    request_sync(rs, q, dn + 2.0f, rowoff + 2u * stepb, rowb, A2);
+    request_sync(rs, q, dn + 3.0f, rowoff + 3u * stepb, rowb, A3);
+    float s0 = 0.0f, s1 = 0.0f;
+    const int cnt = whi - wlo + 1;
+    for (int k = 0; k < cnt; k += 4) {
+        s0 += interp(A0);
+        request_sync(rs, q, dn + 4.0f, rowoff + 4u * stepb, rowb, A0);
+        const float v1 = interp(A1);
+        s1 += (k + 1 < cnt) ? v1 : 0.0f;
+        request_sync(rs, q, dn + 5.0f, rowoff + 5u * stepb, rowb, A1);
+        const float v2 = interp(A2);
+        s0 += (k + 2 < cnt) ? v2 : 0.0f;
+        request_sync(rs, q, dn + 6.0f, rowoff + 6u * stepb, rowb, A2);
+        const float v3 = interp(A3);
+        s1 += (k + 3 < cnt) ? v3 : 0.0f;
+        request_sync(rs, q, dn + 7.0f, rowoff + 7u * stepb, rowb, A3);
+        dn += 4.0f;
+        rowoff += 4u * stepb;
+    }
+    return valid ? r.len * (s0 + s1) : 0.0f;
+}
+
 // One instance of the padded forward projector as seen by a block.
 struct FpView {
     const AngleParams *angles;
@@ -375,7 +463,9 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
     const int b = rest / tiles;
     const int j = (tile % v.tiles_u) * FP_BX + threadIdx.x;
     const int i = (tile / v.tiles_u) * FP_BY + threadIdx.y;
-    if (j >= v.cols || i >= v.rows || b >= v.batch) return;
+    if (b >= v.batch) return;  // uniform over the block
+    const bool inside = j < v.cols && i < v.rows;
+    if (PIPE != 3 && !inside) return;
     const AngleParams &A = v.angles[a];
     const float *lp = v.L.ptr[A.m] + (int64_t)b * v.L.stride;
     const __amdgpu_buffer_rsrc_t rs =
@@ -389,12 +479,18 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
             RaySetup r;
             const double fj = SS == 1 ? (double)j : j + (q + 0.5) / SS - 0.5;
             const double fi = SS == 1 ? (double)i : i + (p + 0.5) / SS - 0.5;
-            if (ray_setup(A, fj, fi, (double)PAD, r))
+            if (PIPE == 3) {
+                // every lane of the wave joins the march (wave-wide reductions)
+                const bool ok = inside && ray_setup(A, fj, fi, (double)PAD, r);
+                acc += march_padded_sync(rs, r, ok, A.n[0] / 2, A.n[1], A.n[2], st_m, st_1);
+            } else if (ray_setup(A, fj, fi, (double)PAD, r)) {
                 acc += PIPE == 2   ? march_padded_pipe4(rs, r, st_m, st_1)
                        : PIPE == 1 ? march_padded_pipe(rs, r, st_m, st_1)
                                    : march_padded(rs, r, st_m, st_1);
+            }
         }
     }
+    if (!inside) return;
     if (SS > 1) acc *= 1.0f / (float)(SS * SS);
     emit(v.proj + (int64_t)b * v.proj_stride, ((int64_t)i * v.n_angles + a) * v.cols + j, b, acc,
          v.ep);

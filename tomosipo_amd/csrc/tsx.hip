@@ -282,12 +282,13 @@ struct DeviceGuard {
 };
 
 // Kernel variant of the padded forward projector: 0 = plain loop,
-// 1 = two samples in flight, 2 = four samples in flight (default).
+// 1 = two samples in flight, 2 = four samples in flight, 3 = four in flight
+// with the lanes of a wave marching the same slices (default).
 // TSX_FP_VARIANT overrides.
 int fp_variant()
 {
     const char *e = std::getenv("TSX_FP_VARIANT");
-    return e ? std::atoi(e) : 2;
+    return e ? std::atoi(e) : 3;
 }
 
 // z-extent of the separable backprojector's bricks: 32 (default) or 16.
@@ -469,7 +470,9 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     hipLaunchKernelGGL((fp_padded<K, V>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream, v, n_blocks)
 #define TSX_FP_CASE(K)                                                                             \
     case K:                                                                                        \
-        if (variant == 2)                                                                          \
+        if (variant == 3)                                                                          \
+            TSX_FP_LAUNCH(K, 3);                                                                   \
+        else if (variant == 2)                                                                     \
             TSX_FP_LAUNCH(K, 2);                                                                   \
         else if (variant == 1)                                                                     \
             TSX_FP_LAUNCH(K, 1);                                                                   \
@@ -641,7 +644,9 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
         const unsigned grid = (unsigned)(((m.begin[m.n] + 7) / 8) * 8);
-        if (variant == 1)
+        if (variant == 3)
+            hipLaunchKernelGGL(fp_padded_multi<3>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
+        else if (variant == 1)
             hipLaunchKernelGGL(fp_padded_multi<1>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
         else if (variant == 0)
             hipLaunchKernelGGL(fp_padded_multi<0>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
