@@ -95,9 +95,25 @@ typedef struct {
     int32_t axis_count[3];     /* number of angles stepping along x, y, z */
     int32_t fp_kernel;         /* 0 = general, 1 = padded per-axis layouts */
     int32_t bp_kernel;         /* 0 = general, 1 = separable (upright detector) */
+    int32_t fp_variant;        /* padded forward projector: 2 = per-lane slice ranges,
+                                  3 = wave-synchronous slices */
+    int32_t bp_brick_z;        /* z-extent of the separable backprojector's bricks (16 or 32) */
+    int32_t tuned;             /* 1 once tsx_plan_autotune has chosen the variants */
 } tsx_plan_info;
 
 int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);
+
+/* No counterpart in the reference (ASTRA has one kernel per geometry type).
+ * Times the kernel variants of the plan's forward projector and
+ * backprojector on its own shapes (at most 64 evenly spaced angles of its
+ * geometry, on zero-filled scratch buffers allocated and freed on `stream`)
+ * and keeps the fastest for later launches of this plan. Choices are cached
+ * per (type, detector, volume) shape, so a second plan of the same shape
+ * takes them without timing. Synchronises `stream`. Must not run while
+ * launches of the same plan are being issued from another thread. The
+ * TSX_FP_VARIANT and TSX_BP_BRICK_Z environment variables, when set,
+ * override any choice. */
+int tsx_plan_autotune(tsx_plan *plan, void *stream);
 
 /* Dominant (stepping) axis of every angle: 0 = x, 1 = y, 2 = z. */
 int tsx_plan_axes(const tsx_plan *plan, int32_t *axes_out);

@@ -57,6 +57,13 @@ def test_kernel_selection():
     tilted = ts.rotate(pos=0, axis=(0, 1, 1), angles=0.2) * base.to_vec()
     assert _plan(vg, tilted).info()["bp_kernel"] == "general"
     assert _plan(vg, base, det_supersampling=2).info()["bp_kernel"] == "general"
+    # a c1 row of the x-stepping layout of 2048^2 slices is 16.8 MB: beyond the
+    # 24-bit row offsets of the padded kernel, so the general kernel is chosen
+    wide = ts.volume(shape=(8, 2048, 2048), size=(0.004, 1, 1))
+    par = ts.parallel(angles=4, shape=(8, 2048), size=(0.004, 1.5))
+    assert _plan(wide, par).info()["fp_kernel"] == "general"
+    assert _plan(ts.volume(shape=(8, 1024, 1024), size=(0.008, 1, 1)),
+                 ts.parallel(angles=4, shape=(8, 1024), size=(0.008, 1.5))).info()["fp_kernel"] == "padded"
 
 
 @pytest.mark.parametrize("bad", ["nan", "zero_u", "parallel_uv", "zero_ray"])
@@ -92,6 +99,10 @@ def test_compute_on_host_plan_reports_no_device():
     with pytest.raises(_tsx.TsxError) as e:
         p.forward(1, 1, False, None)
     assert e.value.code == -4
+    with pytest.raises(_tsx.TsxError) as e:
+        p.autotune(None)
+    assert e.value.code == -4
+    assert p.info()["tuned"] is False
     lib = _tsx.load()
     assert lib.tsx_direct_fpbp3d(p.handle, None, None, 1, 7, None) != 0
 

@@ -269,3 +269,51 @@ def test_fused_multi_instance_equals_single():
     bs = fused_backward(ops, ys)
     for A, y, b in zip(ops, ys, bs):
         assert torch.equal(b, A.T(y))
+
+
+def test_autotune_keeps_parity_and_caches_choices():
+    """Autotuning picks kernel variants per plan; results stay within the
+    oracle tolerance, a second plan of the same shape takes the cached
+    choices, and fused launches over differently tuned plans still equal
+    per-operator calls."""
+    from tomosipo_amd.fused import fused_forward, fused_backward
+
+    vg = ts.volume(shape=(48, 64, 56), size=1)
+    pg = _cone_vec_helix(40, 48, 72, 1.4, 2.0, 0.1, 1e-3, seed=3)
+    x = ts.phantom.ellipsoids(64, 6, seed=3)[:48, :64, :56].copy()
+    A = ts.operator(vg, pg)
+    assert not A.projector.autotune           # small: not tuned automatically
+    xd = torch.from_numpy(x).to(DEV)
+    y0 = A(xd)
+    b0 = A.T(y0)
+    plan = A.projector.plan(0)
+    assert plan.info()["tuned"] is False
+    plan.autotune(torch.cuda.current_stream().cuda_stream)
+    info = plan.info()
+    print(f"[autotune] {info}", flush=True)
+    assert info["tuned"] is True
+    assert info["fp_variant"] in (2, 3)
+    assert info["bp_brick_z"] in (16, 32)
+    y1 = A(xd)
+    b1 = A.T(y0)
+    g = oracle.oracle_geometry(vg, pg)
+    assert rel_l2(y1.cpu().numpy(), oracle.fp(g, x)) <= TOL
+    assert rel_l2(b1.cpu().numpy(), oracle.bp(g, y0.cpu().numpy())) <= TOL
+    assert rel_l2(y1.cpu().numpy(), y0.cpu().numpy()) <= 1e-6
+    assert rel_l2(b1.cpu().numpy(), b0.cpu().numpy()) <= 1e-6
+    # same shape, new plan: cached choices, no timing
+    B = ts.operator(vg, _cone_vec_helix(40, 48, 72, 1.4, 2.0, 0.1, 1e-3, seed=4))
+    pb = B.projector.plan(0)
+    pb.autotune(torch.cuda.current_stream().cuda_stream)
+    ib = pb.info()
+    assert (ib["fp_variant"], ib["bp_brick_z"]) == (info["fp_variant"], info["bp_brick_z"])
+    # fused over a tuned and an untuned plan
+    C = ts.operator(vg, _cone_vec_helix(40, 48, 72, 1.4, 2.0, 0.1, 1e-3, seed=5))
+    ops = [A, C, B]
+    xs = [xd, xd * 0.5, xd + 1.0]
+    ys = fused_forward(ops, xs)
+    for op, xi, yi in zip(ops, xs, ys):
+        assert torch.equal(yi, op(xi))
+    bs = fused_backward(ops, ys)
+    for op, yi, bi in zip(ops, ys, bs):
+        assert torch.equal(bi, op.T(yi))

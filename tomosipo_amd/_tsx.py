@@ -35,6 +35,7 @@ _STATUS = {
 
 EXPORTED = (
     "tsx_plan_create", "tsx_plan_destroy", "tsx_plan_info_get", "tsx_plan_axes",
+    "tsx_plan_autotune",
     "tsx_workspace_size",
     "tsx_direct_fpbp3d", "tsx_forward", "tsx_backward", "tsx_forward_batched",
     "tsx_backward_batched", "tsx_multi_workspace_size", "tsx_multi_forward",
@@ -78,6 +79,9 @@ class TsxPlanInfo(ctypes.Structure):
         ("axis_count", ctypes.c_int32 * 3),
         ("fp_kernel", ctypes.c_int32),
         ("bp_kernel", ctypes.c_int32),
+        ("fp_variant", ctypes.c_int32),
+        ("bp_brick_z", ctypes.c_int32),
+        ("tuned", ctypes.c_int32),
     ]
 
 
@@ -104,6 +108,7 @@ def load():
         lib.tsx_plan_destroy.argtypes = [P]
         lib.tsx_plan_info_get.argtypes = [P, ctypes.POINTER(TsxPlanInfo)]
         lib.tsx_plan_axes.argtypes = [P, P]
+        lib.tsx_plan_autotune.argtypes = [P, P]
         lib.tsx_direct_fpbp3d.argtypes = [P, P, P, i32, i32, P]
         lib.tsx_forward.argtypes = [P, P, P, i32, P]
         lib.tsx_backward.argtypes = [P, P, P, i32, P]
@@ -195,7 +200,14 @@ class Plan:
             "plan_bytes": inf.plan_bytes, "axis_count": tuple(inf.axis_count),
             "fp_kernel": ("general", "padded")[inf.fp_kernel],
             "bp_kernel": ("general", "separable")[inf.bp_kernel],
+            "fp_variant": inf.fp_variant, "bp_brick_z": inf.bp_brick_z,
+            "tuned": bool(inf.tuned),
         }
+
+    def autotune(self, stream):
+        """Time the kernel variants on this plan's shapes and keep the fastest
+        (``tsx_plan_autotune``; synchronises ``stream``)."""
+        _check(self._lib.tsx_plan_autotune(self._h, stream))
 
     def axes(self):
         n = self.info()["n_angles"]

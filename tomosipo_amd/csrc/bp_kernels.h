@@ -16,9 +16,13 @@
 //   so per slice n
 //     x[n, kz, kc2] = sum_j L(Q_c2(j; n) - kc2) * u[n, j, kz],
 //     u[n, j, kz]   = sum_i L(A(j, n) + i B(j, n) - kz) * ly[i, j],
-//   with ly = l * y precomputed. A block owns a 16^3 brick; per angle it
-//   builds u for its 16 slices in LDS (pass 1, gathers from ly) and then
-//   every thread folds u into the 16 voxels of its z-column (pass 2). No
+//   with ly = l * y precomputed. A block owns a brick of 16 x 16 columns by
+//   BZ voxels along z; per angle it builds u for its 16 slices in LDS (pass
+//   1: thread (slice, column) splats its detector column onto the z-grid),
+//   stores the columns of a slice in increasing order of Q_c2, and then every
+//   thread folds exactly the columns with |Q_c2 - kc2| < 1 into the BZ voxels
+//   of its z-column (pass 2; the range is found by counting). The slice
+//   ranges of the next angle are computed during the current one. No
 //   atomics: the result is deterministic.
 #pragma once
 
@@ -28,7 +32,7 @@
 namespace tsx {
 
 constexpr int BP_BX = 64, BP_BY = 4;
-constexpr int BR = 16;  // brick edge of bp_separable
+constexpr int BR = 16;  // brick edge (x, y) of bp_separable
 
 __device__ __forceinline__ int sel3(int axis, int x, int y, int z)
 {
@@ -179,20 +183,6 @@ __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_an
     out[(int64_t)b * out_stride + o] = len * proj[(int64_t)b * proj_stride + o];
 }
 
-// Column j of slice s keeps u at (BZ + 8)-float stride; kz = -1 .. BZ live at
-// UOFF - 1 .. UOFF + BZ so that the splat needs no range test beyond a clamp
-// and the BZ voxel entries are 16-byte aligned.
-constexpr int UOFF = 4;
-
-template <int BZ>
-struct BpShared {
-    static constexpr int UCOL = BZ + 8;
-    float u[BR][BR][UCOL];  // [slice s][column jj][UOFF + kz]
-    float q2[BR][BR];       // Q_c2 - c0 of column jj in slice s (1e30 if unused)
-    int jlo[BR];
-    int cnt[BR];
-};
-
 // Detector column where the ray through in-slice point (n, c) (axes m, c2)
 // meets the detector. Independent of z for an upright detector.
 __device__ __forceinline__ double column_of(const AngleParams &A, double n, double c)
@@ -217,11 +207,47 @@ struct BpView {
     Epilogue ep;
 };
 
-// Work item `logical` of one instance: (batch entry, brick), a brick being
-// BR x BR columns (x, y) by BZ voxels along z.
+// LDS layout of bp_separable: a column of u holds kz = -1 .. BZ; columns are
+// BZ + 4 floats apart and slices BR * (BZ + 4) + 4 floats apart, so that the
+// columns of different slices start in different banks (slice stride = 4 mod
+// 64 floats) while every column stays 16-byte aligned for float4 reads.
+constexpr int UOFF = 4;  // kz = 0 sits at offset 4 of a column; kz = -1 at 3
+
+template <int BZ>
+struct BpShared {
+    // A column is BZ + 4 floats; its kz = BZ guard falls on offset 0 of the
+    // next column, which no column uses (kz = -1 is at offset 3).
+    static constexpr int UCOL = BZ + 4;
+    static constexpr int SLICE = BR * UCOL + 4;
+    static_assert(UCOL % 4 == 0 && SLICE % 64 == 4, "LDS strides");
+    float u[BR * SLICE];
+    float q2[BR][20];     // Q_c2 - c0 per stored column, increasing; 1e30 if unused
+    int jlo[2][BR];       // first detector column of each slice (double-buffered by angle)
+    int cnt[2][BR];       // number of columns of each slice
+    int inc[2][BR];       // 1 if Q_c2 increases with the detector column
+};
+
+// Slice range of slice t of a brick (first column, count, direction).
+__device__ __forceinline__ void bp_slice_range(const AngleParams &A, int bx, int by, int cols, int t,
+                                                int &jlo, int &cnt, int &inc)
+{
+    const bool mx = A.m == 0;
+    const int m0 = mx ? bx * BR : by * BR;
+    const int c0 = mx ? by * BR : bx * BR;
+    const double n = (double)(m0 + t);
+    const double ja = column_of(A, n, (double)c0 - 1.0);
+    const double jb = column_of(A, n, (double)(c0 + BR));
+    const int lo = max(0, (int)ceil(fmin(ja, jb) - 1e-3));
+    const int hi = min(cols - 1, (int)floor(fmax(ja, jb) + 1e-3));
+    jlo = lo;
+    cnt = max(0, hi - lo + 1);
+    inc = ja <= jb;
+}
+
 template <int BZ>
 __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, BpShared<BZ> &sh)
 {
+    using S = BpShared<BZ>;
     const int n_angles = v.n_angles, rows = v.rows, cols = v.cols;
     const int bx = logical % v.bricks_x;
     const int by = (logical / v.bricks_x) % v.bricks_y;
@@ -229,46 +255,40 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
     const int b = logical / (v.bricks_x * v.bricks_y * v.bricks_z);
     if (b >= v.batch) return;
     const AngleParams *__restrict__ angles = v.angles;
-    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * BR + tx;
+    const int tx = threadIdx.x, ty = threadIdx.y;
     const int x = bx * BR + tx, y = by * BR + ty, z0 = bz * BZ;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)(v.ly + (int64_t)b * ((int64_t)v.ly_bytes / 4)), (short)0, (int)v.ly_bytes,
-        0x00020000);
+        (void *)uniform_ptr(v.ly + (int64_t)b * ((int64_t)v.ly_bytes / 4)), (short)0,
+        (int)__builtin_amdgcn_readfirstlane(v.ly_bytes), 0x00020000);
     const uint32_t row_bytes = (uint32_t)n_angles * (uint32_t)cols * 4u;
     float acc[BZ];
 #pragma unroll
     for (int k = 0; k < BZ; ++k) acc[k] = 0.0f;
+    // slice ranges of angle 0; thread (ty, 0) handles slice ty
+    if (tx == 0) bp_slice_range(angles[0], bx, by, cols, ty, sh.jlo[0][ty], sh.cnt[0][ty], sh.inc[0][ty]);
+    __syncthreads();
     for (int a = 0; a < n_angles; ++a) {
+        const int buf = a & 1;
         const AngleParams &A = angles[a];
         const bool mx = A.m == 0;
         const int m0 = mx ? bx * BR : by * BR;
         const int c0 = mx ? by * BR : bx * BR;
-        // slice ranges: columns j whose Q_c2 lies in (c0 - 1, c0 + BR)
-        if (t < BR) {
-            const double n = (double)(m0 + t);
-            const double ja = column_of(A, n, (double)c0 - 1.0);
-            const double jb = column_of(A, n, (double)(c0 + BR));
-            const int lo = max(0, (int)ceil(fmin(ja, jb) - 1e-3));
-            const int hi = min(cols - 1, (int)floor(fmax(ja, jb) + 1e-3));
-            sh.jlo[t] = lo;
-            sh.cnt[t] = max(0, hi - lo + 1);
-        }
-        __syncthreads();
         int chunks = 0;
 #pragma unroll
-        for (int s = 0; s < BR; ++s) chunks = max(chunks, (sh.cnt[s] + BR - 1) / BR);
+        for (int s = 0; s < BR; ++s) chunks = max(chunks, (sh.cnt[buf][s] + BR - 1) / BR);
         for (int c = 0; c < chunks; ++c) {
-            // pass 1: thread (s = ty, jj = tx) splats detector column j onto kz:
-            // row i lands at z-offset t_i = Q_z(i) - z0 and adds (1 - frac) to
-            // floor(t_i) and frac to floor(t_i) + 1 of its own LDS column.
+            // pass 1: thread (s = ty, jj = tx) splats detector column j onto kz
             {
                 const int s = ty, jj = tx;
-                const int j = sh.jlo[s] + c * BR + jj;
-                float *u = &sh.u[s][jj][UOFF];
-                float4 *u4 = reinterpret_cast<float4 *>(&sh.u[s][jj][0]);
+                const int m = min(BR, sh.cnt[buf][s] - c * BR);  // valid columns of this chunk
+                const bool valid = jj < m;
+                const int pos = (!valid || sh.inc[buf][s]) ? jj : m - 1 - jj;
+                const int j = sh.jlo[buf][s] + c * BR + jj;
+                float *col = &sh.u[s * S::SLICE + pos * S::UCOL];
+                float4 *c4 = reinterpret_cast<float4 *>(col);
 #pragma unroll
-                for (int k = 0; k < BpShared<BZ>::UCOL / 4; ++k) u4[k] = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (c * BR + jj < sh.cnt[s]) {
+                for (int k = 0; k < S::UCOL / 4; ++k) c4[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (valid) {
                     const double n = (double)(m0 + s);
                     const double Pm = A.P0[0] + j * A.U[0];
                     const double Pz = A.P0[1] + j * A.U[1];
@@ -285,7 +305,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                         Bz = A.V[1];
                         Q2 = P2 + tau * A.S[2];
                     }
-                    sh.q2[s][jj] = (float)(Q2 - (double)c0);
+                    sh.q2[s][pos] = (float)(Q2 - (double)c0);
                     // rows with t_i = Arel + i * Bz in (-1, BZ)
                     const double Arel = Az - (double)z0;
                     const double iBz = 1.0 / Bz;
@@ -295,9 +315,10 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                     const float t0 = (float)(Arel + (double)i0 * Bz);
                     const float fB = (float)Bz;
                     uint32_t off = (uint32_t)i0 * row_bytes + ((uint32_t)a * (uint32_t)cols + (uint32_t)j) * 4u;
-                    // four rows per step: their loads are in flight together; rows past
-                    // i1 are read (inside the buffer or zero by the range check) and
-                    // given weight zero
+                    float *u = col + UOFF;
+                    // four rows per step, their loads in flight together; rows
+                    // past i1 are read (inside the buffer or zero by the range
+                    // check) and given weight zero
                     float tb = t0;
                     for (int i = i0; i <= i1; i += 4) {
                         float yv[4];
@@ -318,31 +339,43 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                         tb = fmaf(4.0f, fB, tb);
                     }
                 } else {
-                    sh.q2[s][jj] = 1e30f;
+                    sh.q2[s][pos] = 1e30f;
                 }
             }
+            // slice ranges of the next angle (the buffer was last read in the
+            // previous angle, before its final barrier)
+            if (c == 0 && tx == 0 && a + 1 < n_angles)
+                bp_slice_range(angles[a + 1], bx, by, cols, ty, sh.jlo[buf ^ 1][ty],
+                                sh.cnt[buf ^ 1][ty], sh.inc[buf ^ 1][ty]);
             __syncthreads();
-            // pass 2: fold the columns into this thread's z-column
+            // pass 2: the stored columns with |Q_c2 - kc| < 1 form the range
+            // [start, end) of the increasing sequence q2[s][.]
             {
                 const int s = mx ? tx : ty;
                 const float kc = (float)(mx ? ty : tx);
-#pragma unroll 1
-                for (int jj = 0; jj < BR; ++jj) {
-                    const float w = 1.0f - fabsf(sh.q2[s][jj] - kc);
-                    if (w > 0.0f) {
-                        const float4 *u4 = reinterpret_cast<const float4 *>(&sh.u[s][jj][UOFF]);
+                const float4 *q4 = reinterpret_cast<const float4 *>(sh.q2[s]);
+                int start = 0, end = 0;
 #pragma unroll
-                        for (int q = 0; q < BZ / 4; ++q) {
-                            const float4 v = u4[q];
-                            acc[4 * q + 0] = fmaf(w, v.x, acc[4 * q + 0]);
-                            acc[4 * q + 1] = fmaf(w, v.y, acc[4 * q + 1]);
-                            acc[4 * q + 2] = fmaf(w, v.z, acc[4 * q + 2]);
-                            acc[4 * q + 3] = fmaf(w, v.w, acc[4 * q + 3]);
-                        }
+                for (int k = 0; k < BR / 4; ++k) {
+                    const float4 q = q4[k];
+                    start += (q.x <= kc - 1.0f) + (q.y <= kc - 1.0f) + (q.z <= kc - 1.0f) + (q.w <= kc - 1.0f);
+                    end += (q.x < kc + 1.0f) + (q.y < kc + 1.0f) + (q.z < kc + 1.0f) + (q.w < kc + 1.0f);
+                }
+                const float *us = &sh.u[s * S::SLICE + UOFF];
+                for (int pos = start; pos < end; ++pos) {
+                    const float w = 1.0f - fabsf(sh.q2[s][pos] - kc);
+                    const float4 *u4 = reinterpret_cast<const float4 *>(us + pos * S::UCOL);
+#pragma unroll
+                    for (int q = 0; q < BZ / 4; ++q) {
+                        const float4 uv = u4[q];
+                        acc[4 * q + 0] = fmaf(w, uv.x, acc[4 * q + 0]);
+                        acc[4 * q + 1] = fmaf(w, uv.y, acc[4 * q + 1]);
+                        acc[4 * q + 2] = fmaf(w, uv.z, acc[4 * q + 2]);
+                        acc[4 * q + 3] = fmaf(w, uv.w, acc[4 * q + 3]);
                     }
                 }
             }
-            __syncthreads();  // u is rewritten by the next chunk or angle
+            __syncthreads();  // u and q2 are rewritten by the next chunk or angle
         }
     }
     if (x < v.nx && y < v.ny) {
@@ -358,7 +391,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
 template <int BZ>
 __global__ void __launch_bounds__(BR *BR) bp_separable(BpView v, int n_blocks)
 {
-    __shared__ BpShared<BZ> sh;
+    __shared__ __align__(16) BpShared<BZ> sh;
     const int logical = xcd_logical(n_blocks);
     if (logical >= n_blocks) return;
     bp_separable_work<BZ>(v, logical, sh);
@@ -373,7 +406,7 @@ struct BpMulti {
 template <int BZ>
 __global__ void __launch_bounds__(BR *BR) bp_separable_multi(BpMulti m)
 {
-    __shared__ BpShared<BZ> sh;
+    __shared__ __align__(16) BpShared<BZ> sh;
     const int total = m.begin[m.n];
     const int logical = xcd_logical(total);
     if (logical >= total) return;

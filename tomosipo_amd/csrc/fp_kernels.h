@@ -209,6 +209,19 @@ __global__ void pad_transpose_x(const float *__restrict__ src, float *__restrict
     }
 }
 
+// A block-uniform pointer moved to scalar registers. The compiler cannot
+// always prove that a pointer derived from the block index is uniform; a
+// buffer descriptor built from a pointer it keeps in vector registers turns
+// every buffer load into a loop over distinct descriptors.
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p)
+{
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T *)(((uint64_t)hi << 32) | lo);
+}
+
 __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, uint32_t byte_off)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, byte_off, 0, 0));
@@ -220,43 +233,6 @@ struct Layouts {
     int64_t stride;        // elements between batch entries of the same layout
 };
 
-// Marches one ray through a padded layout. Q and k are in padded coordinates.
-__device__ __forceinline__ float march_padded(__amdgpu_buffer_rsrc_t rs, const RaySetup &r,
-                                              uint32_t st_m, uint32_t st_1)
-{
-    float dn = (float)(r.nlo - r.nref);
-    uint32_t row = (uint32_t)r.nlo * st_m;
-    float total = 0.0f;
-    int n = r.nlo;
-    const uint32_t rowb = st_1 * 4u;
-    while (n <= r.nhi) {
-        const int stop = min(r.nhi, n + 15);
-        float part = 0.0f;
-        for (; n <= stop; ++n) {
-            const float q1 = fmaf(dn, r.a1, r.R1);
-            const float q2 = fmaf(dn, r.a2, r.R2);
-            // The slice range keeps Q within [-1, n_c] (up to rounding), inside the
-            // zero border; any out-of-range offset is caught by the buffer range check.
-            const int k1 = __float2int_rd(q1);
-            const int k2 = __float2int_rd(q2);
-            const float w1 = __builtin_amdgcn_fractf(q1);
-            const float w2 = __builtin_amdgcn_fractf(q2);
-            const uint32_t off = (row + __umul24((uint32_t)k1, st_1) + (uint32_t)k2) * 4u;
-            const float v00 = bload(rs, off);
-            const float v01 = bload(rs, off + 4u);
-            const float v10 = bload(rs, off + rowb);
-            const float v11 = bload(rs, off + rowb + 4u);
-            const float lo = fmaf(w2, v01 - v00, v00);
-            const float hi = fmaf(w2, v11 - v10, v10);
-            part += fmaf(w1, hi - lo, lo);
-            dn += 1.0f;
-            row += st_m;
-        }
-        total += part;
-    }
-    return r.len * total;
-}
-
 __device__ __forceinline__ int floor_to_int(float q)
 {
     int k;
@@ -264,12 +240,7 @@ __device__ __forceinline__ int floor_to_int(float q)
     return k;
 }
 
-// Same sampling as march_padded, software-pipelined and unrolled by two with
-// two register sets: the corners of slice n + 1 are requested before slice n is
-// interpolated, so two samples' loads are in flight per lane. The request past
-// the last slice of the ray reads at most one slice beyond its range, which is
-// inside the layout or returns 0 through the buffer range check, and is not
-// used.
+// Corners of one bilinear sample and its two in-slice weights.
 struct Corners {
     float v00, v01, v10, v11, w1, w2;
 };
@@ -295,32 +266,12 @@ __device__ __forceinline__ float interp(const Corners &c)
     return fmaf(c.w1, hi - lo, lo);
 }
 
-__device__ __forceinline__ float march_padded_pipe(__amdgpu_buffer_rsrc_t rs, const RaySetup &r,
-                                                   uint32_t st_m, uint32_t st_1)
-{
-    const uint32_t rowb = st_1 * 4u, stepb = st_m * 4u;
-    float dn = (float)(r.nlo - r.nref);
-    uint32_t rowoff = (uint32_t)r.nlo * stepb;
-    Corners A, B;
-    request(rs, r, dn, rowoff, rowb, A);
-    float sa = 0.0f, sb = 0.0f;
-    const int cnt = r.nhi - r.nlo + 1;
-    const int pairs = (cnt + 1) >> 1;
-    for (int p = 0; p < pairs; ++p) {
-        request(rs, r, dn + 1.0f, rowoff + stepb, rowb, B);
-        sa += interp(A);
-        request(rs, r, dn + 2.0f, rowoff + 2u * stepb, rowb, A);
-        const float vb = interp(B);
-        sb += (2 * p + 1 < cnt) ? vb : 0.0f;
-        dn += 2.0f;
-        rowoff += 2u * stepb;
-    }
-    return r.len * (sa + sb);
-}
-
-// Four samples in flight per lane: ring of four corner sets, unrolled so
-// that no register copies are needed. Requests run up to three slices past
-// the ray's range (discarded; in-layout or zero by the buffer range check).
+// Marches one ray through a padded layout (Q and k in padded coordinates)
+// over the slices where it has support, with four samples in flight per lane:
+// a ring of four corner sets, unrolled so that no register copies are needed.
+// The slice range keeps Q within [-1, n_c] (up to rounding), inside the zero
+// border. Requests run up to three slices past the ray's range (discarded;
+// in-layout or zero by the buffer range check).
 __device__ __forceinline__ float march_padded_pipe4(__amdgpu_buffer_rsrc_t rs, const RaySetup &r,
                                                     uint32_t st_m, uint32_t st_1)
 {
@@ -467,7 +418,7 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
     const bool inside = j < v.cols && i < v.rows;
     if (PIPE != 3 && !inside) return;
     const AngleParams &A = v.angles[a];
-    const float *lp = v.L.ptr[A.m] + (int64_t)b * v.L.stride;
+    const float *lp = uniform_ptr(v.L.ptr[A.m] + (int64_t)b * v.L.stride);
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void *)lp, (short)0, (int)v.L.bytes[A.m], 0x00020000);
     const uint32_t st_m = (uint32_t)A.pst[0], st_1 = (uint32_t)A.pst[1];
@@ -484,9 +435,7 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
                 const bool ok = inside && ray_setup(A, fj, fi, (double)PAD, r);
                 acc += march_padded_sync(rs, r, ok, A.n[0] / 2, A.n[1], A.n[2], st_m, st_1);
             } else if (ray_setup(A, fj, fi, (double)PAD, r)) {
-                acc += PIPE == 2   ? march_padded_pipe4(rs, r, st_m, st_1)
-                       : PIPE == 1 ? march_padded_pipe(rs, r, st_m, st_1)
-                                   : march_padded(rs, r, st_m, st_1);
+                acc += march_padded_pipe4(rs, r, st_m, st_1);
             }
         }
     }

@@ -18,6 +18,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -61,6 +63,8 @@ struct tsx_plan {
     bool fp_padded = false;   // all used layouts addressable with 32-bit byte offsets
     bool bp_separable = false;  // upright detector, x/y stepping, ss == 1
     std::string bp_reason;      // why bp_separable is off
+    // kernel variants chosen by tsx_plan_autotune (-1: library default)
+    int32_t fp_variant = -1, bp_bz = -1;
 };
 
 // ---------------------------------------------------------------------------
@@ -211,8 +215,14 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
     p->layout_elems[1] = (n3[2] + P2) * n3[1] * (n3[0] + P2);
     p->layout_elems[2] = n3[2] * (n3[1] + P2) * (n3[0] + P2);
     p->fp_padded = true;
-    for (int c = 0; c < 3; ++c)
-        if (p->axis_count[c] > 0 && p->layout_elems[c] * 4 >= ((int64_t)1 << 32)) p->fp_padded = false;
+    // elements between rows of c1 in each layout; the march forms row offsets
+    // with 24-bit multiplies, so a row must stay below 2^24 bytes
+    const int64_t row1[3] = {n3[0] * (n3[1] + P2), n3[1] * (n3[0] + P2), n3[0] + P2};
+    for (int c = 0; c < 3; ++c) {
+        if (p->axis_count[c] == 0) continue;
+        if (p->layout_elems[c] * 4 >= ((int64_t)1 << 32) || row1[c] * 4 >= ((int64_t)1 << 24))
+            p->fp_padded = false;
+    }
     // separable backprojection: upright detector (v along z), stepping along x or y
     p->bp_separable = g->det_supersampling == 1;
     if (!p->bp_separable) p->bp_reason = "detector supersampling";
@@ -281,22 +291,31 @@ struct DeviceGuard {
     }
 };
 
-// Kernel variant of the padded forward projector: 0 = plain loop,
-// 1 = two samples in flight, 2 = four samples in flight, 3 = four in flight
-// with the lanes of a wave marching the same slices (default).
-// TSX_FP_VARIANT overrides.
-int fp_variant()
+// Integer value of an environment variable, or -1 when it is not set.
+int env_int(const char *name)
 {
-    const char *e = std::getenv("TSX_FP_VARIANT");
-    return e ? std::atoi(e) : 3;
+    const char *e = std::getenv(name);
+    return (e && e[0]) ? std::atoi(e) : -1;
+}
+
+// Kernel variant of the padded forward projector: 2 = each lane marches its
+// own ray's slice range, four samples in flight; 3 = the lanes of a wave march
+// the same slices (coalesced reads; default). TSX_FP_VARIANT overrides the
+// plan's choice.
+int fp_variant_of(const tsx_plan *p)
+{
+    const int e = env_int("TSX_FP_VARIANT");
+    if (e == 2 || e == 3) return e;
+    return p->fp_variant >= 0 ? p->fp_variant : 3;
 }
 
 // z-extent of the separable backprojector's bricks: 32 (default) or 16.
-// TSX_BP_BRICK_Z overrides.
-int bp_brick_z()
+// TSX_BP_BRICK_Z overrides the plan's choice.
+int bp_brick_z_of(const tsx_plan *p)
 {
-    const char *e = std::getenv("TSX_BP_BRICK_Z");
-    return (e && std::atoi(e) == 16) ? 16 : 32;
+    const int e = env_int("TSX_BP_BRICK_Z");
+    if (e == 16 || e == 32) return e;
+    return p->bp_bz > 0 ? p->bp_bz : 32;
 }
 
 bool force_general()
@@ -403,6 +422,7 @@ void build_layouts(const tsx_plan *p, const float *src, float *dst, const int64_
     }
 }
 
+// Offsets (in floats) of the layouts of one volume, and their total.
 void layout_offsets(const tsx_plan *p, int64_t offs[3], int64_t *per_entry)
 {
     int64_t e = 0;
@@ -432,11 +452,7 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     // padded copies of every batch entry for each stepping axis in use
     int64_t per_entry = 0;
     int64_t offs[3] = {0, 0, 0};
-    for (int c = 0; c < 3; ++c)
-        if (p->axis_count[c] > 0) {
-            offs[c] = per_entry;
-            per_entry += p->layout_elems[c];
-        }
+    layout_offsets(p, offs, &per_entry);
     Scratch owned;
     void *wptr = nullptr;
     int rc = get_workspace(p, per_entry * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
@@ -464,7 +480,7 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     if (nb > (int64_t)0x7fffffff - 8) return fail(TSX_ERR_UNSUPPORTED, "forward projection too large for one launch");
     const int n_blocks = (int)nb;
     const unsigned grid = (unsigned)(((n_blocks + 7) / 8) * 8);
-    const int variant = fp_variant();
+    const int variant = fp_variant_of(p);
     switch (p->ss) {
 #define TSX_FP_LAUNCH(K, V)                                                                        \
     hipLaunchKernelGGL((fp_padded<K, V>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream, v, n_blocks)
@@ -472,12 +488,8 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     case K:                                                                                        \
         if (variant == 3)                                                                          \
             TSX_FP_LAUNCH(K, 3);                                                                   \
-        else if (variant == 2)                                                                     \
-            TSX_FP_LAUNCH(K, 2);                                                                   \
-        else if (variant == 1)                                                                     \
-            TSX_FP_LAUNCH(K, 1);                                                                   \
         else                                                                                       \
-            TSX_FP_LAUNCH(K, 0);                                                                   \
+            TSX_FP_LAUNCH(K, 2);                                                                   \
         break;
         TSX_FP_CASE(1)
         TSX_FP_CASE(2)
@@ -548,7 +560,7 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
         hipLaunchKernelGGL(scale_by_length, grid, dim3(256), 0, stream, p->dev, p->n_angles, p->rows,
                            p->cols, proj, ps, ly, sino);
     }
-    const int BZ = bp_brick_z();
+    const int BZ = bp_brick_z_of(p);
     BpView v = bp_view(p, BZ, vol, vs, ly, (uint32_t)(sino * 4), (int)batch, ep);
     const int64_t nb = (int64_t)v.bricks_x * v.bricks_y * v.bricks_z * batch;
     if (nb > (int64_t)0x7fffffff - 8) return fail(TSX_ERR_UNSUPPORTED, "backprojection too large for one launch");
@@ -640,16 +652,13 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
     m.n = 0;
     m.begin[0] = 0;
     int64_t woff = 0;
-    const int variant = fp_variant();
+    // one fused launch per run of instances with the same kernel variant
+    int variant = -1;
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
         const unsigned grid = (unsigned)(((m.begin[m.n] + 7) / 8) * 8);
         if (variant == 3)
             hipLaunchKernelGGL(fp_padded_multi<3>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
-        else if (variant == 1)
-            hipLaunchKernelGGL(fp_padded_multi<1>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
-        else if (variant == 0)
-            hipLaunchKernelGGL(fp_padded_multi<0>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
         else
             hipLaunchKernelGGL(fp_padded_multi<2>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
         TSX_HIP_CHECK(hipGetLastError());
@@ -667,6 +676,11 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
             woff += align_up(workspace_bytes(p, TSX_OP_FP, 1));
             continue;
         }
+        if (m.n > 0 && fp_variant_of(p) != variant) {
+            int rc = flush();
+            if (rc) return rc;
+        }
+        variant = fp_variant_of(p);
         int64_t offs[3], per_entry;
         layout_offsets(p, offs, &per_entry);
         float *base = (float *)((char *)wptr + woff);
@@ -717,7 +731,8 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
     }
     const Epilogue ep = plain(additive);
     const bool fast = !force_general();
-    const int BZ = bp_brick_z();
+    // one fused launch per run of instances with the same brick depth
+    int BZ = 32;
     BpMulti m;
     m.n = 0;
     m.begin[0] = 0;
@@ -744,6 +759,11 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
             woff += wb;
             continue;
         }
+        if (m.n > 0 && bp_brick_z_of(p) != BZ) {
+            int rc = flush();
+            if (rc) return rc;
+        }
+        BZ = bp_brick_z_of(p);
         float *ly = (float *)((char *)wptr + woff);
         woff += wb;
         const int64_t sino = bp_workspace_elems(p);
@@ -766,6 +786,143 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
     }
     TSX_HIP_CHECK(hipGetLastError());
     return flush();
+}
+
+// ---------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------
+struct TuneChoice {
+    int fp_variant, bp_bz;
+};
+std::mutex g_tune_mu;
+std::map<std::string, TuneChoice> g_tune_cache;
+
+std::string tune_key(const tsx_plan *p)
+{
+    return std::to_string(p->device) + ":" + std::to_string(p->host.empty() ? 0 : p->host[0].cone) +
+           ":" + std::to_string(p->rows) + "x" + std::to_string(p->cols) + ":" + std::to_string(p->nx) +
+           "x" + std::to_string(p->ny) + "x" + std::to_string(p->nz) + ":" + std::to_string(p->ss) + ":" +
+           std::to_string((int)p->fp_padded) + std::to_string((int)p->bp_separable);
+}
+
+// Mean time of `reps` launches after one warm-up launch, in milliseconds.
+template <typename F>
+int time_launches(F &&launch, hipStream_t stream, hipEvent_t e0, hipEvent_t e1, int reps, float *ms)
+{
+    int rc = launch();
+    if (rc) return rc;
+    TSX_HIP_CHECK(hipEventRecord(e0, stream));
+    for (int r = 0; r < reps; ++r) {
+        rc = launch();
+        if (rc) return rc;
+    }
+    TSX_HIP_CHECK(hipEventRecord(e1, stream));
+    TSX_HIP_CHECK(hipEventSynchronize(e1));
+    float t = 0.0f;
+    TSX_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / reps;
+    return TSX_OK;
+}
+
+// Owns the device allocations of one autotuning run.
+struct TuneScratch {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    void *vol = nullptr, *proj = nullptr;
+    AngleParams *angles = nullptr;
+    ~TuneScratch()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (vol) (void)hipFree(vol);
+        if (proj) (void)hipFree(proj);
+        if (angles) (void)hipFree(angles);
+    }
+};
+
+int autotune(tsx_plan *p, hipStream_t stream)
+{
+    const std::string key = tune_key(p);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            p->fp_variant = it->second.fp_variant;
+            p->bp_bz = it->second.bp_bz;
+            return TSX_OK;
+        }
+    }
+    // sub-plan over at most 64 evenly spaced angles of the plan's geometry
+    tsx_plan sub;
+    sub.device = p->device;
+    sub.kind = p->kind;
+    sub.rows = p->rows;
+    sub.cols = p->cols;
+    sub.nx = p->nx;
+    sub.ny = p->ny;
+    sub.nz = p->nz;
+    sub.ss = p->ss;
+    const int step = (p->n_angles + 63) / 64;
+    for (int a = 0; a < p->n_angles; a += step) {
+        sub.host.push_back(p->host[a]);
+        sub.axis_count[p->host[a].m] += 1;
+    }
+    sub.n_angles = (int32_t)sub.host.size();
+    for (int c = 0; c < 3; ++c) sub.layout_elems[c] = p->layout_elems[c];
+    sub.fp_padded = p->fp_padded;
+    sub.bp_separable = p->bp_separable;
+    TuneScratch t;
+    const size_t vol_bytes = (size_t)p->nx * p->ny * p->nz * 4;
+    const size_t proj_bytes = (size_t)p->rows * sub.n_angles * p->cols * 4;
+    if (hipMalloc(&t.angles, sizeof(AngleParams) * sub.host.size()) != hipSuccess ||
+        hipMalloc(&t.vol, vol_bytes) != hipSuccess || hipMalloc(&t.proj, proj_bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(TSX_ERR_OUT_OF_MEMORY, "autotune: no device memory for the trial buffers");
+    }
+    sub.dev = t.angles;
+    TSX_HIP_CHECK(hipMemcpyAsync(t.angles, sub.host.data(), sizeof(AngleParams) * sub.host.size(),
+                                 hipMemcpyHostToDevice, stream));
+    TSX_HIP_CHECK(hipMemsetAsync(t.vol, 0, vol_bytes, stream));
+    TSX_HIP_CHECK(hipMemsetAsync(t.proj, 0, proj_bytes, stream));
+    TSX_HIP_CHECK(hipEventCreate(&t.e0));
+    TSX_HIP_CHECK(hipEventCreate(&t.e1));
+    const Epilogue ep = plain(0);
+    float *vol = (float *)t.vol, *proj = (float *)t.proj;
+    TuneChoice best{-1, -1};
+    const bool fp_choices = p->fp_padded && p->ss >= 1 && p->ss <= 4 && !force_general();
+    if (fp_choices) {
+        float best_ms = 1e30f;
+        for (int cand : {3, 2}) {
+            sub.fp_variant = cand;
+            float ms = 0.0f;
+            int rc = time_launches([&] { return launch_fp(&sub, vol, 0, proj, 0, 1, ep, stream); },
+                                   stream, t.e0, t.e1, 2, &ms);
+            if (rc) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.fp_variant = cand;
+            }
+        }
+    }
+    if (p->bp_separable && !force_general()) {
+        float best_ms = 1e30f;
+        for (int bz : {32, 16}) {
+            sub.bp_bz = bz;
+            float ms = 0.0f;
+            int rc = time_launches([&] { return launch_bp(&sub, vol, 0, proj, 0, 1, ep, stream); },
+                                   stream, t.e0, t.e1, 2, &ms);
+            if (rc) return rc;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best.bp_bz = bz;
+            }
+        }
+    }
+    TSX_HIP_CHECK(hipStreamSynchronize(stream));
+    p->fp_variant = best.fp_variant;
+    p->bp_bz = best.bp_bz;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+    return TSX_OK;
 }
 
 }  // namespace
@@ -868,7 +1025,19 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     for (int c = 0; c < 3; ++c) info->axis_count[c] = p->axis_count[c];
     info->fp_kernel = p->fp_padded && !force_general() ? 1 : 0;
     info->bp_kernel = p->bp_separable && !force_general() ? 1 : 0;
+    info->fp_variant = fp_variant_of(p);
+    info->bp_brick_z = bp_brick_z_of(p);
+    info->tuned = p->fp_variant >= 0 || p->bp_bz > 0;
     return TSX_OK;
+}
+
+int tsx_plan_autotune(tsx_plan *p, void *stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    DeviceGuard dg(p->device);
+    if (!dg.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
+    return autotune(p, (hipStream_t)stream);
 }
 
 int tsx_plan_axes(const tsx_plan *p, int32_t *axes)

@@ -25,9 +25,15 @@ from . import _tsx
 from .links.base import are_compatible
 
 
+# Plans with at least this many voxel-angles are autotuned on their device
+# when first used (a one-off cost of a few launches over at most 64 angles,
+# cached per shape by the library).
+AUTOTUNE_MIN_VOXEL_ANGLES = 1 << 30
+
+
 class Projector:
     def __init__(self, volume_geometry, projection_geometry, *, voxel_supersampling=1,
-                 detector_supersampling=1):
+                 detector_supersampling=1, autotune=None):
         from .geometry import VolumeGeometry
 
         if not isinstance(volume_geometry, VolumeGeometry):
@@ -39,6 +45,10 @@ class Projector:
         self.vol_shape = tuple(volume_geometry.shape)
         rows, cols = projection_geometry.det_shape
         self.proj_shape = (rows, projection_geometry.num_angles, cols)
+        if autotune is None:
+            n = int(np.prod(self.vol_shape)) * projection_geometry.num_angles
+            autotune = n >= AUTOTUNE_MIN_VOXEL_ANGLES
+        self.autotune = bool(autotune)
         # Host-only plan: validates now, as astra.create_projector would.
         self._host_plan = self._make_plan(-1)
         self._plans = {}
@@ -57,6 +67,9 @@ class Projector:
                 p = self._plans.get(device)
                 if p is None:
                     p = self._make_plan(device)
+                    if self.autotune:
+                        torch = _torch()
+                        p.autotune(torch.cuda.current_stream(device).cuda_stream)
                     self._plans[device] = p
         return p
 
@@ -65,9 +78,9 @@ class Projector:
 
 
 def create_projector(volume_geometry, projection_geometry, *, voxel_supersampling=1,
-                     detector_supersampling=1):
+                     detector_supersampling=1, autotune=None):
     return Projector(volume_geometry, projection_geometry, voxel_supersampling=voxel_supersampling,
-                     detector_supersampling=detector_supersampling)
+                     detector_supersampling=detector_supersampling, autotune=autotune)
 
 
 def _torch():
