@@ -97,26 +97,70 @@ def cpu_baseline(ts, target_s=12.0):
     }
 
 
-def serving_benchmark(n_requests):
-    """Mixed request stream (Poisson 200 req/s; sides 64-256; 60-720 angles; cone or
-    parallel; FP, BP or 20-iteration SIRT): continuous batching vs one request per launch."""
+def _serve_both(reqs, warm):
     from tomosipo_amd import serving
 
-    reqs = serving.make_stream(n_requests, rate=200.0, seed=7)
     # warm both servers' plans and kernels on a few requests first
-    warm = serving.make_stream(6, rate=1000.0, seed=8, sirt_iterations=1)
     serving.Server().run(warm)
     serving.serve_one_at_a_time(warm)
     batched = serving.Server(hbm_budget_bytes=24 << 30).run(reqs)
     single = serving.serve_one_at_a_time(reqs)
     keys = ("req_per_s", "p50_latency_s", "p99_latency_s", "makespan_s")
     return {
-        "stream": f"{n_requests} requests, Poisson 200 req/s, seed 7",
         "continuous_batching": {k: round(batched[k], 4) for k in keys} |
         {"deferred_admissions": batched["deferred_admissions"]},
         "one_request_per_launch": {k: round(single[k], 4) for k in keys},
         "req_per_s_speedup": round(batched["req_per_s"] / single["req_per_s"], 3),
     }
+
+
+def serving_benchmark(n_requests):
+    """Mixed request stream (Poisson 200 req/s; sides 64-256; 60-720 angles; cone or
+    parallel; FP, BP or 20-iteration SIRT): continuous batching vs one request per launch.
+    A second stream, a burst of small FP/BP requests (sides 64-128, 60-180 angles, all
+    arriving within a few ms), measures capacity where requests do not fill the GPU on
+    their own."""
+    from tomosipo_amd import serving
+
+    out = {"stream": f"{n_requests} requests, Poisson 200 req/s, seed 7"}
+    out |= _serve_both(serving.make_stream(n_requests, rate=200.0, seed=7),
+                       serving.make_stream(6, rate=1000.0, seed=8, sirt_iterations=1))
+    small = dict(sides=(64, 96, 128), angles=(60, 180), ops=("fp", "bp"))
+    out["small_requests"] = {"stream": f"{3 * n_requests} FP/BP requests, sides 64-128, 60-180 "
+                                       "angles, Poisson 1e5 req/s (a burst), seed 9"}
+    out["small_requests"] |= _serve_both(serving.make_stream(3 * n_requests, rate=1e5, seed=9, **small),
+                                         serving.make_stream(6, rate=1000.0, seed=10, **small))
+    return out
+
+
+def sirt_benchmark(torch, ts, iterations=50):
+    """256^3 unit-cube volume, cone beam with 360 angles over 2 pi, 256 x 384 detector of size
+    (sqrt 2, 1.5 sqrt 2), cone_angle 0.5: SIRT iterations captured as a HIP graph with the
+    fused residual / update kernels, against the eager loop of separate operator calls and
+    elementwise kernels (tomosipo's own loop)."""
+    from tomosipo_amd.sirt import SIRT
+
+    vg = ts.volume(shape=256, size=1)
+    pg = ts.cone(angles=360, shape=(256, 384), size=SIZE, cone_angle=0.5)
+    A = ts.operator(vg, pg)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3000)
+    x = torch.randn((256,) * 3, generator=g, device="cuda") * 0.01
+    x[64:192, 64:192, 64:192] += 1.0
+    y = A(x)
+    res = {"workload": "SIRT, 256^3 cone, 360 angles, 256x384 detector", "iterations": iterations}
+    for name, kw in (("graph_fused", dict(fused=True, graph=True)),
+                     ("eager_unfused", dict(fused=False, graph=False))):
+        s = SIRT(A, y, fused=kw["fused"])
+        s.run(1, graph=kw["graph"])          # warm-up (and graph capture)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.run(iterations, graph=kw["graph"])
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        res[name] = {"ms_per_iteration": round(dt * 1e3 / iterations, 3),
+                     "gups": round(2 * 256 ** 3 * 360 * iterations / dt / 1e9, 1)}
+    return res
 
 
 def main():
@@ -129,6 +173,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--serving-requests", type=int, default=120,
                     help="requests in the mixed serving stream (0 skips the serving run)")
+    ap.add_argument("--no-sirt", action="store_true", help="skip the 256^3 SIRT timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="finish with a 1 GiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -241,6 +286,8 @@ def main():
             roofline["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
     if args.serving_requests > 0:
         rec["serving"] = serving_benchmark(args.serving_requests)
+    if not args.no_sirt:
+        rec["sirt"] = sirt_benchmark(torch, ts)
     if args.calibrate_copy:
         src = torch.empty(256 * 1024 * 1024, device=dev)
         dst = torch.empty_like(src)

@@ -207,19 +207,21 @@ struct BpView {
     Epilogue ep;
 };
 
-// LDS layout of bp_separable: a column of u holds kz = -1 .. BZ; columns are
-// BZ + 4 floats apart and slices BR * (BZ + 4) + 4 floats apart, so that the
-// columns of different slices start in different banks (slice stride = 4 mod
-// 64 floats) while every column stays 16-byte aligned for float4 reads.
+// LDS layout of bp_separable: a column of u holds kz = -1 .. BZ at offsets
+// 3 .. BZ + 4; columns are BZ + 4 floats apart (the kz = BZ guard of one
+// column falls on offset 0 of the next, which no column uses) and slices
+// BR * (BZ + 4) + 1 floats apart. With the odd slice stride, the 64 lanes of a
+// pass-1 wave (16 columns of 4 slices) writing the same kz hit 64 different
+// banks, and the lanes of a pass-2 wave (16 slices) hit at most two per bank.
+// Columns are therefore only 4-byte aligned; pass 2 reads them as pairs of
+// dwords.
 constexpr int UOFF = 4;  // kz = 0 sits at offset 4 of a column; kz = -1 at 3
 
 template <int BZ>
 struct BpShared {
-    // A column is BZ + 4 floats; its kz = BZ guard falls on offset 0 of the
-    // next column, which no column uses (kz = -1 is at offset 3).
     static constexpr int UCOL = BZ + 4;
-    static constexpr int SLICE = BR * UCOL + 4;
-    static_assert(UCOL % 4 == 0 && SLICE % 64 == 4, "LDS strides");
+    static constexpr int SLICE = BR * UCOL + 1;
+    static_assert(SLICE % 64 == 1, "LDS strides");
     float u[BR * SLICE];
     float q2[BR][20];     // Q_c2 - c0 per stored column, increasing; 1e30 if unused
     int jlo[2][BR];       // first detector column of each slice (double-buffered by angle)
@@ -264,8 +266,9 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
     float acc[BZ];
 #pragma unroll
     for (int k = 0; k < BZ; ++k) acc[k] = 0.0f;
-    // slice ranges of angle 0; thread (ty, 0) handles slice ty
-    if (tx == 0) bp_slice_range(angles[0], bx, by, cols, ty, sh.jlo[0][ty], sh.cnt[0][ty], sh.inc[0][ty]);
+    // slice ranges (16 lanes of the first wave, one slice each) of angle 0
+    const int t = ty * BR + tx;
+    if (t < BR) bp_slice_range(angles[0], bx, by, cols, t, sh.jlo[0][t], sh.cnt[0][t], sh.inc[0][t]);
     __syncthreads();
     for (int a = 0; a < n_angles; ++a) {
         const int buf = a & 1;
@@ -285,9 +288,8 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                 const int pos = (!valid || sh.inc[buf][s]) ? jj : m - 1 - jj;
                 const int j = sh.jlo[buf][s] + c * BR + jj;
                 float *col = &sh.u[s * S::SLICE + pos * S::UCOL];
-                float4 *c4 = reinterpret_cast<float4 *>(col);
 #pragma unroll
-                for (int k = 0; k < S::UCOL / 4; ++k) c4[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int k = UOFF - 1; k <= UOFF + BZ; ++k) col[k] = 0.0f;
                 if (valid) {
                     const double n = (double)(m0 + s);
                     const double Pm = A.P0[0] + j * A.U[0];
@@ -344,35 +346,29 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
             }
             // slice ranges of the next angle (the buffer was last read in the
             // previous angle, before its final barrier)
-            if (c == 0 && tx == 0 && a + 1 < n_angles)
-                bp_slice_range(angles[a + 1], bx, by, cols, ty, sh.jlo[buf ^ 1][ty],
-                                sh.cnt[buf ^ 1][ty], sh.inc[buf ^ 1][ty]);
+            if (c == 0 && t < BR && a + 1 < n_angles)
+                bp_slice_range(angles[a + 1], bx, by, cols, t, sh.jlo[buf ^ 1][t],
+                                sh.cnt[buf ^ 1][t], sh.inc[buf ^ 1][t]);
             __syncthreads();
-            // pass 2: the stored columns with |Q_c2 - kc| < 1 form the range
-            // [start, end) of the increasing sequence q2[s][.]
+            // pass 2: the stored columns with |Q_c2 - kc| < 1 follow the first
+            // one with Q_c2 > kc - 1 (binary search over the increasing q2[s][.])
             {
                 const int s = mx ? tx : ty;
                 const float kc = (float)(mx ? ty : tx);
-                const float4 *q4 = reinterpret_cast<const float4 *>(sh.q2[s]);
-                int start = 0, end = 0;
+                const float *qs = sh.q2[s];
+                int pos = 0;
 #pragma unroll
-                for (int k = 0; k < BR / 4; ++k) {
-                    const float4 q = q4[k];
-                    start += (q.x <= kc - 1.0f) + (q.y <= kc - 1.0f) + (q.z <= kc - 1.0f) + (q.w <= kc - 1.0f);
-                    end += (q.x < kc + 1.0f) + (q.y < kc + 1.0f) + (q.z < kc + 1.0f) + (q.w < kc + 1.0f);
-                }
+                for (int step = BR / 2; step > 0; step >>= 1)
+                    pos += (qs[pos + step - 1] <= kc - 1.0f) ? step : 0;
+                pos += (qs[pos] <= kc - 1.0f) ? 1 : 0;  // all BR entries below
                 const float *us = &sh.u[s * S::SLICE + UOFF];
-                for (int pos = start; pos < end; ++pos) {
-                    const float w = 1.0f - fabsf(sh.q2[s][pos] - kc);
-                    const float4 *u4 = reinterpret_cast<const float4 *>(us + pos * S::UCOL);
+                for (; pos < BR; ++pos) {
+                    const float d = qs[pos] - kc;
+                    if (!(d < 1.0f)) break;
+                    const float w = 1.0f - fabsf(d);
+                    const float *uc = us + pos * S::UCOL;
 #pragma unroll
-                    for (int q = 0; q < BZ / 4; ++q) {
-                        const float4 uv = u4[q];
-                        acc[4 * q + 0] = fmaf(w, uv.x, acc[4 * q + 0]);
-                        acc[4 * q + 1] = fmaf(w, uv.y, acc[4 * q + 1]);
-                        acc[4 * q + 2] = fmaf(w, uv.z, acc[4 * q + 2]);
-                        acc[4 * q + 3] = fmaf(w, uv.w, acc[4 * q + 3]);
-                    }
+                    for (int k = 0; k < BZ; ++k) acc[k] = fmaf(w, uc[k], acc[k]);
                 }
             }
             __syncthreads();  // u and q2 are rewritten by the next chunk or angle
