@@ -210,18 +210,20 @@ struct BpView {
 // LDS layout of bp_separable: a column of u holds kz = -1 .. BZ at offsets
 // 3 .. BZ + 4; columns are BZ + 4 floats apart (the kz = BZ guard of one
 // column falls on offset 0 of the next, which no column uses) and slices
-// BR * (BZ + 4) + 1 floats apart. With the odd slice stride, the 64 lanes of a
-// pass-1 wave (16 columns of 4 slices) writing the same kz hit 64 different
-// banks, and the lanes of a pass-2 wave (16 slices) hit at most two per bank.
-// Columns are therefore only 4-byte aligned; pass 2 reads them as pairs of
-// dwords.
+// BR * (BZ + 4) + 2 floats apart (= 2 mod 64). The strides were chosen with a
+// simulation of the kernel's LDS addresses on a cone-beam geometry (dword
+// banks of 64): this slice stride spreads the z-cells that the lanes of a
+// pass-1 wave (16 columns of 4 slices) and of a pass-2 wave (16 slices) touch
+// at the same time over the banks, while every column stays 8-byte aligned so
+// that pass 2 reads its cells as float2. Only the cells kz = 0 .. BZ - 1 are
+// read; the guard cells kz = -1 and kz = BZ take the splat's clamped rows.
 constexpr int UOFF = 4;  // kz = 0 sits at offset 4 of a column; kz = -1 at 3
 
 template <int BZ>
 struct BpShared {
     static constexpr int UCOL = BZ + 4;
-    static constexpr int SLICE = BR * UCOL + 1;
-    static_assert(SLICE % 64 == 1, "LDS strides");
+    static constexpr int SLICE = BR * UCOL + 2;
+    static_assert(UCOL % 2 == 0 && SLICE % 64 == 2, "LDS strides");
     float u[BR * SLICE];
     float q2[BR][20];     // Q_c2 - c0 per stored column, increasing; 1e30 if unused
     int jlo[2][BR];       // first detector column of each slice (double-buffered by angle)
@@ -288,8 +290,9 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                 const int pos = (!valid || sh.inc[buf][s]) ? jj : m - 1 - jj;
                 const int j = sh.jlo[buf][s] + c * BR + jj;
                 float *col = &sh.u[s * S::SLICE + pos * S::UCOL];
+                float2 *c2 = reinterpret_cast<float2 *>(col + UOFF);
 #pragma unroll
-                for (int k = UOFF - 1; k <= UOFF + BZ; ++k) col[k] = 0.0f;
+                for (int k = 0; k < BZ / 2; ++k) c2[k] = make_float2(0.0f, 0.0f);
                 if (valid) {
                     const double n = (double)(m0 + s);
                     const double Pm = A.P0[0] + j * A.U[0];
@@ -318,16 +321,17 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                     const float fB = (float)Bz;
                     uint32_t off = (uint32_t)i0 * row_bytes + ((uint32_t)a * (uint32_t)cols + (uint32_t)j) * 4u;
                     float *u = col + UOFF;
-                    // four rows per step, their loads in flight together; rows
-                    // past i1 are read (inside the buffer or zero by the range
-                    // check) and given weight zero
-                    float tb = t0;
-                    for (int i = i0; i <= i1; i += 4) {
-                        float yv[4];
+                    // rows in groups of four, the loads of the next group in
+                    // flight while the current one is splatted; rows past i1
+                    // are read (inside the buffer or zero by the range check)
+                    // and given weight zero
+                    auto load4 = [&](uint32_t o, float (&yv)[4]) {
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
                             yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                                  rs, off + (uint32_t)r * row_bytes, 0, 0));
+                                                                  rs, o + (uint32_t)r * row_bytes, 0, 0));
+                    };
+                    auto splat4 = [&](int i, float tb, const float (&yv)[4]) {
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const float ti = fmaf((float)r, fB, tb);
@@ -337,8 +341,20 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                             u[kf] = fmaf(1.0f - w, yy, u[kf]);
                             u[kf + 1] = fmaf(w, yy, u[kf + 1]);
                         }
-                        off += 4u * row_bytes;
-                        tb = fmaf(4.0f, fB, tb);
+                    };
+                    float ya[4], yb[4];
+                    float tb = t0;
+                    load4(off, ya);
+                    for (int i = i0; i <= i1; i += 8) {
+                        load4(off + 4u * row_bytes, yb);
+                        __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the splat
+                        splat4(i, tb, ya);
+                        if (i + 4 > i1) break;
+                        load4(off + 8u * row_bytes, ya);
+                        __builtin_amdgcn_sched_barrier(0);
+                        splat4(i + 4, fmaf(4.0f, fB, tb), yb);
+                        off += 8u * row_bytes;
+                        tb = fmaf(8.0f, fB, tb);
                     }
                 } else {
                     sh.q2[s][pos] = 1e30f;
@@ -366,9 +382,13 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                     const float d = qs[pos] - kc;
                     if (!(d < 1.0f)) break;
                     const float w = 1.0f - fabsf(d);
-                    const float *uc = us + pos * S::UCOL;
+                    const float2 *uc = reinterpret_cast<const float2 *>(us + pos * S::UCOL);
 #pragma unroll
-                    for (int k = 0; k < BZ; ++k) acc[k] = fmaf(w, uc[k], acc[k]);
+                    for (int k = 0; k < BZ / 2; ++k) {
+                        const float2 uv = uc[k];
+                        acc[2 * k] = fmaf(w, uv.x, acc[2 * k]);
+                        acc[2 * k + 1] = fmaf(w, uv.y, acc[2 * k + 1]);
+                    }
                 }
             }
             __syncthreads();  // u and q2 are rewritten by the next chunk or angle
@@ -385,7 +405,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
 }
 
 template <int BZ>
-__global__ void __launch_bounds__(BR *BR) bp_separable(BpView v, int n_blocks)
+__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ == 32 ? 4 : 1))) bp_separable(BpView v, int n_blocks)
 {
     __shared__ __align__(16) BpShared<BZ> sh;
     const int logical = xcd_logical(n_blocks);
@@ -400,7 +420,7 @@ struct BpMulti {
 };
 
 template <int BZ>
-__global__ void __launch_bounds__(BR *BR) bp_separable_multi(BpMulti m)
+__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ == 32 ? 4 : 1))) bp_separable_multi(BpMulti m)
 {
     __shared__ __align__(16) BpShared<BZ> sh;
     const int total = m.begin[m.n];
