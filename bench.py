@@ -133,6 +133,23 @@ def serving_benchmark(n_requests):
     return out
 
 
+def fdk_benchmark(torch, ts, reps=3):
+    """FDK reconstruction (ts.fdk) of one instance of the workload: 512^3, 720 angles, 512x768."""
+    vg, pg = instance_geometry(0, ts)
+    A = ts.operator(vg, pg)
+    y = A(phantom(0, torch, "cuda"))
+    out = torch.empty(A.domain_shape, device="cuda")
+    ts.fdk(A, y, out=out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ts.fdk(A, y, out=out)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"workload": "FDK of instance 0 (cone_vec 512^3, 720 angles, 512x768)",
+            "ms": round(dt * 1e3, 2), "gups": round(N_VOX ** 3 * N_ANGLES / dt / 1e9, 1)}
+
+
 def sirt_benchmark(torch, ts, iterations=50):
     """256^3 unit-cube volume, cone beam with 360 angles over 2 pi, 256 x 384 detector of size
     (sqrt 2, 1.5 sqrt 2), cone_angle 0.5: SIRT iterations captured as a HIP graph with the
@@ -174,6 +191,7 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=120,
                     help="requests in the mixed serving stream (0 skips the serving run)")
     ap.add_argument("--no-sirt", action="store_true", help="skip the 256^3 SIRT timing")
+    ap.add_argument("--no-fdk", action="store_true", help="skip the 512^3 FDK timing")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="finish with a 1 GiB device-to-device copy (reference for HBM counters)")
     args = ap.parse_args()
@@ -288,6 +306,8 @@ def main():
         rec["serving"] = serving_benchmark(args.serving_requests)
     if not args.no_sirt:
         rec["sirt"] = sirt_benchmark(torch, ts)
+    if not args.no_fdk:
+        rec["fdk"] = fdk_benchmark(torch, ts)
     if args.calibrate_copy:
         src = torch.empty(256 * 1024 * 1024, device=dev)
         dst = torch.empty_like(src)

@@ -115,6 +115,22 @@ int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);
  * override any choice. */
 int tsx_plan_autotune(tsx_plan *plan, void *stream);
 
+/* Replaces astra.experimental.accumulate_FDK(projector, vol, proj)
+ * (tomosipo/astra.py:374-406): FDK reconstruction of the projections `proj`
+ * ([row][angle][col], float32) of a cone_vec plan into `vol` ([z][y][x]),
+ * added to it when `additive` is non-zero, written otherwise. Cosine
+ * pre-weighting on a virtual detector through the volume centre, Ram-Lak
+ * ramp filter along detector rows (linear convolution of the zero-padded
+ * row), voxel-driven bilinear backprojection weighted by (dbeta / 2) *
+ * (D / U)^2, dbeta being the source's angular step around the z axis through
+ * the volume centre. `workspace` holds the filtered projections
+ * (tsx_fdk_workspace_size bytes); NULL takes stream-ordered device memory.
+ * TSX_ERR_UNSUPPORTED for parallel-beam plans, detector supersampling or more
+ * than 4096 detector columns. */
+int tsx_fdk_workspace_size(const tsx_plan *plan, int64_t *bytes);
+int tsx_fdk(const tsx_plan *plan, float *vol, const float *proj, int additive, void *workspace,
+            int64_t workspace_bytes, void *stream);
+
 /* Dominant (stepping) axis of every angle: 0 = x, 1 = y, 2 = z. */
 int tsx_plan_axes(const tsx_plan *plan, int32_t *axes_out);
 

@@ -12,6 +12,8 @@ Two independent restatements of the same projector:
 * ``numpy_twin``: a dense-matrix restatement in world coordinates for tiny
   problems (a few hundred voxels), written separately so that a convention
   mistake in one of them shows up as a disagreement.
+* ``fdk``: the FDK reconstruction (cosine weighting, Ram-Lak filter,
+  voxel-driven backprojection) in FP64 numpy, for small problems.
 
 The geometry handed to both is the ASTRA form that tomosipo passes to its
 projector (``tomosipo/astra.py:80-98``): the volume dictionary
@@ -22,6 +24,7 @@ projector (``tomosipo/astra.py:80-98``): the volume dictionary
 from .oracle import (OracleGeometry, oracle_geometry, fp, bp, axes, num_threads, lib_path,
                      build)
 from . import numpy_twin
+from . import fdk
 
 __all__ = ["OracleGeometry", "oracle_geometry", "fp", "bp", "axes", "num_threads", "lib_path",
-           "build", "numpy_twin"]
+           "build", "numpy_twin", "fdk"]

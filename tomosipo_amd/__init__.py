@@ -23,3 +23,5 @@ from .links.base import link  # noqa: E402
 from .Data import data  # noqa: E402
 from .Operator import operator  # noqa: E402
 from . import phantom  # noqa: E402
+from . import astra  # noqa: E402
+from .fdk import fdk  # noqa: E402

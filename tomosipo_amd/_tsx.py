@@ -35,7 +35,7 @@ _STATUS = {
 
 EXPORTED = (
     "tsx_plan_create", "tsx_plan_destroy", "tsx_plan_info_get", "tsx_plan_axes",
-    "tsx_plan_autotune",
+    "tsx_plan_autotune", "tsx_fdk_workspace_size", "tsx_fdk",
     "tsx_workspace_size",
     "tsx_direct_fpbp3d", "tsx_forward", "tsx_backward", "tsx_forward_batched",
     "tsx_backward_batched", "tsx_multi_workspace_size", "tsx_multi_forward",
@@ -109,6 +109,8 @@ def load():
         lib.tsx_plan_info_get.argtypes = [P, ctypes.POINTER(TsxPlanInfo)]
         lib.tsx_plan_axes.argtypes = [P, P]
         lib.tsx_plan_autotune.argtypes = [P, P]
+        lib.tsx_fdk_workspace_size.argtypes = [P, ctypes.POINTER(i64)]
+        lib.tsx_fdk.argtypes = [P, P, P, i32, P, i64, P]
         lib.tsx_direct_fpbp3d.argtypes = [P, P, P, i32, i32, P]
         lib.tsx_forward.argtypes = [P, P, P, i32, P]
         lib.tsx_backward.argtypes = [P, P, P, i32, P]
@@ -203,6 +205,16 @@ class Plan:
             "fp_variant": inf.fp_variant, "bp_brick_z": inf.bp_brick_z,
             "tuned": bool(inf.tuned),
         }
+
+    def fdk_workspace_size(self):
+        out = ctypes.c_int64()
+        _check(self._lib.tsx_fdk_workspace_size(self._h, ctypes.byref(out)))
+        return int(out.value)
+
+    def fdk(self, vol_ptr, proj_ptr, additive, stream, workspace=None, workspace_bytes=0):
+        """FDK reconstruction of the projections into the volume (``tsx_fdk``)."""
+        _check(self._lib.tsx_fdk(self._h, vol_ptr, proj_ptr, int(bool(additive)), workspace,
+                                 int(workspace_bytes), stream))
 
     def autotune(self, stream):
         """Time the kernel variants on this plan's shapes and keep the fastest

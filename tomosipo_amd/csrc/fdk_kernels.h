@@ -1,0 +1,102 @@
+// FDK reconstruction kernels (included by tsx.hip); cone-beam plans only.
+//
+// fdk_filter: one block per (detector row i, angle a). The row is weighted by
+//   the cosine of each pixel's ray against the detector normal, staged in LDS
+//   between zero margins, and convolved with the sampled Ram-Lak kernel
+//   g[0] = 1/4, g[odd m] = -1/(pi^2 m^2) over every lag of the row (the linear
+//   convolution of the zero-padded row), scaled by 1 / |U|.
+// fdk_bp: one thread per voxel; for every angle the voxel centre is projected
+//   from the source onto the virtual detector (FP64), the filtered
+//   projections are interpolated bilinearly (zero outside) and accumulated
+//   with weight (dbeta / 2) * lambda^2, lambda = D / ((X - s) . n).
+#pragma once
+
+#include "tsx_plan.h"
+
+namespace tsx {
+
+constexpr int FDK_FILTER_THREADS = 256;
+constexpr int FDK_MAX_COLS = 4096;  // LDS: (3 cols + cols / 2) floats per block
+constexpr int FDK_BX = 64, FDK_BY = 4;
+
+__global__ void __launch_bounds__(FDK_FILTER_THREADS)
+fdk_filter(const FdkAngle *__restrict__ angles, int n_angles, int rows, int cols,
+           const float *__restrict__ proj, float *__restrict__ q)
+{
+    extern __shared__ float lds[];
+    float *buf = lds;             // [0, cols): 0 | [cols, 2 cols): weighted row | [2 cols, 3 cols): 0
+    float *coef = lds + 3 * cols; // coef[k] = 1 / (pi^2 (2k + 1)^2)
+    const int i = blockIdx.x, a = blockIdx.y;
+    const FdkAngle &A = angles[a];
+    const int64_t row = ((int64_t)i * n_angles + a) * cols;
+    for (int k = threadIdx.x; k < 3 * cols; k += blockDim.x) {
+        const int j = k - cols;
+        float v = 0.0f;
+        if (j >= 0 && j < cols) {
+            const double px = A.P0s[0] + j * A.U[0] + i * A.V[0];
+            const double py = A.P0s[1] + j * A.U[1] + i * A.V[1];
+            const double pz = A.P0s[2] + j * A.U[2] + i * A.V[2];
+            v = (float)(A.D / sqrt(px * px + py * py + pz * pz) * (double)proj[row + j]);
+        }
+        buf[k] = v;
+    }
+    const int nodd = cols / 2;  // odd lags 1, 3, ..., <= cols - 1
+    for (int k = threadIdx.x; k < nodd; k += blockDim.x) {
+        const double m = 2.0 * k + 1.0;
+        coef[k] = (float)(1.0 / (M_PI * M_PI * m * m));
+    }
+    __syncthreads();
+    const float inv_du = (float)A.inv_du;
+    for (int j = threadIdx.x; j < cols; j += blockDim.x) {
+        const float *c = buf + cols + j;
+        float part = 0.0f;
+        for (int k = 0; k < nodd; ++k) {
+            const int m = 2 * k + 1;
+            part = fmaf(coef[k], c[-m] + c[m], part);
+        }
+        q[row + j] = inv_du * fmaf(0.25f, c[0], -part);
+    }
+}
+
+__global__ void __launch_bounds__(FDK_BX *FDK_BY)
+fdk_bp(const FdkAngle *__restrict__ angles, int n_angles, int rows, int cols,
+       const float *__restrict__ q, float *__restrict__ vol, int nx, int ny, int nz, double lo_x,
+       double lo_y, double lo_z, double h_x, double h_y, double h_z, int additive)
+{
+    const int x = blockIdx.x * FDK_BX + threadIdx.x;
+    const int y = blockIdx.y * FDK_BY + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= nx || y >= ny) return;
+    const double X = lo_x + (x + 0.5) * h_x, Y = lo_y + (y + 0.5) * h_y, Z = lo_z + (z + 0.5) * h_z;
+    const int64_t astride = cols, rstride = (int64_t)n_angles * cols;
+    float acc = 0.0f;
+    for (int a = 0; a < n_angles; ++a) {
+        const FdkAngle &A = angles[a];
+        const double wx = X - A.s[0], wy = Y - A.s[1], wz = Z - A.s[2];
+        const double den = wx * A.n[0] + wy * A.n[1] + wz * A.n[2];
+        if (!(den > 0.0)) continue;  // voxel not in front of the source
+        const double lam = A.D / den;
+        const float fj = (float)(A.cj + lam * (wx * A.Bj[0] + wy * A.Bj[1] + wz * A.Bj[2]));
+        const float fi = (float)(A.ci + lam * (wx * A.Bi[0] + wy * A.Bi[1] + wz * A.Bi[2]));
+        if (!(fj > -1.0f && fj < (float)cols && fi > -1.0f && fi < (float)rows)) continue;
+        const float fj0 = floorf(fj), fi0 = floorf(fi);
+        const int j0 = (int)fj0, i0 = (int)fi0;
+        const float wj = fj - fj0, wi = fi - fi0;
+        // corners outside the detector read a clamped (in-bounds) pixel and count 0
+        const float *base = q + (int64_t)a * astride;
+        const bool j_lo = j0 >= 0, j_hi = j0 + 1 < cols, i_lo = i0 >= 0, i_hi = i0 + 1 < rows;
+        const int jc0 = max(j0, 0), jc1 = min(j0 + 1, cols - 1);
+        const int64_t r0 = (int64_t)max(i0, 0) * rstride, r1 = (int64_t)min(i0 + 1, rows - 1) * rstride;
+        const float v00 = (i_lo && j_lo) ? base[r0 + jc0] : 0.0f;
+        const float v01 = (i_lo && j_hi) ? base[r0 + jc1] : 0.0f;
+        const float v10 = (i_hi && j_lo) ? base[r1 + jc0] : 0.0f;
+        const float v11 = (i_hi && j_hi) ? base[r1 + jc1] : 0.0f;
+        const float lo = fmaf(wj, v01 - v00, v00);
+        const float hi = fmaf(wj, v11 - v10, v10);
+        acc = fmaf((float)(A.half_dbeta * lam * lam), fmaf(wi, hi - lo, lo), acc);
+    }
+    float *o = vol + ((int64_t)z * ny + y) * nx + x;
+    *o = additive ? *o + acc : acc;
+}
+
+}  // namespace tsx

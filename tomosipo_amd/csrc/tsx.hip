@@ -27,6 +27,7 @@
 #include "tsx_plan.h"
 #include "fp_kernels.h"
 #include "bp_kernels.h"
+#include "fdk_kernels.h"
 
 using tsx::AngleParams;
 using namespace tsx;
@@ -65,6 +66,10 @@ struct tsx_plan {
     std::string bp_reason;      // why bp_separable is off
     // kernel variants chosen by tsx_plan_autotune (-1: library default)
     int32_t fp_variant = -1, bp_bz = -1;
+    // FDK (cone plans): per-angle quantities and the volume window
+    std::vector<FdkAngle> fdk_host;
+    FdkAngle *fdk_dev = nullptr;
+    double vlo[3] = {0, 0, 0}, vh[3] = {0, 0, 0};
 };
 
 // ---------------------------------------------------------------------------
@@ -95,6 +100,63 @@ int dominant_axis(V3 e)
     if (ax >= ay && ax >= az) return 0;
     if (ay >= az) return 1;
     return 2;
+}
+
+// FDK quantities of every angle (world coordinates; virtual detector through
+// the volume centre c; angular steps of the source around the z axis through
+// c from central differences of its unwrapped azimuth, one-sided at the ends).
+void build_fdk(const tsx_geometry *g, tsx_plan *p)
+{
+    const int na = g->n_angles;
+    V3 c;
+    for (int k = 0; k < 3; ++k) c[k] = 0.5 * (g->window_min[k] + g->window_max[k]);
+    std::vector<double> beta(na);
+    for (int a = 0; a < na; ++a) {
+        const double *vec = g->vectors + 12 * (int64_t)a;
+        beta[a] = std::atan2(vec[1] - c[1], vec[0] - c[0]);
+        if (a > 0) {  // unwrap
+            while (beta[a] - beta[a - 1] > M_PI) beta[a] -= 2 * M_PI;
+            while (beta[a] - beta[a - 1] < -M_PI) beta[a] += 2 * M_PI;
+        }
+    }
+    p->fdk_host.resize(na);
+    for (int a = 0; a < na; ++a) {
+        const double *vec = g->vectors + 12 * (int64_t)a;
+        V3 s = mk(vec[0], vec[1], vec[2]), det = mk(vec[3], vec[4], vec[5]);
+        V3 u = mk(vec[6], vec[7], vec[8]), v = mk(vec[9], vec[10], vec[11]);
+        V3 n = cross(u, v);
+        n = (1.0 / std::sqrt(dot(n, n))) * n;
+        if (dot(det - s, n) < 0) n = -1.0 * n;
+        V3 P0 = det + (0.5 - 0.5 * g->det_col_count) * u + (0.5 - 0.5 * g->det_row_count) * v;
+        const double lam = dot(c - s, n) / dot(P0 - s, n);
+        V3 P0s = lam * (P0 - s), U = lam * u, Vv = lam * v;
+        V3 VxN = cross(Vv, n), UxN = cross(U, n);
+        V3 Bj = (1.0 / dot(VxN, U)) * VxN, Bi = (1.0 / dot(UxN, Vv)) * UxN;
+        FdkAngle &F = p->fdk_host[a];
+        for (int k = 0; k < 3; ++k) {
+            F.s[k] = s[k];
+            F.n[k] = n[k];
+            F.P0s[k] = P0s[k];
+            F.U[k] = U[k];
+            F.V[k] = Vv[k];
+            F.Bj[k] = Bj[k];
+            F.Bi[k] = Bi[k];
+        }
+        F.D = dot(c - s, n);
+        F.cj = -dot(P0s, Bj);
+        F.ci = -dot(P0s, Bi);
+        double db;
+        if (na < 2)
+            db = 2 * M_PI;
+        else if (a == 0)
+            db = std::fabs(beta[1] - beta[0]);
+        else if (a == na - 1)
+            db = std::fabs(beta[na - 1] - beta[na - 2]);
+        else
+            db = 0.5 * std::fabs(beta[a + 1] - beta[a - 1]);
+        F.half_dbeta = 0.5 * db;
+        F.inv_du = 1.0 / std::sqrt(dot(U, U));
+    }
 }
 
 int build_angles(const tsx_geometry *g, tsx_plan *p)
@@ -209,6 +271,11 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
         A.cone = cone ? 1 : 0;
         p->axis_count[m] += 1;
     }
+    for (int c = 0; c < 3; ++c) {
+        p->vlo[c] = g->window_min[c];
+        p->vh[c] = h[c];
+    }
+    if (cone) build_fdk(g, p);
     // padded layouts of the forward projector
     const int64_t P2 = 2 * PAD;
     p->layout_elems[0] = (n3[2] + P2) * n3[0] * (n3[1] + P2);
@@ -994,6 +1061,17 @@ int tsx_plan_create(const tsx_geometry *g, int device, tsx_plan **out)
             delete p;
             return fail(TSX_ERR_HIP, std::string("hipMemcpy failed: ") + hipGetErrorString(e));
         }
+        if (!p->fdk_host.empty()) {
+            const size_t fb = sizeof(FdkAngle) * p->fdk_host.size();
+            e = hipMalloc((void **)&p->fdk_dev, fb);
+            if (e == hipSuccess) e = hipMemcpy(p->fdk_dev, p->fdk_host.data(), fb, hipMemcpyHostToDevice);
+            if (e != hipSuccess) {
+                if (p->fdk_dev) (void)hipFree(p->fdk_dev);
+                (void)hipFree(p->dev);
+                delete p;
+                return fail(TSX_ERR_OUT_OF_MEMORY, std::string("FDK tables: ") + hipGetErrorString(e));
+            }
+        }
         p->device = device;
     }
     *out = p;
@@ -1006,6 +1084,7 @@ int tsx_plan_destroy(tsx_plan *p)
     if (p->dev) {
         DeviceGuard guard(p->device);
         (void)hipFree(p->dev);
+        if (p->fdk_dev) (void)hipFree(p->fdk_dev);
     }
     delete p;
     return TSX_OK;
@@ -1028,6 +1107,50 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->fp_variant = fp_variant_of(p);
     info->bp_brick_z = bp_brick_z_of(p);
     info->tuned = p->fp_variant >= 0 || p->bp_bz > 0;
+    return TSX_OK;
+}
+
+int tsx_fdk_workspace_size(const tsx_plan *p, int64_t *bytes)
+{
+    if (!p || !bytes) return fail(TSX_ERR_INVALID_ARGUMENT, "plan and bytes must not be NULL");
+    *bytes = (int64_t)p->rows * p->n_angles * p->cols * 4;
+    return TSX_OK;
+}
+
+int tsx_fdk(const tsx_plan *p, float *vol, const float *proj, int additive, void *workspace,
+            int64_t workspace_bytes, void *stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+    if (p->kind != TSX_CONE_VEC || !p->fdk_dev)
+        return fail(TSX_ERR_UNSUPPORTED, "FDK needs a cone-beam (cone_vec) geometry");
+    if (p->ss != 1)
+        return fail(TSX_ERR_UNSUPPORTED, "FDK with detector supersampling is not supported");
+    if (p->cols > FDK_MAX_COLS)
+        return fail(TSX_ERR_UNSUPPORTED, "FDK supports at most " + std::to_string(FDK_MAX_COLS) +
+                                             " detector columns");
+    if (p->rows > 65535 || p->n_angles > 65535 || p->nz > 65535)
+        return fail(TSX_ERR_UNSUPPORTED, "FDK launch grid limit (65535 rows, angles or slices)");
+    DeviceGuard dg(p->device);
+    if (!dg.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t need = (int64_t)p->rows * p->n_angles * p->cols * 4;
+    Scratch owned;
+    void *wptr = nullptr;
+    rc = get_workspace(p, need, workspace, workspace_bytes, s, owned, &wptr);
+    if (rc) return rc;
+    float *q = (float *)wptr;
+    const size_t lds = (size_t)(3 * p->cols + p->cols / 2) * sizeof(float);
+    hipLaunchKernelGGL(fdk_filter, dim3((unsigned)p->rows, (unsigned)p->n_angles), dim3(FDK_FILTER_THREADS),
+                       lds, s, p->fdk_dev, p->n_angles, p->rows, p->cols, proj, q);
+    TSX_HIP_CHECK(hipGetLastError());
+    dim3 grid((unsigned)((p->nx + FDK_BX - 1) / FDK_BX), (unsigned)((p->ny + FDK_BY - 1) / FDK_BY),
+              (unsigned)p->nz);
+    hipLaunchKernelGGL(fdk_bp, grid, dim3(FDK_BX, FDK_BY), 0, s, p->fdk_dev, p->n_angles, p->rows,
+                       p->cols, q, vol, p->nx, p->ny, p->nz, p->vlo[0], p->vlo[1], p->vlo[2], p->vh[0],
+                       p->vh[1], p->vh[2], additive ? 1 : 0);
+    TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
 }
 

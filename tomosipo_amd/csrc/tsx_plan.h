@@ -27,6 +27,21 @@ struct AngleParams {
     int64_t pst[3];       // element strides along (m, c1, c2) in the padded layout of axis m
 };
 
+// Per-angle FDK quantities of a cone-beam plan, world (x, y, z) coordinates.
+// The detector is moved to the parallel plane through the volume centre c
+// (virtual detector: pixel (i, j) centre at s + P0s + j*U + i*V).
+struct FdkAngle {
+    double s[3];          // source
+    double n[3];          // unit detector normal, pointing away from the source
+    double P0s[3];        // virtual centre of pixel (0, 0) minus the source
+    double U[3], V[3];    // virtual column and row steps
+    double Bj[3], Bi[3];  // dual basis on the virtual plane
+    double D;             // (c - s) . n
+    double cj, ci;        // pixel coordinates (j, i) of the source's foot point: -P0s . Bj, -P0s . Bi
+    double half_dbeta;    // half the angular step of the source around the z axis through c
+    double inv_du;        // 1 / |U|
+};
+
 // How a kernel writes its result `acc` to element o of batch entry b.
 struct Epilogue {
     int32_t mode;    // 0: out = acc; 1: out += acc; 2: out = w * (y - acc); 3: out += w * acc

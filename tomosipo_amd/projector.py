@@ -158,3 +158,43 @@ def direct_fp(projector, vol_link, proj_link, additive=False):
 
 def direct_bp(projector, vol_link, proj_link, additive=False):
     return direct_project(projector, vol_link, proj_link, forward=False, additive=additive)
+
+
+def _run_fdk(projector, vol_t, proj_t, additive):
+    torch = _torch()
+    if vol_t.dtype != torch.float32 or proj_t.dtype != torch.float32:
+        raise TypeError("FDK takes float32 volume and projections")
+    dev = vol_t.device.index if vol_t.device.index is not None else torch.cuda.current_device()
+    plan = projector.plan(dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    nbytes = plan.fdk_workspace_size()
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=vol_t.device)
+    plan.fdk(vol_t.data_ptr(), proj_t.data_ptr(), additive, stream, ws.data_ptr(), nbytes)
+
+
+def direct_fdk(projector, vol_link, proj_link, additive=True):
+    """FDK reconstruction of ``proj_link`` into ``vol_link`` (``tsx_fdk``), with the
+    staging rules of ``direct_project``: device tensors in place on their stream,
+    numpy arrays and CPU tensors through device memory."""
+    if not are_compatible(vol_link, proj_link):
+        raise ValueError(
+            "Cannot perform FDK on volume and projection data, because they are not "
+            "compatible. Usually, this indicates that the data are located on different computing "
+            "devices. ")
+    torch = _torch()
+    vol, proj = vol_link.data, proj_link.data
+    with vol_link.context():
+        if isinstance(vol, torch.Tensor) and vol.is_cuda:
+            _run_fdk(projector, vol, proj, additive)
+            return
+        _require_device(torch)
+        dev = torch.device("cuda", torch.cuda.current_device())
+        to_dev = (lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)) \
+            if isinstance(vol, np.ndarray) else (lambda a: a.detach().to(dev))
+        vol_d = to_dev(vol)
+        _run_fdk(projector, vol_d, to_dev(proj), additive)
+        if isinstance(vol, np.ndarray):
+            vol[...] = vol_d.cpu().numpy()
+        else:
+            with torch.no_grad():
+                vol.copy_(vol_d.cpu())
