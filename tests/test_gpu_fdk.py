@@ -26,11 +26,13 @@ def _helix(n_angles, rows, cols, seed):
                        det_u=base.det_u)
 
 
-@pytest.mark.parametrize("kind", ["circular", "helix"])
+@pytest.mark.parametrize("kind", ["circular", "helix", "tilted"])
 def test_fdk_matches_oracle(kind):
+    """circular and helix take the upright-detector kernel, tilted the general one."""
     vg = ts.volume(shape=(28, 32, 30), size=1)
-    pg = (ts.cone(angles=40, shape=(36, 44), size=(1.4, 1.8), cone_angle=0.4) if kind == "circular"
-          else _helix(40, 36, 44, seed=5))
+    circ = ts.cone(angles=40, shape=(36, 44), size=(1.4, 1.8), cone_angle=0.4)
+    pg = {"circular": circ, "helix": _helix(40, 36, 44, seed=5),
+          "tilted": ts.rotate(pos=0, axis=(0, 1, 1), angles=0.2) * circ.to_vec()}[kind]
     print(f"[fdk] {kind}", flush=True)
     A = ts.operator(vg, pg)
     g = oracle.oracle_geometry(vg, pg)

@@ -12,6 +12,7 @@
 #pragma once
 
 #include "tsx_plan.h"
+#include "fp_kernels.h"
 
 namespace tsx {
 
@@ -97,6 +98,86 @@ fdk_bp(const FdkAngle *__restrict__ angles, int n_angles, int rows, int cols,
     }
     float *o = vol + ((int64_t)z * ny + y) * nx + x;
     *o = additive ? *o + acc : acc;
+}
+
+// Upright detectors (v parallel to z, detector normal perpendicular to z, every
+// angle; at least 2 detector columns): along a voxel column (x, y) the depth,
+// the detector column and the weight are constant and the detector row is
+// affine in z. One thread per (x, y) and FDK_ZC voxels along z: FP64 setup
+// once per angle, FP32 per voxel.
+constexpr int FDK_ZC = 16;
+
+__global__ void __launch_bounds__(FDK_BX *FDK_BY)
+fdk_bp_upright(const FdkAngle *__restrict__ angles, int n_angles, int rows, int cols,
+               const float *__restrict__ q, float *__restrict__ vol, int nx, int ny, int nz,
+               double lo_x, double lo_y, double lo_z, double h_x, double h_y, double h_z, int additive)
+{
+    const int x = blockIdx.x * FDK_BX + threadIdx.x;
+    const int y = blockIdx.y * FDK_BY + threadIdx.y;
+    const int z0 = blockIdx.z * FDK_ZC;
+    if (x >= nx || y >= ny) return;
+    const double X = lo_x + (x + 0.5) * h_x, Y = lo_y + (y + 0.5) * h_y, Z0 = lo_z + (z0 + 0.5) * h_z;
+    // the filtered projections are addressed with 32-bit byte offsets (< 4 GiB)
+    const uint32_t rbytes = (uint32_t)n_angles * (uint32_t)cols * 4u;
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)q, (short)0, (int)(rbytes * (uint32_t)rows), 0x00020000);
+    float acc[FDK_ZC];
+#pragma unroll
+    for (int k = 0; k < FDK_ZC; ++k) acc[k] = 0.0f;
+    for (int a = 0; a < n_angles; ++a) {
+        const FdkAngle &A = angles[a];
+        const double wx = X - A.s[0], wy = Y - A.s[1], wz = Z0 - A.s[2];
+        const double den = wx * A.n[0] + wy * A.n[1];
+        if (!(den > 0.0)) continue;
+        const double lam = A.D / den;
+        const float fj = (float)(A.cj + lam * (wx * A.Bj[0] + wy * A.Bj[1]));
+        if (!(fj > -1.0f && fj < (float)cols)) continue;
+        const float fi0 = (float)(A.ci + lam * (wx * A.Bi[0] + wy * A.Bi[1] + wz * A.Bi[2]));
+        const float dfi = (float)(lam * h_z * A.Bi[2]);
+        const float wgt = (float)(A.half_dbeta * lam * lam);
+        const float fj0 = floorf(fj);
+        const int j0 = (int)fj0;
+        const float wj = fj - fj0;
+        // columns (jb, jb + 1), jb = j0 clamped into [0, cols - 2], read as one
+        // 8-byte pair; the weights of columns outside the detector are 0
+        const int jb = min(max(j0, 0), max(cols - 2, 0));
+        float ca, cb;
+        if (j0 == jb) {
+            ca = 1.0f - wj;
+            cb = j0 + 1 < cols ? wj : 0.0f;
+        } else if (j0 < jb) {  // j0 = -1: only column 0 = jb, weight wj
+            ca = wj;
+            cb = 0.0f;
+        } else {               // j0 = cols - 1: only column jb + 1, weight 1 - wj
+            ca = 0.0f;
+            cb = 1.0f - wj;
+        }
+        const uint32_t cboff = ((uint32_t)a * (uint32_t)cols + (uint32_t)jb) * 4u;
+#pragma unroll
+        for (int k = 0; k < FDK_ZC; ++k) {
+            const float fi = fmaf((float)k, dfi, fi0);
+            const float fi0f = floorf(fi);
+            const int i0 = (int)fi0f;
+            const float wi = fi - fi0f;
+            const bool r0 = i0 >= 0 && i0 < rows, r1 = i0 + 1 >= 0 && i0 + 1 < rows;
+            const uint32_t o0 = (uint32_t)min(max(i0, 0), rows - 1) * rbytes + cboff;
+            const uint32_t o1 = (uint32_t)min(max(i0 + 1, 0), rows - 1) * rbytes + cboff;
+            // clamped rows are inside the buffer; rows outside the detector count 0
+            const float a0 = bload(rs, o0), b0 = bload(rs, o0 + 4u);
+            const float a1 = bload(rs, o1), b1 = bload(rs, o1 + 4u);
+            const float top = r0 ? fmaf(ca, a0, cb * b0) : 0.0f;
+            const float bot = r1 ? fmaf(ca, a1, cb * b1) : 0.0f;
+            acc[k] = fmaf(wgt, fmaf(wi, bot - top, top), acc[k]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < FDK_ZC; ++k) {
+        const int z = z0 + k;
+        if (z < nz) {
+            float *o = vol + ((int64_t)z * ny + y) * nx + x;
+            *o = additive ? *o + acc[k] : acc[k];
+        }
+    }
 }
 
 }  // namespace tsx

@@ -69,6 +69,7 @@ struct tsx_plan {
     // FDK (cone plans): per-angle quantities and the volume window
     std::vector<FdkAngle> fdk_host;
     FdkAngle *fdk_dev = nullptr;
+    bool fdk_upright = false;  // every angle: v parallel to z, normal perpendicular to z
     double vlo[3] = {0, 0, 0}, vh[3] = {0, 0, 0};
 };
 
@@ -120,6 +121,7 @@ void build_fdk(const tsx_geometry *g, tsx_plan *p)
         }
     }
     p->fdk_host.resize(na);
+    p->fdk_upright = true;
     for (int a = 0; a < na; ++a) {
         const double *vec = g->vectors + 12 * (int64_t)a;
         V3 s = mk(vec[0], vec[1], vec[2]), det = mk(vec[3], vec[4], vec[5]);
@@ -132,6 +134,9 @@ void build_fdk(const tsx_geometry *g, tsx_plan *p)
         V3 P0s = lam * (P0 - s), U = lam * u, Vv = lam * v;
         V3 VxN = cross(Vv, n), UxN = cross(U, n);
         V3 Bj = (1.0 / dot(VxN, U)) * VxN, Bi = (1.0 / dot(UxN, Vv)) * UxN;
+        const double vn = std::sqrt(dot(v, v));
+        if (std::fabs(v[0]) > 1e-12 * vn || std::fabs(v[1]) > 1e-12 * vn || std::fabs(n[2]) > 1e-12)
+            p->fdk_upright = false;
         FdkAngle &F = p->fdk_host[a];
         for (int k = 0; k < 3; ++k) {
             F.s[k] = s[k];
@@ -1145,11 +1150,17 @@ int tsx_fdk(const tsx_plan *p, float *vol, const float *proj, int additive, void
     hipLaunchKernelGGL(fdk_filter, dim3((unsigned)p->rows, (unsigned)p->n_angles), dim3(FDK_FILTER_THREADS),
                        lds, s, p->fdk_dev, p->n_angles, p->rows, p->cols, proj, q);
     TSX_HIP_CHECK(hipGetLastError());
-    dim3 grid((unsigned)((p->nx + FDK_BX - 1) / FDK_BX), (unsigned)((p->ny + FDK_BY - 1) / FDK_BY),
-              (unsigned)p->nz);
-    hipLaunchKernelGGL(fdk_bp, grid, dim3(FDK_BX, FDK_BY), 0, s, p->fdk_dev, p->n_angles, p->rows,
-                       p->cols, q, vol, p->nx, p->ny, p->nz, p->vlo[0], p->vlo[1], p->vlo[2], p->vh[0],
-                       p->vh[1], p->vh[2], additive ? 1 : 0);
+    const unsigned gx = (unsigned)((p->nx + FDK_BX - 1) / FDK_BX), gy = (unsigned)((p->ny + FDK_BY - 1) / FDK_BY);
+    const bool upright = p->fdk_upright && !force_general() && p->cols >= 2 && need < ((int64_t)1 << 32);
+    if (upright)
+        hipLaunchKernelGGL(fdk_bp_upright, dim3(gx, gy, (unsigned)((p->nz + FDK_ZC - 1) / FDK_ZC)),
+                           dim3(FDK_BX, FDK_BY), 0, s, p->fdk_dev, p->n_angles, p->rows, p->cols, q, vol,
+                           p->nx, p->ny, p->nz, p->vlo[0], p->vlo[1], p->vlo[2], p->vh[0], p->vh[1],
+                           p->vh[2], additive ? 1 : 0);
+    else
+        hipLaunchKernelGGL(fdk_bp, dim3(gx, gy, (unsigned)p->nz), dim3(FDK_BX, FDK_BY), 0, s, p->fdk_dev,
+                           p->n_angles, p->rows, p->cols, q, vol, p->nx, p->ny, p->nz, p->vlo[0],
+                           p->vlo[1], p->vlo[2], p->vh[0], p->vh[1], p->vh[2], additive ? 1 : 0);
     TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
 }
