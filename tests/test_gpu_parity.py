@@ -71,6 +71,23 @@ def test_cone_vec_helix_jitter():
     _check_fp_bp(vg, pg, x)
 
 
+@pytest.mark.parametrize("fp_variant,brick_z", [("2", "32"), ("3", "16")])
+def test_autotuner_alternatives_match_oracle(monkeypatch, fp_variant, brick_z):
+    """The kernel variants the autotuner may choose besides the defaults: the
+    per-lane forward march and 16-deep backprojector bricks (forced through the
+    environment overrides), on a helical cone_vec geometry with a volume depth
+    that is not a multiple of the brick depth."""
+    monkeypatch.setenv("TSX_FP_VARIANT", fp_variant)
+    monkeypatch.setenv("TSX_BP_BRICK_Z", brick_z)
+    vg = ts.volume(shape=(44, 40, 36), size=1)
+    pg = _cone_vec_helix(30, 44, 60, np.sqrt(2), 1.5 * np.sqrt(2), 0.25, 1e-3, seed=7)
+    x = ts.phantom.ellipsoids(44, count=8, seed=7)[:, :40, :36].copy()
+    A = ts.operator(vg, pg)
+    info = A.projector.plan(0).info()
+    assert info["fp_variant"] == int(fp_variant) and info["bp_brick_z"] == int(brick_z)
+    _check_fp_bp(vg, pg, x)
+
+
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])
 def test_random_geometries(seed):
     rng = np.random.default_rng(100 + seed)

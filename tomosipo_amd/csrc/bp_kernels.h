@@ -229,7 +229,9 @@ struct BpShared {
     int jlo[2][BR];       // first detector column of each slice (double-buffered by angle)
     int cnt[2][BR];       // number of columns of each slice
     int inc[2][BR];       // 1 if Q_c2 increases with the detector column
+    int chunks[2];        // max over the slices of ceil(cnt / BR)
 };
+
 
 // Slice range of slice t of a brick (first column, count, direction).
 __device__ __forceinline__ void bp_slice_range(const AngleParams &A, int bx, int by, int cols, int t,
@@ -246,6 +248,24 @@ __device__ __forceinline__ void bp_slice_range(const AngleParams &A, int bx, int
     jlo = lo;
     cnt = max(0, hi - lo + 1);
     inc = ja <= jb;
+}
+
+// Slice ranges of one angle for the 16 slices of a brick, computed by lanes
+// t = 0 .. BR - 1 of the first wave (all active together), which also publish
+// the number of column chunks.
+template <int BZ>
+__device__ __forceinline__ void bp_ranges(const AngleParams &A, int bx, int by, int cols, int t,
+                                          BpShared<BZ> &sh, int buf)
+{
+    int jlo, cnt, inc;
+    bp_slice_range(A, bx, by, cols, t, jlo, cnt, inc);
+    sh.jlo[buf][t] = jlo;
+    sh.cnt[buf][t] = cnt;
+    sh.inc[buf][t] = inc;
+    int ch = (cnt + BR - 1) / BR;
+#pragma unroll
+    for (int o = BR / 2; o > 0; o >>= 1) ch = max(ch, __shfl_xor(ch, o, BR));
+    if (t == 0) sh.chunks[buf] = ch;
 }
 
 template <int BZ>
@@ -270,7 +290,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
     for (int k = 0; k < BZ; ++k) acc[k] = 0.0f;
     // slice ranges (16 lanes of the first wave, one slice each) of angle 0
     const int t = ty * BR + tx;
-    if (t < BR) bp_slice_range(angles[0], bx, by, cols, t, sh.jlo[0][t], sh.cnt[0][t], sh.inc[0][t]);
+    if (t < BR) bp_ranges(angles[0], bx, by, cols, t, sh, 0);
     __syncthreads();
     for (int a = 0; a < n_angles; ++a) {
         const int buf = a & 1;
@@ -278,9 +298,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
         const bool mx = A.m == 0;
         const int m0 = mx ? bx * BR : by * BR;
         const int c0 = mx ? by * BR : bx * BR;
-        int chunks = 0;
-#pragma unroll
-        for (int s = 0; s < BR; ++s) chunks = max(chunks, (sh.cnt[buf][s] + BR - 1) / BR);
+        const int chunks = sh.chunks[buf];
         for (int c = 0; c < chunks; ++c) {
             // pass 1: thread (s = ty, jj = tx) splats detector column j onto kz
             {
@@ -362,9 +380,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
             }
             // slice ranges of the next angle (the buffer was last read in the
             // previous angle, before its final barrier)
-            if (c == 0 && t < BR && a + 1 < n_angles)
-                bp_slice_range(angles[a + 1], bx, by, cols, t, sh.jlo[buf ^ 1][t],
-                                sh.cnt[buf ^ 1][t], sh.inc[buf ^ 1][t]);
+            if (c == 0 && t < BR && a + 1 < n_angles) bp_ranges(angles[a + 1], bx, by, cols, t, sh, buf ^ 1);
             __syncthreads();
             // pass 2: the stored columns with |Q_c2 - kc| < 1 follow the first
             // one with Q_c2 > kc - 1 (binary search over the increasing q2[s][.])
