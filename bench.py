@@ -184,6 +184,37 @@ def fdk_benchmark(torch, ts, reps=3):
             "ms": round(dt * 1e3, 2), "gups": round(N_VOX ** 3 * N_ANGLES / dt / 1e9, 1)}
 
 
+def c4_benchmark(torch, ts, reps=10):
+    """Autograd step on a batch of 8 volumes 128^3, parallel3d 180 angles, 128x192 detector:
+    loss = |f(x) - y|^2 with f = to_autograd(A, num_extra_dims=1), forward + backward."""
+    from tomosipo_amd.torch_support import to_autograd
+
+    vg = ts.volume(shape=128, size=1)
+    pg = ts.parallel(angles=180, shape=(128, 192), size=(1, 1.5))
+    A = ts.operator(vg, pg)
+    f = to_autograd(A, num_extra_dims=1)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(4000)
+    x = torch.rand((8,) + A.domain_shape, generator=g, device="cuda", requires_grad=True)
+    y = torch.rand((8,) + A.range_shape, generator=g, device="cuda")
+
+    def step():
+        x.grad = None
+        loss = torch.sum((f(x) - y) ** 2)
+        loss.backward()
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return {"workload": "autograd |A x - y|^2, batch 8 x 128^3, parallel3d 180 angles, 128x192",
+            "ms_per_step": round(dt * 1e3, 3),
+            "gups": round(2 * 8 * 128 ** 3 * 180 / dt / 1e9, 1)}
+
+
 def sirt_benchmark(torch, ts, iterations=50):
     """256^3 unit-cube volume, cone beam with 360 angles over 2 pi, 256 x 384 detector of size
     (sqrt 2, 1.5 sqrt 2), cone_angle 0.5: SIRT iterations captured as a HIP graph with the
@@ -339,6 +370,7 @@ def main():
     if args.serving_requests > 0:
         rec["serving"] = serving_benchmark(args.serving_requests)
     rec["c1"] = c1_benchmark(torch, ts)
+    rec["c4"] = c4_benchmark(torch, ts)
     if not args.no_sirt:
         rec["sirt"] = sirt_benchmark(torch, ts)
     if not args.no_fdk:
