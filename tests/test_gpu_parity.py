@@ -71,12 +71,12 @@ def test_cone_vec_helix_jitter():
     _check_fp_bp(vg, pg, x)
 
 
-@pytest.mark.parametrize("fp_variant,brick_z", [("2", "32"), ("3", "16")])
+@pytest.mark.parametrize("fp_variant,brick_z", [("2", "32"), ("3", "16"), ("4", "32")])
 def test_autotuner_alternatives_match_oracle(monkeypatch, fp_variant, brick_z):
     """The kernel variants the autotuner may choose besides the defaults: the
-    per-lane forward march and 16-deep backprojector bricks (forced through the
-    environment overrides), on a helical cone_vec geometry with a volume depth
-    that is not a multiple of the brick depth."""
+    per-lane forward march, the row-paired layout and 16-deep backprojector
+    bricks (forced through the environment overrides), on a helical cone_vec
+    geometry with a volume depth that is not a multiple of the brick depth."""
     monkeypatch.setenv("TSX_FP_VARIANT", fp_variant)
     monkeypatch.setenv("TSX_BP_BRICK_Z", brick_z)
     vg = ts.volume(shape=(44, 40, 36), size=1)
@@ -309,7 +309,7 @@ def test_autotune_keeps_parity_and_caches_choices():
     info = plan.info()
     print(f"[autotune] {info}", flush=True)
     assert info["tuned"] is True
-    assert info["fp_variant"] in (2, 3)
+    assert info["fp_variant"] in (2, 3, 4)
     assert info["bp_brick_z"] in (16, 32)
     y1 = A(xd)
     b1 = A.T(y0)

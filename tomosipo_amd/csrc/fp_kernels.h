@@ -168,10 +168,13 @@ fp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
 // Layout for stepping axis z: dims [nz][ny + 2P][nx + 2P], c1 = y, c2 = x.
 // Element (m, c1, c2) of an angle sits at m*pst[0] + (c1+P)*pst[1] + (c2+P).
 
+// With `pair`, element (c1, c2) of a layout holds the two floats of rows c1
+// and c1 + 1 at c2 (8 bytes; the row past the last is 0), so that the four
+// corners of a bilinear sample are 16 contiguous bytes.
 __global__ void pad_copy_yz(const float *__restrict__ src, float *__restrict__ dst, int nx, int ny,
-                            int nz, int axis)
+                            int nz, int axis, int pair)
 {
-    // axis 1: dst [nz+2P][ny][nx+2P];  axis 2: dst [nz][ny+2P][nx+2P]
+    // axis 1: dst [nz+2P][ny][nx+2P] (c1 = z);  axis 2: dst [nz][ny+2P][nx+2P] (c1 = y)
     const int X = nx + 2 * PAD;
     const int Y = axis == 1 ? ny : ny + 2 * PAD;
     const int xp = blockIdx.x * blockDim.x + threadIdx.x;
@@ -181,31 +184,46 @@ __global__ void pad_copy_yz(const float *__restrict__ src, float *__restrict__ d
     const int x = xp - PAD;
     const int y = axis == 1 ? yp : yp - PAD;
     const int z = axis == 1 ? zp - PAD : zp;
-    float v = 0.0f;
-    if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
-        v = src[((int64_t)z * ny + y) * nx + x];
-    dst[((int64_t)zp * Y + yp) * X + xp] = v;
+    auto at = [&](int zz, int yy) {
+        return (x >= 0 && x < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz)
+                   ? src[((int64_t)zz * ny + yy) * nx + x] : 0.0f;
+    };
+    const int64_t o = ((int64_t)zp * Y + yp) * X + xp;
+    if (pair)
+        reinterpret_cast<float2 *>(dst)[o] =
+            make_float2(at(z, y), axis == 1 ? at(z + 1, y) : at(z, y + 1));
+    else
+        dst[o] = at(z, y);
 }
 
-// dst [nz+2P][nx][ny+2P] from src [nz][ny][nx], 32x32 tiles through LDS.
+// dst [nz+2P][nx][ny+2P] (c1 = z) from src [nz][ny][nx], 32x32 tiles through
+// LDS (two z planes for the paired layout).
 __global__ void pad_transpose_x(const float *__restrict__ src, float *__restrict__ dst, int nx,
-                                int ny, int nz)
+                                int ny, int nz, int pair)
 {
-    __shared__ float tile[32][33];
+    __shared__ float tile[2][32][33];
     const int Yp = ny + 2 * PAD;
     const int zp = blockIdx.z;
-    const int z = zp - PAD;
     const int x0 = blockIdx.x * 32, yp0 = blockIdx.y * 32;
-    for (int r = threadIdx.y; r < 32; r += blockDim.y) {
-        const int yp = yp0 + r, x = x0 + threadIdx.x, y = yp - PAD;
-        float v = 0.0f;
-        if (z >= 0 && z < nz && y >= 0 && y < ny && x < nx) v = src[((int64_t)z * ny + y) * nx + x];
-        tile[r][threadIdx.x] = v;
+    for (int d = 0; d <= pair; ++d) {
+        const int z = zp - PAD + d;
+        for (int r = threadIdx.y; r < 32; r += blockDim.y) {
+            const int yp = yp0 + r, x = x0 + threadIdx.x, y = yp - PAD;
+            float v = 0.0f;
+            if (z >= 0 && z < nz && y >= 0 && y < ny && x < nx) v = src[((int64_t)z * ny + y) * nx + x];
+            tile[d][r][threadIdx.x] = v;
+        }
     }
     __syncthreads();
     for (int r = threadIdx.y; r < 32; r += blockDim.y) {
         const int x = x0 + r, yp = yp0 + threadIdx.x;
-        if (x < nx && yp < Yp) dst[((int64_t)zp * nx + x) * Yp + yp] = tile[threadIdx.x][r];
+        if (x < nx && yp < Yp) {
+            const int64_t o = ((int64_t)zp * nx + x) * Yp + yp;
+            if (pair)
+                reinterpret_cast<float2 *>(dst)[o] = make_float2(tile[0][threadIdx.x][r], tile[1][threadIdx.x][r]);
+            else
+                dst[o] = tile[0][threadIdx.x][r];
+        }
     }
 }
 
@@ -342,6 +360,24 @@ __device__ __forceinline__ void request_sync(__amdgpu_buffer_rsrc_t rs, const Sy
     c.w2 = __builtin_amdgcn_fractf(q2);
 }
 
+// Paired layout: the 16 bytes at element (k1, k2) are v[k1][k2], v[k1+1][k2],
+// v[k1][k2+1], v[k1+1][k2+1] (one wave-wide 16-byte load per sample).
+__device__ __forceinline__ void request_rowpair(__amdgpu_buffer_rsrc_t rs, const SyncRay &r, float dn,
+                                                uint32_t rowoff, uint32_t rowb, Corners &c)
+{
+    const float q1 = __builtin_amdgcn_fmed3f(fmaf(dn, r.a1, r.R1), r.lo1, r.hi1);
+    const float q2 = __builtin_amdgcn_fmed3f(fmaf(dn, r.a2, r.R2), r.lo2, r.hi2);
+    const uint32_t off = rowoff + __umul24((uint32_t)floor_to_int(q1), rowb) +
+                         ((uint32_t)floor_to_int(q2) << 3);
+    c.v00 = bload(rs, off);
+    c.v10 = bload(rs, off + 4u);
+    c.v01 = bload(rs, off + 8u);
+    c.v11 = bload(rs, off + 12u);
+    c.w1 = __builtin_amdgcn_fractf(q1);
+    c.w2 = __builtin_amdgcn_fractf(q2);
+}
+
+template <bool ROWPAIR>
 __device__ __forceinline__ float march_padded_sync(__amdgpu_buffer_rsrc_t rs, const RaySetup &r,
                                                    bool valid, int nref, int n1, int n2,
                                                    uint32_t st_m, uint32_t st_1)
@@ -363,28 +399,35 @@ __device__ __forceinline__ float march_padded_sync(__amdgpu_buffer_rsrc_t rs, co
         q.a1 = q.a2 = 0.0f;
         q.R1 = q.R2 = q.lo1;
     }
-    const uint32_t rowb = st_1 * 4u, stepb = st_m * 4u;
+    const uint32_t es = ROWPAIR ? 8u : 4u;  // bytes per layout element
+    const uint32_t rowb = st_1 * es, stepb = st_m * es;
+    auto req = [&](float dn_, uint32_t ro_, Corners &c_) {
+        if (ROWPAIR)
+            request_rowpair(rs, q, dn_, ro_, rowb, c_);
+        else
+            request_sync(rs, q, dn_, ro_, rowb, c_);
+    };
     float dn = (float)(wlo - nref);
     uint32_t rowoff = (uint32_t)wlo * stepb;
     Corners A0, A1, A2, A3;
-    request_sync(rs, q, dn, rowoff, rowb, A0);
-    request_sync(rs, q, dn + 1.0f, rowoff + stepb, rowb, A1);
-    request_sync(rs, q, dn + 2.0f, rowoff + 2u * stepb, rowb, A2);
-    request_sync(rs, q, dn + 3.0f, rowoff + 3u * stepb, rowb, A3);
+    req(dn, rowoff, A0);
+    req(dn + 1.0f, rowoff + stepb, A1);
+    req(dn + 2.0f, rowoff + 2u * stepb, A2);
+    req(dn + 3.0f, rowoff + 3u * stepb, A3);
     float s0 = 0.0f, s1 = 0.0f;
     const int cnt = whi - wlo + 1;
     for (int k = 0; k < cnt; k += 4) {
         s0 += interp(A0);
-        request_sync(rs, q, dn + 4.0f, rowoff + 4u * stepb, rowb, A0);
+        req(dn + 4.0f, rowoff + 4u * stepb, A0);
         const float v1 = interp(A1);
         s1 += (k + 1 < cnt) ? v1 : 0.0f;
-        request_sync(rs, q, dn + 5.0f, rowoff + 5u * stepb, rowb, A1);
+        req(dn + 5.0f, rowoff + 5u * stepb, A1);
         const float v2 = interp(A2);
         s0 += (k + 2 < cnt) ? v2 : 0.0f;
-        request_sync(rs, q, dn + 6.0f, rowoff + 6u * stepb, rowb, A2);
+        req(dn + 6.0f, rowoff + 6u * stepb, A2);
         const float v3 = interp(A3);
         s1 += (k + 3 < cnt) ? v3 : 0.0f;
-        request_sync(rs, q, dn + 7.0f, rowoff + 7u * stepb, rowb, A3);
+        req(dn + 7.0f, rowoff + 7u * stepb, A3);
         dn += 4.0f;
         rowoff += 4u * stepb;
     }
@@ -416,7 +459,7 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
     const int i = (tile / v.tiles_u) * FP_BY + threadIdx.y;
     if (b >= v.batch) return;  // uniform over the block
     const bool inside = j < v.cols && i < v.rows;
-    if (PIPE != 3 && !inside) return;
+    if (PIPE < 3 && !inside) return;
     const AngleParams &A = v.angles[a];
     const float *lp = uniform_ptr(v.L.ptr[A.m] + (int64_t)b * v.L.stride);
     const __amdgpu_buffer_rsrc_t rs =
@@ -430,10 +473,10 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
             RaySetup r;
             const double fj = SS == 1 ? (double)j : j + (q + 0.5) / SS - 0.5;
             const double fi = SS == 1 ? (double)i : i + (p + 0.5) / SS - 0.5;
-            if (PIPE == 3) {
+            if (PIPE >= 3) {
                 // every lane of the wave joins the march (wave-wide reductions)
                 const bool ok = inside && ray_setup(A, fj, fi, (double)PAD, r);
-                acc += march_padded_sync(rs, r, ok, A.n[0] / 2, A.n[1], A.n[2], st_m, st_1);
+                acc += march_padded_sync<PIPE == 4>(rs, r, ok, A.n[0] / 2, A.n[1], A.n[2], st_m, st_1);
             } else if (ray_setup(A, fj, fi, (double)PAD, r)) {
                 acc += march_padded_pipe4(rs, r, st_m, st_1);
             }

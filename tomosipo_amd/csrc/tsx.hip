@@ -62,6 +62,7 @@ struct tsx_plan {
     // padded per-axis layouts for the forward projector (elements per layout)
     int64_t layout_elems[3] = {0, 0, 0};
     bool fp_padded = false;   // all used layouts addressable with 32-bit byte offsets
+    bool fp_pair = false;     // ... also with row-paired (8-byte) elements
     bool bp_separable = false;  // upright detector, x/y stepping, ss == 1
     std::string bp_reason;      // why bp_separable is off
     // kernel variants chosen by tsx_plan_autotune (-1: library default)
@@ -290,11 +291,15 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
     // elements between rows of c1 in each layout; the march forms row offsets
     // with 24-bit multiplies, so a row must stay below 2^24 bytes
     const int64_t row1[3] = {n3[0] * (n3[1] + P2), n3[1] * (n3[0] + P2), n3[0] + P2};
+    p->fp_pair = true;
     for (int c = 0; c < 3; ++c) {
         if (p->axis_count[c] == 0) continue;
         if (p->layout_elems[c] * 4 >= ((int64_t)1 << 32) || row1[c] * 4 >= ((int64_t)1 << 24))
             p->fp_padded = false;
+        if (p->layout_elems[c] * 8 >= ((int64_t)1 << 32) || row1[c] * 8 >= ((int64_t)1 << 24))
+            p->fp_pair = false;
     }
+    p->fp_pair = p->fp_pair && p->fp_padded;
     // separable backprojection: upright detector (v along z), stepping along x or y
     p->bp_separable = g->det_supersampling == 1;
     if (!p->bp_separable) p->bp_reason = "detector supersampling";
@@ -372,14 +377,19 @@ int env_int(const char *name)
 
 // Kernel variant of the padded forward projector: 2 = each lane marches its
 // own ray's slice range, four samples in flight; 3 = the lanes of a wave march
-// the same slices (coalesced reads; default). TSX_FP_VARIANT overrides the
-// plan's choice.
+// the same slices (coalesced reads); 4 = as 3 on row-paired layouts (one
+// 16-byte load per sample; default; falls back to 3 where those layouts would
+// not fit 32-bit offsets). TSX_FP_VARIANT overrides the plan's choice.
 int fp_variant_of(const tsx_plan *p)
 {
     const int e = env_int("TSX_FP_VARIANT");
-    if (e == 2 || e == 3) return e;
-    return p->fp_variant >= 0 ? p->fp_variant : 3;
+    int v = (e >= 2 && e <= 4) ? e : (p->fp_variant >= 0 ? p->fp_variant : 4);
+    if (v == 4 && !p->fp_pair) v = 3;
+    return v;
 }
+
+// Floats per element of the forward projector's padded layouts.
+int fp_esize(const tsx_plan *p) { return fp_variant_of(p) == 4 ? 2 : 1; }
 
 // z-extent of the separable backprojector's bricks: 32 (default) or 16.
 // TSX_BP_BRICK_Z overrides the plan's choice.
@@ -436,7 +446,7 @@ int64_t fp_workspace_elems(const tsx_plan *p)
 {
     int64_t per_entry = 0;
     for (int c = 0; c < 3; ++c)
-        if (p->axis_count[c] > 0) per_entry += p->layout_elems[c];
+        if (p->axis_count[c] > 0) per_entry += p->layout_elems[c] * fp_esize(p);
     return per_entry;
 }
 
@@ -474,23 +484,24 @@ int get_workspace(const tsx_plan *p, int64_t bytes, void *ws, int64_t ws_bytes, 
 void build_layouts(const tsx_plan *p, const float *src, float *dst, const int64_t offs[3],
                    hipStream_t stream)
 {
+    const int pair = fp_esize(p) == 2;
     if (p->axis_count[0] > 0) {
         dim3 grid((unsigned)((p->nx + 31) / 32), (unsigned)((p->ny + 2 * PAD + 31) / 32),
                   (unsigned)(p->nz + 2 * PAD));
         hipLaunchKernelGGL(pad_transpose_x, grid, dim3(32, 8), 0, stream, src, dst + offs[0], p->nx,
-                           p->ny, p->nz);
+                           p->ny, p->nz, pair);
     }
     if (p->axis_count[1] > 0) {
         dim3 grid((unsigned)((p->nx + 2 * PAD + 255) / 256), (unsigned)p->ny,
                   (unsigned)(p->nz + 2 * PAD));
         hipLaunchKernelGGL(pad_copy_yz, grid, dim3(256), 0, stream, src, dst + offs[1], p->nx, p->ny,
-                           p->nz, 1);
+                           p->nz, 1, pair);
     }
     if (p->axis_count[2] > 0) {
         dim3 grid((unsigned)((p->nx + 2 * PAD + 255) / 256), (unsigned)(p->ny + 2 * PAD),
                   (unsigned)p->nz);
         hipLaunchKernelGGL(pad_copy_yz, grid, dim3(256), 0, stream, src, dst + offs[2], p->nx, p->ny,
-                           p->nz, 2);
+                           p->nz, 2, pair);
     }
 }
 
@@ -500,7 +511,7 @@ void layout_offsets(const tsx_plan *p, int64_t offs[3], int64_t *per_entry)
     int64_t e = 0;
     for (int c = 0; c < 3; ++c) {
         offs[c] = e;
-        if (p->axis_count[c] > 0) e += p->layout_elems[c];
+        if (p->axis_count[c] > 0) e += p->layout_elems[c] * fp_esize(p);
     }
     *per_entry = e;
 }
@@ -542,7 +553,7 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     v.batch = (int)batch;
     for (int c = 0; c < 3; ++c) {
         v.L.ptr[c] = p->axis_count[c] > 0 ? base + offs[c] : nullptr;
-        v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4);
+        v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4 * fp_esize(p));
     }
     v.L.stride = per_entry;
     v.proj = proj;
@@ -558,7 +569,9 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     hipLaunchKernelGGL((fp_padded<K, V>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream, v, n_blocks)
 #define TSX_FP_CASE(K)                                                                             \
     case K:                                                                                        \
-        if (variant == 3)                                                                          \
+        if (variant == 4)                                                                          \
+            TSX_FP_LAUNCH(K, 4);                                                                   \
+        else if (variant == 3)                                                                     \
             TSX_FP_LAUNCH(K, 3);                                                                   \
         else                                                                                       \
             TSX_FP_LAUNCH(K, 2);                                                                   \
@@ -729,7 +742,9 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
         const unsigned grid = (unsigned)(((m.begin[m.n] + 7) / 8) * 8);
-        if (variant == 3)
+        if (variant == 4)
+            hipLaunchKernelGGL(fp_padded_multi<4>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
+        else if (variant == 3)
             hipLaunchKernelGGL(fp_padded_multi<3>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
         else
             hipLaunchKernelGGL(fp_padded_multi<2>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
@@ -768,7 +783,7 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         v.batch = 1;
         for (int c = 0; c < 3; ++c) {
             v.L.ptr[c] = p->axis_count[c] > 0 ? base + offs[c] : nullptr;
-            v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4);
+            v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4 * fp_esize(p));
         }
         v.L.stride = 0;
         v.proj = projs[k];
@@ -874,7 +889,8 @@ std::string tune_key(const tsx_plan *p)
     return std::to_string(p->device) + ":" + std::to_string(p->host.empty() ? 0 : p->host[0].cone) +
            ":" + std::to_string(p->rows) + "x" + std::to_string(p->cols) + ":" + std::to_string(p->nx) +
            "x" + std::to_string(p->ny) + "x" + std::to_string(p->nz) + ":" + std::to_string(p->ss) + ":" +
-           std::to_string((int)p->fp_padded) + std::to_string((int)p->bp_separable);
+           std::to_string((int)p->fp_padded) + std::to_string((int)p->fp_pair) +
+           std::to_string((int)p->bp_separable);
 }
 
 // Mean time of `reps` launches after one warm-up launch, in milliseconds.
@@ -941,6 +957,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
     sub.n_angles = (int32_t)sub.host.size();
     for (int c = 0; c < 3; ++c) sub.layout_elems[c] = p->layout_elems[c];
     sub.fp_padded = p->fp_padded;
+    sub.fp_pair = p->fp_pair;
     sub.bp_separable = p->bp_separable;
     TuneScratch t;
     const size_t vol_bytes = (size_t)p->nx * p->ny * p->nz * 4;
@@ -963,7 +980,8 @@ int autotune(tsx_plan *p, hipStream_t stream)
     const bool fp_choices = p->fp_padded && p->ss >= 1 && p->ss <= 4 && !force_general();
     if (fp_choices) {
         float best_ms = 1e30f;
-        for (int cand : {3, 2}) {
+        for (int cand : {4, 3, 2}) {
+            if (cand == 4 && !p->fp_pair) continue;
             sub.fp_variant = cand;
             float ms = 0.0f;
             int rc = time_launches([&] { return launch_fp(&sub, vol, 0, proj, 0, 1, ep, stream); },
