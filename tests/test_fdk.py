@@ -78,4 +78,5 @@ def test_fdk_api_without_device():
     with pytest.raises(_tsx.TsxError) as e:
         p.fdk(None, None, False, None)
     assert e.value.code == -4
-    assert p.fdk_workspace_size() == 8 * 4 * 8 * 4
+    # an upright cone: row-paired filtered projections, 8 bytes per pixel
+    assert p.fdk_workspace_size() == 8 * 4 * 8 * 8

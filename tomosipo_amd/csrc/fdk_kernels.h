@@ -20,9 +20,11 @@ constexpr int FDK_FILTER_THREADS = 256;
 constexpr int FDK_MAX_COLS = 4096;  // LDS: (3 cols + cols / 2) floats per block
 constexpr int FDK_BX = 64, FDK_BY = 4;
 
+// With `pair`, q is written row-paired: element (i, a, j) holds the filtered
+// values of rows i and i + 1 (0 past the last row), 8 bytes.
 __global__ void __launch_bounds__(FDK_FILTER_THREADS)
 fdk_filter(const FdkAngle *__restrict__ angles, int n_angles, int rows, int cols,
-           const float *__restrict__ proj, float *__restrict__ q)
+           const float *__restrict__ proj, float *__restrict__ q, int pair)
 {
     extern __shared__ float lds[];
     float *buf = lds;             // [0, cols): 0 | [cols, 2 cols): weighted row | [2 cols, 3 cols): 0
@@ -55,7 +57,15 @@ fdk_filter(const FdkAngle *__restrict__ angles, int n_angles, int rows, int cols
             const int m = 2 * k + 1;
             part = fmaf(coef[k], c[-m] + c[m], part);
         }
-        q[row + j] = inv_du * fmaf(0.25f, c[0], -part);
+        const float v = inv_du * fmaf(0.25f, c[0], -part);
+        if (!pair) {
+            q[row + j] = v;
+        } else {
+            const int64_t prev = ((int64_t)(i - 1) * n_angles + a) * cols;
+            q[2 * (row + j)] = v;
+            if (i > 0) q[2 * (prev + j) + 1] = v;
+            if (i == rows - 1) q[2 * (row + j) + 1] = 0.0f;
+        }
     }
 }
 
@@ -101,7 +111,7 @@ fdk_bp(const FdkAngle *__restrict__ angles, int n_angles, int rows, int cols,
 }
 
 // Upright detectors (v parallel to z, detector normal perpendicular to z, every
-// angle; at least 2 detector columns): along a voxel column (x, y) the depth,
+// angle; at least 2 detector columns; row-paired q): along a voxel column (x, y) the depth,
 // the detector column and the weight are constant and the detector row is
 // affine in z. One thread per (x, y) and FDK_ZC voxels along z: FP64 setup
 // once per angle, FP32 per voxel.
@@ -117,8 +127,8 @@ fdk_bp_upright(const FdkAngle *__restrict__ angles, int n_angles, int rows, int 
     const int z0 = blockIdx.z * FDK_ZC;
     if (x >= nx || y >= ny) return;
     const double X = lo_x + (x + 0.5) * h_x, Y = lo_y + (y + 0.5) * h_y, Z0 = lo_z + (z0 + 0.5) * h_z;
-    // the filtered projections are addressed with 32-bit byte offsets (< 4 GiB)
-    const uint32_t rbytes = (uint32_t)n_angles * (uint32_t)cols * 4u;
+    // the row-paired filtered projections are addressed with 32-bit byte offsets (< 4 GiB)
+    const uint32_t rbytes = (uint32_t)n_angles * (uint32_t)cols * 8u;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)q, (short)0, (int)(rbytes * (uint32_t)rows), 0x00020000);
     float acc[FDK_ZC];
@@ -152,21 +162,24 @@ fdk_bp_upright(const FdkAngle *__restrict__ angles, int n_angles, int rows, int 
             ca = 0.0f;
             cb = 1.0f - wj;
         }
-        const uint32_t cboff = ((uint32_t)a * (uint32_t)cols + (uint32_t)jb) * 4u;
+        // q is row-paired: the 16 bytes at (row i, column jb) are
+        // q[i][jb], q[i+1][jb], q[i][jb+1], q[i+1][jb+1]
+        const uint32_t cboff = ((uint32_t)a * (uint32_t)cols + (uint32_t)jb) * 8u;
 #pragma unroll
         for (int k = 0; k < FDK_ZC; ++k) {
             const float fi = fmaf((float)k, dfi, fi0);
             const float fi0f = floorf(fi);
             const int i0 = (int)fi0f;
             const float wi = fi - fi0f;
-            const bool r0 = i0 >= 0 && i0 < rows, r1 = i0 + 1 >= 0 && i0 + 1 < rows;
-            const uint32_t o0 = (uint32_t)min(max(i0, 0), rows - 1) * rbytes + cboff;
-            const uint32_t o1 = (uint32_t)min(max(i0 + 1, 0), rows - 1) * rbytes + cboff;
-            // clamped rows are inside the buffer; rows outside the detector count 0
-            const float a0 = bload(rs, o0), b0 = bload(rs, o0 + 4u);
-            const float a1 = bload(rs, o1), b1 = bload(rs, o1 + 4u);
-            const float top = r0 ? fmaf(ca, a0, cb * b0) : 0.0f;
-            const float bot = r1 ? fmaf(ca, a1, cb * b1) : 0.0f;
+            // rows i0, i0 + 1 from the element of row max(i0, 0) (inside the
+            // buffer); for i0 = -1 that element's first row is row i0 + 1
+            const uint32_t o = (uint32_t)min(max(i0, 0), rows - 1) * rbytes + cboff;
+            const float x0 = bload(rs, o), y0 = bload(rs, o + 4u);
+            const float x1 = bload(rs, o + 8u), y1 = bload(rs, o + 12u);
+            const float first = fmaf(ca, x0, cb * x1), second = fmaf(ca, y0, cb * y1);
+            const bool in = i0 >= 0 && i0 < rows;
+            const float top = in ? first : 0.0f;
+            const float bot = in ? second : (i0 == -1 ? first : 0.0f);
             acc[k] = fmaf(wgt, fmaf(wi, bot - top, top), acc[k]);
         }
     }

@@ -1015,6 +1015,14 @@ int autotune(tsx_plan *p, hipStream_t stream)
     return TSX_OK;
 }
 
+// The upright FDK backprojector reads row-paired filtered projections (8 bytes
+// per pixel, addressed with 32-bit offsets); the general one plain ones.
+bool fdk_paired(const tsx_plan *p)
+{
+    return p->fdk_upright && !force_general() && p->cols >= 2 &&
+           (int64_t)p->rows * p->n_angles * p->cols * 8 < ((int64_t)1 << 32);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1136,7 +1144,7 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
 int tsx_fdk_workspace_size(const tsx_plan *p, int64_t *bytes)
 {
     if (!p || !bytes) return fail(TSX_ERR_INVALID_ARGUMENT, "plan and bytes must not be NULL");
-    *bytes = (int64_t)p->rows * p->n_angles * p->cols * 4;
+    *bytes = (int64_t)p->rows * p->n_angles * p->cols * (fdk_paired(p) ? 8 : 4);
     return TSX_OK;
 }
 
@@ -1158,7 +1166,8 @@ int tsx_fdk(const tsx_plan *p, float *vol, const float *proj, int additive, void
     DeviceGuard dg(p->device);
     if (!dg.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
     hipStream_t s = (hipStream_t)stream;
-    const int64_t need = (int64_t)p->rows * p->n_angles * p->cols * 4;
+    const bool upright = fdk_paired(p);
+    const int64_t need = (int64_t)p->rows * p->n_angles * p->cols * (upright ? 8 : 4);
     Scratch owned;
     void *wptr = nullptr;
     rc = get_workspace(p, need, workspace, workspace_bytes, s, owned, &wptr);
@@ -1166,10 +1175,9 @@ int tsx_fdk(const tsx_plan *p, float *vol, const float *proj, int additive, void
     float *q = (float *)wptr;
     const size_t lds = (size_t)(3 * p->cols + p->cols / 2) * sizeof(float);
     hipLaunchKernelGGL(fdk_filter, dim3((unsigned)p->rows, (unsigned)p->n_angles), dim3(FDK_FILTER_THREADS),
-                       lds, s, p->fdk_dev, p->n_angles, p->rows, p->cols, proj, q);
+                       lds, s, p->fdk_dev, p->n_angles, p->rows, p->cols, proj, q, upright ? 1 : 0);
     TSX_HIP_CHECK(hipGetLastError());
     const unsigned gx = (unsigned)((p->nx + FDK_BX - 1) / FDK_BX), gy = (unsigned)((p->ny + FDK_BY - 1) / FDK_BY);
-    const bool upright = p->fdk_upright && !force_general() && p->cols >= 2 && need < ((int64_t)1 << 32);
     if (upright)
         hipLaunchKernelGGL(fdk_bp_upright, dim3(gx, gy, (unsigned)((p->nz + FDK_ZC - 1) / FDK_ZC)),
                            dim3(FDK_BX, FDK_BY), 0, s, p->fdk_dev, p->n_angles, p->rows, p->cols, q, vol,
