@@ -134,7 +134,9 @@ def c1_benchmark(torch, ts, reps=20):
 def _serve_both(reqs, warm):
     from tomosipo_amd import serving
 
-    # warm both servers' plans and kernels on a few requests first
+    # tune every request shape once (a one-off cost per shape in a long-running
+    # server), then warm both servers' plans and kernels on a few requests
+    serving.warm_plans(reqs)
     serving.Server().run(warm)
     serving.serve_one_at_a_time(warm)
     batched = serving.Server(hbm_budget_bytes=24 << 30).run(reqs)

@@ -106,10 +106,14 @@ int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);
 
 /* No counterpart in the reference (ASTRA has one kernel per geometry type).
  * Times the kernel variants of the plan's forward projector and
- * backprojector on its own shapes (at most 64 evenly spaced angles of its
- * geometry, on zero-filled scratch buffers allocated and freed on `stream`)
- * and keeps the fastest for later launches of this plan. Choices are cached
- * per (type, detector, volume) shape, so a second plan of the same shape
+ * backprojector on its own geometry and keeps the fastest for later launches
+ * of this plan: forward variants on every angle over bands of 64 and 32
+ * detector rows, compared on the predicted time of the whole plan (fixed
+ * layout-building cost plus cost per row); backprojector brick depths on at
+ * most 64 of its angles, in runs of consecutive angles from four quadrants of
+ * the scan. Trials use zero-filled scratch buffers allocated and freed on
+ * `stream`. Choices are cached per (type, detector, volume, angle count
+ * rounded up to a power of two) shape, so a second plan of the same shape
  * takes them without timing. Synchronises `stream`. Must not run while
  * launches of the same plan are being issued from another thread. The
  * TSX_FP_VARIANT and TSX_BP_BRICK_Z environment variables, when set,

@@ -4,6 +4,8 @@ Every request's result is bit-identical alone (one request per launch) and in
 the mix (fused launches with other requests); a small HBM budget forces
 deferred admission without changing results.
 """
+import dataclasses
+
 import pytest
 
 from tomosipo_amd import serving
@@ -37,3 +39,13 @@ def test_small_budget_defers_admission():
     assert out["deferred_admissions"] > 0
     for r in reqs:
         assert torch.equal(out["results"][r.rid], ref["results"][r.rid])
+
+
+def test_warm_plans_tunes_each_shape_once():
+    # two large requests (128^3 x 600 angles) are autotuned, the rest are not
+    big = serving.make_stream(2, rate=2000.0, seed=6, sides=(128,), angles=(600, 600), ops=("fp",))
+    reqs = _stream(12) + [dataclasses.replace(r, rid=100 + r.rid) for r in big]
+    shapes = {(r.kind, r.side, r.n_angles) for r in reqs}
+    assert serving.warm_plans(reqs) == len(shapes)
+    out = serving.Server(keep_results=True).run(reqs)
+    assert out["completed"] == len(reqs)

@@ -884,13 +884,24 @@ struct TuneChoice {
 std::mutex g_tune_mu;
 std::map<std::string, TuneChoice> g_tune_cache;
 
+// Angle counts are keyed by the next power of two: how well the forward
+// kernel's resident blocks share a tile's slab of the volume changes with the
+// angle count only gradually, and a server with many angle counts then tunes
+// a few shapes, not one per request.
+int angle_bucket(int n)
+{
+    int b = 1;
+    while (b < n && b < (1 << 30)) b <<= 1;
+    return b;
+}
+
 std::string tune_key(const tsx_plan *p)
 {
     return std::to_string(p->device) + ":" + std::to_string(p->host.empty() ? 0 : p->host[0].cone) +
            ":" + std::to_string(p->rows) + "x" + std::to_string(p->cols) + ":" + std::to_string(p->nx) +
            "x" + std::to_string(p->ny) + "x" + std::to_string(p->nz) + ":" + std::to_string(p->ss) + ":" +
            std::to_string((int)p->fp_padded) + std::to_string((int)p->fp_pair) +
-           std::to_string((int)p->bp_separable);
+           std::to_string((int)p->bp_separable) + ":" + std::to_string(angle_bucket(p->n_angles));
 }
 
 // Mean time of `reps` launches after one warm-up launch, in milliseconds.
@@ -915,10 +926,12 @@ int time_launches(F &&launch, hipStream_t stream, hipEvent_t e0, hipEvent_t e1, 
 // Owns the device allocations of one autotuning run.
 struct TuneScratch {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    void *vol = nullptr, *proj = nullptr;
-    AngleParams *angles = nullptr;
+    void *vol = nullptr, *proj = nullptr, *band_proj = nullptr;
+    AngleParams *angles = nullptr, *band_angles = nullptr;
     ~TuneScratch()
     {
+        if (band_proj) (void)hipFree(band_proj);
+        if (band_angles) (void)hipFree(band_angles);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (vol) (void)hipFree(vol);
@@ -939,7 +952,10 @@ int autotune(tsx_plan *p, hipStream_t stream)
             return TSX_OK;
         }
     }
-    // sub-plan over at most 64 evenly spaced angles of the plan's geometry
+    // Backprojector trials: a sub-plan of at most 64 of the plan's angles, in
+    // four runs of 16 consecutive angles (the plan's own angular spacing, so that
+    // the cache reuse between neighbouring angles is as in the full launch)
+    // starting in the four quadrants of the scan.
     tsx_plan sub;
     sub.device = p->device;
     sub.kind = p->kind;
@@ -949,8 +965,15 @@ int autotune(tsx_plan *p, hipStream_t stream)
     sub.ny = p->ny;
     sub.nz = p->nz;
     sub.ss = p->ss;
-    const int step = (p->n_angles + 63) / 64;
-    for (int a = 0; a < p->n_angles; a += step) {
+    const int N = p->n_angles;
+    std::vector<int> pick;
+    if (N <= 64) {
+        for (int a = 0; a < N; ++a) pick.push_back(a);
+    } else {
+        for (int k = 0; k < 4; ++k)
+            for (int i = 0; i < 16; ++i) pick.push_back((k * (N / 4) + i) % N);
+    }
+    for (int a : pick) {
         sub.host.push_back(p->host[a]);
         sub.axis_count[p->host[a].m] += 1;
     }
@@ -962,8 +985,11 @@ int autotune(tsx_plan *p, hipStream_t stream)
     TuneScratch t;
     const size_t vol_bytes = (size_t)p->nx * p->ny * p->nz * 4;
     const size_t proj_bytes = (size_t)p->rows * sub.n_angles * p->cols * 4;
+    const size_t band_bytes = (size_t)std::min(p->rows, 64) * p->n_angles * p->cols * 4;
     if (hipMalloc(&t.angles, sizeof(AngleParams) * sub.host.size()) != hipSuccess ||
-        hipMalloc(&t.vol, vol_bytes) != hipSuccess || hipMalloc(&t.proj, proj_bytes) != hipSuccess) {
+        hipMalloc(&t.band_angles, sizeof(AngleParams) * p->host.size()) != hipSuccess ||
+        hipMalloc(&t.vol, vol_bytes) != hipSuccess || hipMalloc(&t.proj, proj_bytes) != hipSuccess ||
+        hipMalloc(&t.band_proj, band_bytes) != hipSuccess) {
         (void)hipGetLastError();
         return fail(TSX_ERR_OUT_OF_MEMORY, "autotune: no device memory for the trial buffers");
     }
@@ -979,16 +1005,59 @@ int autotune(tsx_plan *p, hipStream_t stream)
     TuneChoice best{-1, -1};
     const bool fp_choices = p->fp_padded && p->ss >= 1 && p->ss <= 4 && !force_general();
     if (fp_choices) {
-        float best_ms = 1e30f;
+        // The forward kernel walks all angles of one detector tile before the
+        // next tile, so its cache reuse depends on the angle count: the
+        // candidates are timed on every angle of the plan, over bands of 64 and
+        // 32 detector rows through the middle of the detector (the band's
+        // pixel (0, 0) moved by r0 rows). A launch costs a fixed part (building
+        // the padded layouts, larger for the row-paired ones) plus a part per
+        // row; both are fitted from the two bands and the candidates compared
+        // on the predicted time of the whole plan.
+        const int band_hi = std::min(p->rows, 64), band_lo = std::min(p->rows, 32);
+        std::vector<AngleParams> shifted(p->host);
+        tsx_plan band;
+        band.device = p->device;
+        band.kind = p->kind;
+        band.cols = p->cols;
+        band.nx = p->nx;
+        band.ny = p->ny;
+        band.nz = p->nz;
+        band.ss = p->ss;
+        band.n_angles = p->n_angles;
+        for (int c = 0; c < 3; ++c) {
+            band.axis_count[c] = p->axis_count[c];
+            band.layout_elems[c] = p->layout_elems[c];
+        }
+        band.fp_padded = p->fp_padded;
+        band.fp_pair = p->fp_pair;
+        band.dev = t.band_angles;
+        auto time_band = [&](int cand, int band_rows, float *ms) -> int {
+            const int r0 = (p->rows - band_rows) / 2;
+            for (int a = 0; a < p->n_angles; ++a)
+                for (int k = 0; k < 3; ++k) shifted[a].P0[k] = p->host[a].P0[k] + r0 * p->host[a].V[k];
+            TSX_HIP_CHECK(hipMemcpyAsync(t.band_angles, shifted.data(), sizeof(AngleParams) * shifted.size(),
+                                         hipMemcpyHostToDevice, stream));
+            band.rows = band_rows;
+            band.fp_variant = cand;
+            return time_launches([&] { return launch_fp(&band, vol, 0, (float *)t.band_proj, 0, 1, ep, stream); },
+                                 stream, t.e0, t.e1, 2, ms);
+        };
+        double best_ms = 1e300;
         for (int cand : {4, 3, 2}) {
             if (cand == 4 && !p->fp_pair) continue;
-            sub.fp_variant = cand;
-            float ms = 0.0f;
-            int rc = time_launches([&] { return launch_fp(&sub, vol, 0, proj, 0, 1, ep, stream); },
-                                   stream, t.e0, t.e1, 2, &ms);
+            float ms_hi = 0.0f, ms_lo = 0.0f;
+            int rc = time_band(cand, band_hi, &ms_hi);
             if (rc) return rc;
-            if (ms < best_ms) {
-                best_ms = ms;
+            double predicted = ms_hi;
+            if (band_hi < p->rows) {
+                rc = time_band(cand, band_lo, &ms_lo);
+                if (rc) return rc;
+                const double per_row = std::max(0.0, (double)(ms_hi - ms_lo) / (band_hi - band_lo));
+                const double fixed = std::max(0.0, ms_hi - per_row * band_hi);
+                predicted = fixed + per_row * p->rows;
+            }
+            if (predicted < best_ms) {
+                best_ms = predicted;
                 best.fp_variant = cand;
             }
         }

@@ -26,8 +26,8 @@ from .links.base import are_compatible
 
 
 # Plans with at least this many voxel-angles are autotuned on their device
-# when first used (a one-off cost of a few launches over at most 64 angles,
-# cached per shape by the library).
+# when first used (a one-off cost of a few trial launches on bands of detector
+# rows and on at most 64 angles, cached per shape by the library).
 AUTOTUNE_MIN_VOXEL_ANGLES = 1 << 30
 
 

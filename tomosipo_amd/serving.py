@@ -249,6 +249,25 @@ class Server:
         return _summary(records, results, deferred)
 
 
+def warm_plans(requests, device=0):
+    """Create the device plan of every distinct request shape in ``requests``,
+    which autotunes the large ones once (the library caches the choices per
+    shape), so that no served request pays for the tuning of its shape."""
+    import torch
+
+    from .Operator import operator
+
+    seen = set()
+    with torch.cuda.device(torch.device("cuda", device)):
+        for r in requests:
+            key = (r.kind, r.side, r.n_angles)
+            if key not in seen:
+                seen.add(key)
+                operator(*request_geometry(r)).projector.plan(device)
+        torch.cuda.synchronize()
+    return len(seen)
+
+
 def serve_one_at_a_time(requests, device=0, keep_results=False):
     """Reference server: arrival order, one request at a time, one launch per operator call."""
     import torch
