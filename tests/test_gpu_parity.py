@@ -71,6 +71,23 @@ def test_cone_vec_helix_jitter():
     _check_fp_bp(vg, pg, x)
 
 
+@pytest.mark.parametrize("kind", ["parallel", "cone"])
+def test_truncated_detector(kind):
+    """A detector much narrower than the volume: for about a third of the
+    (brick, angle) pairs of the separable backprojector no detector column
+    meets the brick, and some bricks meet none at one angle and some at the
+    next (the slice ranges must still advance through such angles)."""
+    vg = ts.volume(shape=(20, 48, 48), size=(20 / 48, 1, 1))
+    if kind == "parallel":
+        pg = ts.parallel(angles=36, shape=(24, 6), size=(1.0, 0.3)).to_vec()
+    else:
+        pg = ts.cone(angles=36, shape=(24, 8), size=(1.5, 0.4), src_orig_dist=3,
+                     src_det_dist=5).to_vec()
+    assert ts.operator(vg, pg).projector.plan(0).info()["bp_kernel"] == "separable"
+    x = ts.phantom.box_noise((20, 48, 48), 8, 40, 0.01, seed=4)
+    _check_fp_bp(vg, pg, x)
+
+
 @pytest.mark.parametrize("fp_variant,brick_z", [("2", "32"), ("3", "16"), ("4", "32")])
 def test_autotuner_alternatives_match_oracle(monkeypatch, fp_variant, brick_z):
     """The kernel variants the autotuner may choose besides the defaults: the

@@ -207,17 +207,20 @@ struct BpView {
     Epilogue ep;
 };
 
-// LDS layout of bp_separable: a column of u holds kz = -1 .. BZ at offsets
-// 3 .. BZ + 4; columns are BZ + 4 floats apart (the kz = BZ guard of one
-// column falls on offset 0 of the next, which no column uses) and slices
-// BR * (BZ + 4) + 2 floats apart (= 2 mod 64). The strides were chosen with a
-// simulation of the kernel's LDS addresses on a cone-beam geometry (dword
-// banks of 64): this slice stride spreads the z-cells that the lanes of a
-// pass-1 wave (16 columns of 4 slices) and of a pass-2 wave (16 slices) touch
-// at the same time over the banks, while every column stays 8-byte aligned so
-// that pass 2 reads its cells as float2. Only the cells kz = 0 .. BZ - 1 are
-// read; the guard cells kz = -1 and kz = BZ take the splat's clamped rows.
-constexpr int UOFF = 4;  // kz = 0 sits at offset 4 of a column; kz = -1 at 3
+// LDS layout of bp_separable: a column of u holds kz = -4 .. BZ - 1 at
+// offsets 0 .. BZ + 3 (kz = 0 at offset UOFF); columns are BZ + 4 floats
+// apart and slices BR * (BZ + 4) + 2 floats apart (= 2 mod 64). Only the
+// cells kz = 0 .. BZ - 1 are read. The splat clamps the lower cell of a row
+// to [-3, BZ], so the weights of rows outside the brick land in cells
+// -3 .. -1 of their own column or in cells BZ, BZ + 1, which are offsets 0
+// and 1 of the next column (kz = -4, -3 there) or the 2-float gap after a
+// slice's last column: cells that are never read. The strides were chosen
+// with a simulation of the kernel's LDS addresses on a cone-beam geometry
+// (dword banks of 64): this slice stride spreads the z-cells that the lanes
+// of a pass-1 wave (16 columns of 4 slices) and of a pass-2 wave (16 slices)
+// touch at the same time over the banks, while every column stays 8-byte
+// aligned so that pass 2 reads its cells as float2.
+constexpr int UOFF = 4;  // kz = 0 sits at offset 4 of a column
 
 template <int BZ>
 struct BpShared {
@@ -299,6 +302,13 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
         const int m0 = mx ? bx * BR : by * BR;
         const int c0 = mx ? by * BR : bx * BR;
         const int chunks = sh.chunks[buf];
+        if (chunks == 0) {
+            // no detector column of this angle meets the brick: only the next
+            // angle's slice ranges are due (uniform branch over the block)
+            if (t < BR && a + 1 < n_angles) bp_ranges(angles[a + 1], bx, by, cols, t, sh, buf ^ 1);
+            __syncthreads();
+            continue;
+        }
         for (int c = 0; c < chunks; ++c) {
             // pass 1: thread (s = ty, jj = tx) splats detector column j onto kz
             {
@@ -308,10 +318,11 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                 const int pos = (!valid || sh.inc[buf][s]) ? jj : m - 1 - jj;
                 const int j = sh.jlo[buf][s] + c * BR + jj;
                 float *col = &sh.u[s * S::SLICE + pos * S::UCOL];
-                float2 *c2 = reinterpret_cast<float2 *>(col + UOFF);
-#pragma unroll
-                for (int k = 0; k < BZ / 2; ++k) c2[k] = make_float2(0.0f, 0.0f);
                 if (valid) {
+                    // slots without a column are never read (q2 = 1e30)
+                    float2 *c2 = reinterpret_cast<float2 *>(col + UOFF);
+#pragma unroll
+                    for (int k = 0; k < BZ / 2; ++k) c2[k] = make_float2(0.0f, 0.0f);
                     const double n = (double)(m0 + s);
                     const double Pm = A.P0[0] + j * A.U[0];
                     const double Pz = A.P0[1] + j * A.U[1];
@@ -340,24 +351,27 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                     uint32_t off = (uint32_t)i0 * row_bytes + ((uint32_t)a * (uint32_t)cols + (uint32_t)j) * 4u;
                     float *u = col + UOFF;
                     // rows in groups of four, the loads of the next group in
-                    // flight while the current one is splatted; rows past i1
-                    // are read (inside the buffer or zero by the range check)
-                    // and given weight zero
+                    // flight while the current one is splatted. Rows past i1
+                    // are read too (inside the buffer, or zero by the range
+                    // check); their t lies beyond the brick (t >= BZ when
+                    // Bz > 0, t <= -1 when Bz < 0), and the clamp of kf to
+                    // [-3, BZ] sends their weights to cells that pass 2 never
+                    // reads. For t inside [-3, BZ + 1), kf = floor(t) and
+                    // w = fract(t) are the exact bilinear split.
                     auto load4 = [&](uint32_t o, float (&yv)[4]) {
 #pragma unroll
                         for (int r = 0; r < 4; ++r)
                             yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
                                                                   rs, o + (uint32_t)r * row_bytes, 0, 0));
                     };
-                    auto splat4 = [&](int i, float tb, const float (&yv)[4]) {
+                    auto splat4 = [&](float tb, const float (&yv)[4]) {
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const float ti = fmaf((float)r, fB, tb);
-                            const int kf = min(max(floor_to_int(ti), -1), BZ - 1);
-                            const float w = ti - (float)kf;
-                            const float yy = (i + r <= i1) ? yv[r] : 0.0f;
-                            u[kf] = fmaf(1.0f - w, yy, u[kf]);
-                            u[kf + 1] = fmaf(w, yy, u[kf + 1]);
+                            const int kf = min(max(floor_to_int(ti), -3), BZ);
+                            const float w = __builtin_amdgcn_fractf(ti);
+                            u[kf] = fmaf(1.0f - w, yv[r], u[kf]);
+                            u[kf + 1] = fmaf(w, yv[r], u[kf + 1]);
                         }
                     };
                     float ya[4], yb[4];
@@ -366,11 +380,11 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                     for (int i = i0; i <= i1; i += 8) {
                         load4(off + 4u * row_bytes, yb);
                         __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the splat
-                        splat4(i, tb, ya);
+                        splat4(tb, ya);
                         if (i + 4 > i1) break;
                         load4(off + 8u * row_bytes, ya);
                         __builtin_amdgcn_sched_barrier(0);
-                        splat4(i + 4, fmaf(4.0f, fB, tb), yb);
+                        splat4(fmaf(4.0f, fB, tb), yb);
                         off += 8u * row_bytes;
                         tb = fmaf(8.0f, fB, tb);
                     }
