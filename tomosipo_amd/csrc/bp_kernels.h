@@ -185,7 +185,8 @@ __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_an
 
 // Detector column where the ray through in-slice point (n, c) (axes m, c2)
 // meets the detector. Independent of z for an upright detector.
-__device__ __forceinline__ double column_of(const AngleParams &A, double n, double c)
+template <typename AP>
+__device__ __forceinline__ double column_of(const AP &A, double n, double c)
 {
     if (A.cone) {
         const double W0 = n - A.S[0], W2 = c - A.S[2];
@@ -237,7 +238,8 @@ struct BpShared {
 
 
 // Slice range of slice t of a brick (first column, count, direction).
-__device__ __forceinline__ void bp_slice_range(const AngleParams &A, int bx, int by, int cols, int t,
+template <typename AP>
+__device__ __forceinline__ void bp_slice_range(const AP &A, int bx, int by, int cols, int t,
                                                 int &jlo, int &cnt, int &inc)
 {
     const bool mx = A.m == 0;
@@ -256,8 +258,8 @@ __device__ __forceinline__ void bp_slice_range(const AngleParams &A, int bx, int
 // Slice ranges of one angle for the 16 slices of a brick, computed by lanes
 // t = 0 .. BR - 1 of the first wave (all active together), which also publish
 // the number of column chunks.
-template <int BZ>
-__device__ __forceinline__ void bp_ranges(const AngleParams &A, int bx, int by, int cols, int t,
+template <int BZ, typename AP>
+__device__ __forceinline__ void bp_ranges(const AP &A, int bx, int by, int cols, int t,
                                           BpShared<BZ> &sh, int buf)
 {
     int jlo, cnt, inc;
@@ -281,7 +283,13 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
     const int bz = (logical / (v.bricks_x * v.bricks_y)) % v.bricks_z;
     const int b = logical / (v.bricks_x * v.bricks_y * v.bricks_z);
     if (b >= v.batch) return;
-    const AngleParams *__restrict__ angles = v.angles;
+    // The angle parameters do not change during the launch. Read through a
+    // constant-address-space pointer, the uniform ones become scalar loads
+    // into SGPRs; through a generic pointer the barriers of the angle loop
+    // (memory fences) make the compiler read them with vector loads into
+    // VGPRs (7 % slower at 32-deep bricks, 22 % at 16-deep ones).
+    typedef const __attribute__((address_space(4))) AngleParams ConstAngle;
+    ConstAngle *__restrict__ angles = (ConstAngle *)v.angles;
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x = bx * BR + tx, y = by * BR + ty, z0 = bz * BZ;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
@@ -297,7 +305,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
     __syncthreads();
     for (int a = 0; a < n_angles; ++a) {
         const int buf = a & 1;
-        const AngleParams &A = angles[a];
+        ConstAngle &A = angles[a];
         const bool mx = A.m == 0;
         const int m0 = mx ? bx * BR : by * BR;
         const int c0 = mx ? by * BR : bx * BR;
