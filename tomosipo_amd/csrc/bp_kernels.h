@@ -183,6 +183,16 @@ __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_an
     out[(int64_t)b * out_stride + o] = len * proj[(int64_t)b * proj_stride + o];
 }
 
+// 1/d to FP64 precision: an FP32 reciprocal as the seed (relative error
+// below 3e-7) and one Newton step, which squares the error. Cheaper than an
+// FP64 division; d is a ray extent of the order of the volume, far from the
+// FP32 range limits.
+__device__ __forceinline__ double rcp64(double d)
+{
+    const double r = (double)__builtin_amdgcn_rcpf((float)d);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
 // Detector column where the ray through in-slice point (n, c) (axes m, c2)
 // meets the detector. Independent of z for an upright detector.
 template <typename AP>
@@ -190,10 +200,10 @@ __device__ __forceinline__ double column_of(const AP &A, double n, double c)
 {
     if (A.cone) {
         const double W0 = n - A.S[0], W2 = c - A.S[2];
-        const double lam = A.kappa / (W0 * A.N[0] + W2 * A.N[2]);
+        const double lam = A.kappa * rcp64(W0 * A.N[0] + W2 * A.N[2]);
         return (A.S[0] + lam * W0 - A.P0[0]) * A.Bj[0] + (A.S[2] + lam * W2 - A.P0[2]) * A.Bj[2];
     }
-    const double t0 = ((A.P0[0] - n) * A.N[0] + (A.P0[2] - c) * A.N[2]) / A.EN;
+    const double t0 = ((A.P0[0] - n) * A.N[0] + (A.P0[2] - c) * A.N[2]) * rcp64(A.EN);
     return (n + t0 * A.S[0] - A.P0[0]) * A.Bj[0] + (c + t0 * A.S[2] - A.P0[2]) * A.Bj[2];
 }
 
@@ -331,31 +341,40 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                     float2 *c2 = reinterpret_cast<float2 *>(col + UOFF);
 #pragma unroll
                     for (int k = 0; k < BZ / 2; ++k) c2[k] = make_float2(0.0f, 0.0f);
-                    const double n = (double)(m0 + s);
-                    const double Pm = A.P0[0] + j * A.U[0];
-                    const double Pz = A.P0[1] + j * A.U[1];
-                    const double P2 = A.P0[2] + j * A.U[2];
+                    // ray of detector row 0 in column j: slice n is met at the
+                    // in-slice point (Az, Q2); row i adds i * Bz along z
+                    const double n = (double)(m0 + s), jd = (double)j;
                     double Az, Bz, Q2;
                     if (A.cone) {
-                        const double tau = (n - A.S[0]) / (Pm - A.S[0]);
-                        Az = A.S[1] + tau * (Pz - A.S[1]);
+                        const double Dm = (A.P0[0] - A.S[0]) + jd * A.U[0];
+                        const double Dz = (A.P0[1] - A.S[1]) + jd * A.U[1];
+                        const double D2 = (A.P0[2] - A.S[2]) + jd * A.U[2];
+                        const double tau = (n - A.S[0]) * rcp64(Dm);
+                        Az = A.S[1] + tau * Dz;
                         Bz = tau * A.V[1];
-                        Q2 = A.S[2] + tau * (P2 - A.S[2]);
+                        Q2 = A.S[2] + tau * D2;
                     } else {
-                        const double tau = (n - Pm) / A.S[0];
-                        Az = Pz + tau * A.S[1];
+                        const double Pm = A.P0[0] + jd * A.U[0];
+                        const double tau = (n - Pm) * rcp64(A.S[0]);
+                        Az = (A.P0[1] + jd * A.U[1]) + tau * A.S[1];
                         Bz = A.V[1];
-                        Q2 = P2 + tau * A.S[2];
+                        Q2 = (A.P0[2] + jd * A.U[2]) + tau * A.S[2];
                     }
                     sh.q2[s][pos] = (float)(Q2 - (double)c0);
-                    // rows with t_i = Arel + i * Bz in (-1, BZ)
+                    // rows with t_i = Arel + i * Bz in (-1, BZ), bounded in FP32
+                    // with a margin that exceeds its rounding: a row just outside
+                    // is harmless (its t lies beyond the brick, see the splat
+                    // below), and no row inside is lost
                     const double Arel = Az - (double)z0;
-                    const double iBz = 1.0 / Bz;
-                    const double ra = (-1.0 - Arel) * iBz, rb = ((double)BZ - Arel) * iBz;
-                    const int i0 = max(0, (int)ceil(fmax(fmin(ra, rb), -1e9)));
-                    const int i1 = min(rows - 1, (int)floor(fmin(fmax(ra, rb), 1e9)));
-                    const float t0 = (float)(Arel + (double)i0 * Bz);
                     const float fB = (float)Bz;
+                    const float fA = (float)Arel, iB = __builtin_amdgcn_rcpf(fB);
+                    const float ra = (-1.0f - fA) * iB, rb = ((float)BZ - fA) * iB;
+                    float lo = fminf(ra, rb), hi = fmaxf(ra, rb);
+                    lo -= 1e-3f + 1e-5f * fabsf(lo);
+                    hi += 1e-3f + 1e-5f * fabsf(hi);
+                    const int i0 = max(0, (int)ceilf(fmaxf(lo, -1e9f)));
+                    const int i1 = min(rows - 1, (int)floorf(fminf(hi, 1e9f)));
+                    const float t0 = (float)(Arel + (double)i0 * Bz);
                     uint32_t off = (uint32_t)i0 * row_bytes + ((uint32_t)a * (uint32_t)cols + (uint32_t)j) * 4u;
                     float *u = col + UOFF;
                     // rows in groups of four, the loads of the next group in
