@@ -88,6 +88,19 @@ def test_truncated_detector(kind):
     _check_fp_bp(vg, pg, x)
 
 
+@pytest.mark.parametrize("rows", [64, 39, 38, 24, 18])
+def test_backprojector_row_pitch(rows):
+    """The separable backprojector splats detector rows onto the z-grid in two
+    orders: row by row, or rows two apart together when a row step exceeds
+    1.01 voxels. Parallel beam, 32^3 unit volume, detector 1.2 high: the row
+    step is 38.4 / rows voxels (0.6, 0.985, 1.0105, 1.6, 2.13)."""
+    vg = ts.volume(shape=32, size=1)
+    pg = ts.parallel(angles=24, shape=(rows, 48), size=(1.2, 1.5)).to_vec()
+    assert ts.operator(vg, pg).projector.plan(0).info()["bp_kernel"] == "separable"
+    x = ts.phantom.ellipsoids(32, count=6, seed=rows)
+    _check_fp_bp(vg, pg, x)
+
+
 @pytest.mark.parametrize("fp_variant,brick_z", [("2", "32"), ("3", "16"), ("4", "32")])
 def test_autotuner_alternatives_match_oracle(monkeypatch, fp_variant, brick_z):
     """The kernel variants the autotuner may choose besides the defaults: the

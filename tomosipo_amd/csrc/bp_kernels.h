@@ -32,6 +32,11 @@
 namespace tsx {
 
 constexpr int BP_BX = 64, BP_BY = 4;
+
+template <bool B>
+struct Flag {
+    static constexpr bool value = B;
+};
 constexpr int BR = 16;  // brick edge (x, y) of bp_separable
 
 __device__ __forceinline__ int sel3(int axis, int x, int y, int z)
@@ -391,30 +396,64 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                             yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
                                                                   rs, o + (uint32_t)r * row_bytes, 0, 0));
                     };
-                    auto splat4 = [&](float tb, const float (&yv)[4]) {
+                    // One group of four rows. In order, each row's two cells
+                    // are read, updated and written before the next row's
+                    // (neighbouring rows may share a cell), so every row waits
+                    // for an LDS round trip. With PAIRED, rows 0 and 2, then
+                    // 1 and 3, are read together: rows two apart touch
+                    // disjoint cells when |Bz| > 1.01 (their t differ by more
+                    // than 2; rows clamped to the same edge cell share only
+                    // cells that are never read).
+                    auto splat4 = [&](float tb, const float (&yv)[4], auto paired) {
+                        int kf[4];
+                        float w[4];
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const float ti = fmaf((float)r, fB, tb);
-                            const int kf = min(max(floor_to_int(ti), -3), BZ);
-                            const float w = __builtin_amdgcn_fractf(ti);
-                            u[kf] = fmaf(1.0f - w, yv[r], u[kf]);
-                            u[kf + 1] = fmaf(w, yv[r], u[kf + 1]);
+                            kf[r] = min(max(floor_to_int(ti), -3), BZ);
+                            w[r] = __builtin_amdgcn_fractf(ti);
+                        }
+                        if constexpr (decltype(paired)::value) {
+#pragma unroll
+                            for (int h = 0; h < 2; ++h) {
+                                const int p = h, q = h + 2;
+                                const float p0 = u[kf[p]], p1 = u[kf[p] + 1];
+                                const float q0 = u[kf[q]], q1 = u[kf[q] + 1];
+                                u[kf[p]] = fmaf(1.0f - w[p], yv[p], p0);
+                                u[kf[p] + 1] = fmaf(w[p], yv[p], p1);
+                                u[kf[q]] = fmaf(1.0f - w[q], yv[q], q0);
+                                u[kf[q] + 1] = fmaf(w[q], yv[q], q1);
+                            }
+                        } else {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                u[kf[r]] = fmaf(1.0f - w[r], yv[r], u[kf[r]]);
+                                u[kf[r] + 1] = fmaf(w[r], yv[r], u[kf[r] + 1]);
+                            }
                         }
                     };
-                    float ya[4], yb[4];
-                    float tb = t0;
-                    load4(off, ya);
-                    for (int i = i0; i <= i1; i += 8) {
-                        load4(off + 4u * row_bytes, yb);
-                        __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the splat
-                        splat4(tb, ya);
-                        if (i + 4 > i1) break;
-                        load4(off + 8u * row_bytes, ya);
-                        __builtin_amdgcn_sched_barrier(0);
-                        splat4(fmaf(4.0f, fB, tb), yb);
-                        off += 8u * row_bytes;
-                        tb = fmaf(8.0f, fB, tb);
-                    }
+                    auto splat_rows = [&](auto paired) {
+                        float ya[4], yb[4];
+                        float tb = t0;
+                        uint32_t o = off;
+                        load4(o, ya);
+                        for (int i = i0; i <= i1; i += 8) {
+                            load4(o + 4u * row_bytes, yb);
+                            __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the splat
+                            splat4(tb, ya, paired);
+                            if (i + 4 > i1) break;
+                            load4(o + 8u * row_bytes, ya);
+                            __builtin_amdgcn_sched_barrier(0);
+                            splat4(fmaf(4.0f, fB, tb), yb, paired);
+                            o += 8u * row_bytes;
+                            tb = fmaf(8.0f, fB, tb);
+                        }
+                    };
+                    // the paired order when every column of the wave allows it
+                    if (__builtin_amdgcn_ballot_w64(fabsf(fB) > 1.01f) == __builtin_amdgcn_read_exec())
+                        splat_rows(Flag<true>());
+                    else
+                        splat_rows(Flag<false>());
                 } else {
                     sh.q2[s][pos] = 1e30f;
                 }
