@@ -46,6 +46,22 @@ def test_fdk_matches_oracle(kind):
     assert isinstance(out_np, np.ndarray) and rel_l2(out_np, ref) <= 1e-5
 
 
+@pytest.mark.parametrize("cols", [787, 1536])
+def test_fdk_wide_rows(cols):
+    """Detector rows wider than one pass of the register-tiled ramp filter
+    (768 outputs per pass), one of them of odd width."""
+    vg = ts.volume(shape=(8, 20, 20), size=(0.4, 1, 1))
+    pg = ts.cone(angles=12, shape=(10, cols), size=(0.6, 1.8), cone_angle=0.3)
+    A = ts.operator(vg, pg)
+    g = oracle.oracle_geometry(vg, pg)
+    x = ts.phantom.box_noise((8, 20, 20), 2, 16, 0.01, seed=cols)
+    y = oracle.fp(g, x).astype(np.float32)
+    ref = F.fdk(g, y)
+    out = ts.fdk(A, torch.from_numpy(y).cuda())
+    print(f"[fdk] cols={cols} rel L2 {rel_l2(out.cpu().numpy(), ref):.2e}", flush=True)
+    assert rel_l2(out.cpu().numpy(), ref) <= 1e-5
+
+
 def test_fdk_reconstructs_a_ball():
     N = 64
     vg = ts.volume(shape=N, size=1)

@@ -16,24 +16,59 @@
 
 namespace tsx {
 
-constexpr int FDK_FILTER_THREADS = 256;
-constexpr int FDK_MAX_COLS = 4096;  // LDS: (3 cols + cols / 2) floats per block
+constexpr int FDK_FILTER_THREADS = 128;
+constexpr int FDK_FR = 6;   // consecutive outputs per filter thread
+constexpr int FDK_FK = 8;   // odd lags per register chunk
+constexpr int FDK_MAX_COLS = 4096;
+constexpr int FDK_FW = FDK_FR + 2 * FDK_FK;  // window of one side per chunk, floats
+constexpr int FDK_PASS = FDK_FILTER_THREADS * FDK_FR;  // outputs per pass over a row
+
+// LDS layout of fdk_filter for a row of `cols` pixels: the weighted row at
+// offset fdk_zl(cols) between zeros (wide enough for every lag of every
+// output of the last pass), then the lag coefficients.
+__host__ __device__ inline int fdk_nodd_pad(int cols)
+{
+    return (cols / 2 + FDK_FK - 1) / FDK_FK * FDK_FK;
+}
+__host__ __device__ inline int fdk_zl(int cols) { return 2 * fdk_nodd_pad(cols) + 2; }
+__host__ __device__ inline int fdk_lds_floats(int cols)
+{
+    const int passes = (cols + FDK_PASS - 1) / FDK_PASS;
+    return fdk_zl(cols) + passes * FDK_PASS + 2 * fdk_nodd_pad(cols) + FDK_FW + fdk_nodd_pad(cols);
+}
+
 constexpr int FDK_BX = 64, FDK_BY = 4;
 
 // With `pair`, q is written row-paired: element (i, a, j) holds the filtered
 // values of rows i and i + 1 (0 past the last row), 8 bytes.
+//
+// The convolution is register-tiled: thread t computes the outputs
+// j0 .. j0 + FDK_FR - 1 (j0 = FDK_FR t within a pass) over the odd lags
+// m = 2k + 1 in chunks of FDK_FK, from two windows of the row that it reads
+// once per chunk (the samples j - m and j + m of all its outputs and lags in
+// the chunk), as 8-byte LDS reads; thread starts FDK_FR = 6 floats apart keep
+// them free of bank conflicts. One LDS read per output and lag instead of
+// three.
+//
+// The sum is taken in difference form: the kernel has no DC response
+// (g[0] + 2 sum of g[odd m] = 0 over all lags), so
+//   q[j] = (0.25 - 2 sum_k coef_k) c[j] - sum_k coef_k (c[j - m] + c[j + m] - 2 c[j]),
+// with the first factor (`tail`, over the lags summed here) computed in FP64
+// by the caller. For a row that is smooth on the pixel scale the plain sum
+// nearly cancels 0.25 c[j] and loses about ten times more to FP32 rounding.
 __global__ void __launch_bounds__(FDK_FILTER_THREADS)
 fdk_filter(const FdkAngle *__restrict__ angles, int n_angles, int rows, int cols,
-           const float *__restrict__ proj, float *__restrict__ q, int pair)
+           const float *__restrict__ proj, float *__restrict__ q, int pair, float tail)
 {
-    extern __shared__ float lds[];
-    float *buf = lds;             // [0, cols): 0 | [cols, 2 cols): weighted row | [2 cols, 3 cols): 0
-    float *coef = lds + 3 * cols; // coef[k] = 1 / (pi^2 (2k + 1)^2)
+    extern __shared__ __align__(16) float lds[];
+    const int nodd_pad = fdk_nodd_pad(cols), zl = fdk_zl(cols), nbuf = fdk_lds_floats(cols) - nodd_pad;
+    float *buf = lds;           // buf[zl + j]: weighted pixel j; zeros elsewhere
+    float *coef = lds + nbuf;   // coef[k] = 1 / (pi^2 (2k + 1)^2)
     const int i = blockIdx.x, a = blockIdx.y;
     const FdkAngle &A = angles[a];
     const int64_t row = ((int64_t)i * n_angles + a) * cols;
-    for (int k = threadIdx.x; k < 3 * cols; k += blockDim.x) {
-        const int j = k - cols;
+    for (int k = threadIdx.x; k < nbuf; k += blockDim.x) {
+        const int j = k - zl;
         float v = 0.0f;
         if (j >= 0 && j < cols) {
             const double px = A.P0s[0] + j * A.U[0] + i * A.V[0];
@@ -43,28 +78,58 @@ fdk_filter(const FdkAngle *__restrict__ angles, int n_angles, int rows, int cols
         }
         buf[k] = v;
     }
-    const int nodd = cols / 2;  // odd lags 1, 3, ..., <= cols - 1
-    for (int k = threadIdx.x; k < nodd; k += blockDim.x) {
+    for (int k = threadIdx.x; k < nodd_pad; k += blockDim.x) {
         const double m = 2.0 * k + 1.0;
         coef[k] = (float)(1.0 / (M_PI * M_PI * m * m));
     }
     __syncthreads();
     const float inv_du = (float)A.inv_du;
-    for (int j = threadIdx.x; j < cols; j += blockDim.x) {
-        const float *c = buf + cols + j;
-        float part = 0.0f;
-        for (int k = 0; k < nodd; ++k) {
-            const int m = 2 * k + 1;
-            part = fmaf(coef[k], c[-m] + c[m], part);
+    for (int base = 0; base < cols; base += FDK_PASS) {
+        const int j0 = base + FDK_FR * (int)threadIdx.x;  // even
+        float part[FDK_FR], c2[FDK_FR];
+#pragma unroll
+        for (int r = 0; r < FDK_FR; ++r) {
+            part[r] = 0.0f;
+            c2[r] = 2.0f * buf[zl + j0 + r];
         }
-        const float v = inv_du * fmaf(0.25f, c[0], -part);
-        if (!pair) {
-            q[row + j] = v;
-        } else {
-            const int64_t prev = ((int64_t)(i - 1) * n_angles + a) * cols;
-            q[2 * (row + j)] = v;
-            if (i > 0) q[2 * (prev + j) + 1] = v;
-            if (i == rows - 1) q[2 * (row + j) + 1] = 0.0f;
+        for (int k0 = 0; k0 < nodd_pad; k0 += FDK_FK) {
+            // left window from j0 - 2 (k0 + FDK_FK), right window from j0 + 2 k0 (both even)
+            const float2 *lw = reinterpret_cast<const float2 *>(buf + zl + j0 - 2 * (k0 + FDK_FK));
+            const float2 *rw = reinterpret_cast<const float2 *>(buf + zl + j0 + 2 * k0);
+            float L[FDK_FW], R[FDK_FW], ck[FDK_FK];
+#pragma unroll
+            for (int x = 0; x < FDK_FW / 2; ++x) {
+                const float2 l2 = lw[x], r2 = rw[x];
+                L[2 * x] = l2.x;
+                L[2 * x + 1] = l2.y;
+                R[2 * x] = r2.x;
+                R[2 * x + 1] = r2.y;
+            }
+#pragma unroll
+            for (int kk = 0; kk < FDK_FK; ++kk) ck[kk] = coef[k0 + kk];
+#pragma unroll
+            for (int kk = 0; kk < FDK_FK; ++kk) {
+#pragma unroll
+                for (int r = 0; r < FDK_FR; ++r) {
+                    // samples j - m and j + m, j = j0 + r, m = 2 (k0 + kk) + 1
+                    const float sum = (L[r - 2 * kk - 1 + 2 * FDK_FK] + R[r + 2 * kk + 1]) - c2[r];
+                    part[r] = fmaf(ck[kk], sum, part[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < FDK_FR; ++r) {
+            const int j = j0 + r;
+            if (j >= cols) break;
+            const float v = inv_du * fmaf(tail, buf[zl + j], -part[r]);
+            if (!pair) {
+                q[row + j] = v;
+            } else {
+                const int64_t prev = ((int64_t)(i - 1) * n_angles + a) * cols;
+                q[2 * (row + j)] = v;
+                if (i > 0) q[2 * (prev + j) + 1] = v;
+                if (i == rows - 1) q[2 * (row + j) + 1] = 0.0f;
+            }
         }
     }
 }

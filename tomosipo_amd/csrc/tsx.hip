@@ -1242,9 +1242,15 @@ int tsx_fdk(const tsx_plan *p, float *vol, const float *proj, int additive, void
     rc = get_workspace(p, need, workspace, workspace_bytes, s, owned, &wptr);
     if (rc) return rc;
     float *q = (float *)wptr;
-    const size_t lds = (size_t)(3 * p->cols + p->cols / 2) * sizeof(float);
+    const size_t lds = (size_t)fdk_lds_floats(p->cols) * sizeof(float);
+    double tail = 0.25;  // 0.25 - 2 * sum of the filter's lag coefficients
+    for (int k = fdk_nodd_pad(p->cols) - 1; k >= 0; --k) {
+        const double m = 2.0 * k + 1.0;
+        tail -= 2.0 / (M_PI * M_PI * m * m);
+    }
     hipLaunchKernelGGL(fdk_filter, dim3((unsigned)p->rows, (unsigned)p->n_angles), dim3(FDK_FILTER_THREADS),
-                       lds, s, p->fdk_dev, p->n_angles, p->rows, p->cols, proj, q, upright ? 1 : 0);
+                       lds, s, p->fdk_dev, p->n_angles, p->rows, p->cols, proj, q, upright ? 1 : 0,
+                       (float)tail);
     TSX_HIP_CHECK(hipGetLastError());
     const unsigned gx = (unsigned)((p->nx + FDK_BX - 1) / FDK_BX), gy = (unsigned)((p->ny + FDK_BY - 1) / FDK_BY);
     if (upright)
