@@ -44,6 +44,35 @@ __device__ __forceinline__ int sel3(int axis, int x, int y, int z)
     return axis == 0 ? x : (axis == 1 ? y : z);
 }
 
+// 1/d to FP64 precision: an FP32 reciprocal as the seed (relative error
+// below 3e-7) and one Newton step, which squares the error. Cheaper than an
+// FP64 division; d is a ray extent of the order of the volume, far from the
+// FP32 range limits.
+__device__ __forceinline__ double rcp64(double d)
+{
+    const double r = (double)__builtin_amdgcn_rcpf((float)d);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
+// Arithmetic of bp_general by precision: the float kernel takes the FP64
+// reciprocal above for its FP64 setup and the hardware FP32 reciprocal and
+// square root (1 ulp) per pixel; the double kernel (gradient checks) keeps
+// exact divisions and square roots.
+template <typename T>
+struct GeneralMath;
+template <>
+struct GeneralMath<float> {
+    static __device__ __forceinline__ double recip64(double x) { return rcp64(x); }
+    static __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+    static __device__ __forceinline__ float root(float x) { return __builtin_amdgcn_sqrtf(x); }
+};
+template <>
+struct GeneralMath<double> {
+    static __device__ __forceinline__ double recip64(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+    static __device__ __forceinline__ double root(double x) { return sqrt(x); }
+};
+
 template <typename T>
 __global__ void __launch_bounds__(BP_BX *BP_BY)
 bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
@@ -66,28 +95,31 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
             const double W0 = km - A.S[0], W1 = k1 - A.S[1], W2 = k2 - A.S[2];
             const double dn = W0 * A.N[0] + W1 * A.N[1] + W2 * A.N[2];
             if (dn == 0.0 || W0 == 0.0) continue;
-            const double lam = A.kappa / dn;
+            using M = GeneralMath<T>;
+            const double idn = M::recip64(dn), iW0 = M::recip64(W0);
+            const double lam = A.kappa * idn;
             const double R0 = A.S[0] + lam * W0 - A.P0[0];
             const double R1 = A.S[1] + lam * W1 - A.P0[1];
             const double R2 = A.S[2] + lam * W2 - A.P0[2];
             cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
             ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
-            r1 = W1 / W0;
-            r2 = W2 / W0;
-            Tc = 1.0 / lam;
-            Bjc = A.U[0] / (lam * W0);
-            Bic = A.V[0] / (lam * W0);
-            g = 1.0 / W0;
+            r1 = W1 * iW0;
+            r2 = W2 * iW0;
+            Tc = dn * M::recip64(A.kappa);  // 1 / lam
+            Bjc = A.U[0] * Tc * iW0;
+            Bic = A.V[0] * Tc * iW0;
+            g = iW0;
         } else {
             const double t0 = ((A.P0[0] - km) * A.N[0] + (A.P0[1] - k1) * A.N[1] +
-                               (A.P0[2] - k2) * A.N[2]) / A.EN;
+                               (A.P0[2] - k2) * A.N[2]) * GeneralMath<T>::recip64(A.EN);
             const double R0 = km + t0 * A.S[0] - A.P0[0];
             const double R1 = k1 + t0 * A.S[1] - A.P0[1];
             const double R2 = k2 + t0 * A.S[2] - A.P0[2];
             cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
             ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
-            r1 = A.S[1] / A.S[0];
-            r2 = A.S[2] / A.S[0];
+            const double iS0 = GeneralMath<T>::recip64(A.S[0]);
+            r1 = A.S[1] * iS0;
+            r2 = A.S[2] * iS0;
             Tc = 1.0;
         }
         const T A1j = (T)(Tc * (A.U[1] - r1 * A.U[0]));
@@ -97,7 +129,7 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
         const T fBj = (T)Bjc, fBi = (T)Bic;
         const double det = (double)A1j * A2i - (double)A1i * A2j;
         if (det == 0.0) continue;
-        const double idet = 1.0 / det;
+        const double idet = GeneralMath<T>::recip64(det);
         const double M00 = A2i * idet, M01 = -A1i * idet, M10 = -A2j * idet, M11 = A1j * idet;
         const double cv0 = fBj * M00 + fBi * M10, cv1 = fBj * M01 + fBi * M11;
         double djmin = 1e30, djmax = -1e30, dimin = 1e30, dimax = -1e30;
@@ -107,7 +139,7 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
             const double s1 = (s & 1) ? 1.0 : -1.0, s2 = (s & 2) ? 1.0 : -1.0;
             const double den = 1.0 - (cv0 * s1 + cv1 * s2);
             if (!(den > 1e-6)) unbounded = true;
-            const double iden = 1.0 / den;
+            const double iden = GeneralMath<T>::recip64(den);
             const double dj = (M00 * s1 + M01 * s2) * iden, di = (M10 * s1 + M11 * s2) * iden;
             djmin = fmin(djmin, dj);
             djmax = fmax(djmax, dj);
@@ -127,7 +159,7 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
         if (j0 > j1 || i0 > i1) continue;
         const T fr1 = (T)r1, fr2 = (T)r2, fg = (T)g;
         const T h0 = (T)(A.h[0] * A.h[0]), h1 = (T)(A.h[1] * A.h[1]), h2 = (T)(A.h[2] * A.h[2]);
-        const T len_par = sqrt(h0 + h1 * fr1 * fr1 + h2 * fr2 * fr2);
+        const T len_par = GeneralMath<T>::root(h0 + h1 * fr1 * fr1 + h2 * fr2 * fr2);
         const T dj0 = (T)((double)j0 - cj), di0 = (T)((double)i0 - ci);
         const T tinv = (T)1 / (T)ss;
         T acc_a = T(0);
@@ -141,7 +173,7 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
                     for (int q = 0; q < ss; ++q) {
                         const T dj = dj0 + (T)(j - j0) + ((T)(q + 0.5) * tinv - (T)0.5);
                         const T den = dj * fBj + (di * fBi + T(1));
-                        const T iden = T(1) / den;
+                        const T iden = GeneralMath<T>::recip(den);
                         const T dq1 = (dj * A1j + di * A1i) * iden;
                         const T dq2 = (dj * A2j + di * A2i) * iden;
                         const T w1 = T(1) - fabs(dq1), w2 = T(1) - fabs(dq2);
@@ -149,7 +181,7 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
                         T len = len_par;
                         if (A.cone) {
                             const T s1 = dq1 * fg + fr1, s2 = dq2 * fg + fr2;
-                            len = sqrt(h0 + h1 * s1 * s1 + h2 * s2 * s2);
+                            len = GeneralMath<T>::root(h0 + h1 * s1 * s1 + h2 * s2 * s2);
                         }
                         wsum += w * len;
                     }
@@ -186,16 +218,6 @@ __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_an
                 (float)(A.h[2] * A.h[2]) * a2 * a2);
     const int64_t o = ((int64_t)i * n_angles + a) * cols + j;
     out[(int64_t)b * out_stride + o] = len * proj[(int64_t)b * proj_stride + o];
-}
-
-// 1/d to FP64 precision: an FP32 reciprocal as the seed (relative error
-// below 3e-7) and one Newton step, which squares the error. Cheaper than an
-// FP64 division; d is a ray extent of the order of the volume, far from the
-// FP32 range limits.
-__device__ __forceinline__ double rcp64(double d)
-{
-    const double r = (double)__builtin_amdgcn_rcpf((float)d);
-    return fma(r, fma(-d, r, 1.0), r);
 }
 
 // Detector column where the ray through in-slice point (n, c) (axes m, c2)
