@@ -44,16 +44,6 @@ __device__ __forceinline__ int sel3(int axis, int x, int y, int z)
     return axis == 0 ? x : (axis == 1 ? y : z);
 }
 
-// 1/d to FP64 precision: an FP32 reciprocal as the seed (relative error
-// below 3e-7) and one Newton step, which squares the error. Cheaper than an
-// FP64 division; d is a ray extent of the order of the volume, far from the
-// FP32 range limits.
-__device__ __forceinline__ double rcp64(double d)
-{
-    const double r = (double)__builtin_amdgcn_rcpf((float)d);
-    return fma(r, fma(-d, r, 1.0), r);
-}
-
 // Arithmetic of bp_general by precision: the float kernel takes the FP64
 // reciprocal above for its FP64 setup and the hardware FP32 reciprocal and
 // square root (1 ulp) per pixel; the double kernel (gradient checks) keeps

@@ -151,7 +151,7 @@ fdk_bp(const FdkAngle *__restrict__ angles, int n_angles, int rows, int cols,
         const double wx = X - A.s[0], wy = Y - A.s[1], wz = Z - A.s[2];
         const double den = wx * A.n[0] + wy * A.n[1] + wz * A.n[2];
         if (!(den > 0.0)) continue;  // voxel not in front of the source
-        const double lam = A.D / den;
+        const double lam = A.D * rcp64(den);
         const float fj = (float)(A.cj + lam * (wx * A.Bj[0] + wy * A.Bj[1] + wz * A.Bj[2]));
         const float fi = (float)(A.ci + lam * (wx * A.Bi[0] + wy * A.Bi[1] + wz * A.Bi[2]));
         if (!(fj > -1.0f && fj < (float)cols && fi > -1.0f && fi < (float)rows)) continue;
@@ -206,7 +206,7 @@ fdk_bp_upright(const FdkAngle *__restrict__ angles, int n_angles, int rows, int 
             const double wx = X - A.s[0], wy = Y - A.s[1], wz = Z0 - A.s[2];
             const double den = wx * A.n[0] + wy * A.n[1];
             if (!(den > 0.0)) return;
-            const double lam = A.D / den;
+            const double lam = A.D * rcp64(den);
             const float fj = (float)(A.cj + lam * (wx * A.Bj[0] + wy * A.Bj[1]));
             if (!(fj > -1.0f && fj < (float)cols)) return;
             const float fi0 = (float)(A.ci + lam * (wx * A.Bi[0] + wy * A.Bi[1] + wz * A.Bi[2]));

@@ -251,6 +251,16 @@ struct Layouts {
     int64_t stride;        // elements between batch entries of the same layout
 };
 
+// 1/d to FP64 precision: an FP32 reciprocal as the seed (relative error
+// below 3e-7) and one Newton step, which squares the error. Cheaper than an
+// FP64 division; d is a ray extent of the order of the volume, far from the
+// FP32 range limits.
+__device__ __forceinline__ double rcp64(double d)
+{
+    const double r = (double)__builtin_amdgcn_rcpf((float)d);
+    return fma(r, fma(-d, r, 1.0), r);
+}
+
 __device__ __forceinline__ int floor_to_int(float q)
 {
     int k;
