@@ -117,20 +117,23 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
         const T A2j = (T)(Tc * (A.U[2] - r2 * A.U[0]));
         const T A2i = (T)(Tc * (A.V[2] - r2 * A.V[0]));
         const T fBj = (T)Bjc, fBi = (T)Bic;
-        const double det = (double)A1j * A2i - (double)A1i * A2j;
-        if (det == 0.0) continue;
-        const double idet = GeneralMath<T>::recip64(det);
-        const double M00 = A2i * idet, M01 = -A1i * idet, M10 = -A2j * idet, M11 = A1j * idet;
-        const double cv0 = fBj * M00 + fBi * M10, cv1 = fBj * M01 + fBi * M11;
-        double djmin = 1e30, djmax = -1e30, dimin = 1e30, dimax = -1e30;
+        // the local homography (offsets of O(1) pixels and voxels) in the
+        // kernel's precision; only the voxel's own detector position (cj, ci)
+        // needs FP64
+        const T det = A1j * A2i - A1i * A2j;
+        if (det == T(0)) continue;
+        const T idet = GeneralMath<T>::recip(det);
+        const T M00 = A2i * idet, M01 = -A1i * idet, M10 = -A2j * idet, M11 = A1j * idet;
+        const T cv0 = fBj * M00 + fBi * M10, cv1 = fBj * M01 + fBi * M11;
+        T djmin = T(1e30), djmax = T(-1e30), dimin = T(1e30), dimax = T(-1e30);
         bool unbounded = false;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const double s1 = (s & 1) ? 1.0 : -1.0, s2 = (s & 2) ? 1.0 : -1.0;
-            const double den = 1.0 - (cv0 * s1 + cv1 * s2);
-            if (!(den > 1e-6)) unbounded = true;
-            const double iden = GeneralMath<T>::recip64(den);
-            const double dj = (M00 * s1 + M01 * s2) * iden, di = (M10 * s1 + M11 * s2) * iden;
+            const T s1 = (s & 1) ? T(1) : T(-1), s2 = (s & 2) ? T(1) : T(-1);
+            const T den = T(1) - (cv0 * s1 + cv1 * s2);
+            if (!(den > T(1e-6))) unbounded = true;
+            const T iden = GeneralMath<T>::recip(den);
+            const T dj = (M00 * s1 + M01 * s2) * iden, di = (M10 * s1 + M11 * s2) * iden;
             djmin = fmin(djmin, dj);
             djmax = fmax(djmax, dj);
             dimin = fmin(dimin, di);
@@ -141,10 +144,10 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
             j0 = 0; j1 = cols - 1; i0 = 0; i1 = rows - 1;
         } else {
             const double mj = sub_margin + 1e-3, mi = sub_margin + 1e-3;
-            j0 = (int)fmax(0.0, ceil(cj + djmin - mj));
-            j1 = (int)fmin((double)(cols - 1), floor(cj + djmax + mj));
-            i0 = (int)fmax(0.0, ceil(ci + dimin - mi));
-            i1 = (int)fmin((double)(rows - 1), floor(ci + dimax + mi));
+            j0 = (int)fmax(0.0, ceil(cj + (double)djmin - mj));
+            j1 = (int)fmin((double)(cols - 1), floor(cj + (double)djmax + mj));
+            i0 = (int)fmax(0.0, ceil(ci + (double)dimin - mi));
+            i1 = (int)fmin((double)(rows - 1), floor(ci + (double)dimax + mi));
         }
         if (j0 > j1 || i0 > i1) continue;
         const T fr1 = (T)r1, fr2 = (T)r2, fg = (T)g;
