@@ -80,42 +80,45 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
     for (int a = 0; a < n_angles; ++a) {
         const AngleParams &A = angles[a];
         const double km = sel3(A.m, x, y, z), k1 = sel3(A.c1, x, y, z), k2 = sel3(A.c2, x, y, z);
-        double cj, ci, r1, r2, Tc, Bjc = 0.0, Bic = 0.0, g = 0.0;
+        // (cj, ci): the voxel centre's detector position, FP64; the ray
+        // slopes r1, r2 and the homography terms are O(1) and use T
+        using M = GeneralMath<T>;
+        double cj, ci;
+        T r1, r2, Tc, Bjc = T(0), Bic = T(0), g = T(0);
         if (A.cone) {
             const double W0 = km - A.S[0], W1 = k1 - A.S[1], W2 = k2 - A.S[2];
             const double dn = W0 * A.N[0] + W1 * A.N[1] + W2 * A.N[2];
             if (dn == 0.0 || W0 == 0.0) continue;
-            using M = GeneralMath<T>;
-            const double idn = M::recip64(dn), iW0 = M::recip64(W0);
-            const double lam = A.kappa * idn;
+            const double lam = A.kappa * M::recip64(dn);
             const double R0 = A.S[0] + lam * W0 - A.P0[0];
             const double R1 = A.S[1] + lam * W1 - A.P0[1];
             const double R2 = A.S[2] + lam * W2 - A.P0[2];
             cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
             ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
-            r1 = W1 * iW0;
-            r2 = W2 * iW0;
-            Tc = dn * M::recip64(A.kappa);  // 1 / lam
-            Bjc = A.U[0] * Tc * iW0;
-            Bic = A.V[0] * Tc * iW0;
+            const T iW0 = M::recip((T)W0);
+            r1 = (T)W1 * iW0;
+            r2 = (T)W2 * iW0;
+            Tc = (T)dn * (T)M::recip64(A.kappa);  // 1 / lam
+            Bjc = (T)A.U[0] * Tc * iW0;
+            Bic = (T)A.V[0] * Tc * iW0;
             g = iW0;
         } else {
             const double t0 = ((A.P0[0] - km) * A.N[0] + (A.P0[1] - k1) * A.N[1] +
-                               (A.P0[2] - k2) * A.N[2]) * GeneralMath<T>::recip64(A.EN);
+                               (A.P0[2] - k2) * A.N[2]) * M::recip64(A.EN);
             const double R0 = km + t0 * A.S[0] - A.P0[0];
             const double R1 = k1 + t0 * A.S[1] - A.P0[1];
             const double R2 = k2 + t0 * A.S[2] - A.P0[2];
             cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
             ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
-            const double iS0 = GeneralMath<T>::recip64(A.S[0]);
-            r1 = A.S[1] * iS0;
-            r2 = A.S[2] * iS0;
-            Tc = 1.0;
+            const double iS0 = M::recip64(A.S[0]);
+            r1 = (T)(A.S[1] * iS0);
+            r2 = (T)(A.S[2] * iS0);
+            Tc = T(1);
         }
-        const T A1j = (T)(Tc * (A.U[1] - r1 * A.U[0]));
-        const T A1i = (T)(Tc * (A.V[1] - r1 * A.V[0]));
-        const T A2j = (T)(Tc * (A.U[2] - r2 * A.U[0]));
-        const T A2i = (T)(Tc * (A.V[2] - r2 * A.V[0]));
+        const T A1j = Tc * ((T)A.U[1] - r1 * (T)A.U[0]);
+        const T A1i = Tc * ((T)A.V[1] - r1 * (T)A.V[0]);
+        const T A2j = Tc * ((T)A.U[2] - r2 * (T)A.U[0]);
+        const T A2i = Tc * ((T)A.V[2] - r2 * (T)A.V[0]);
         const T fBj = (T)Bjc, fBi = (T)Bic;
         // the local homography (offsets of O(1) pixels and voxels) in the
         // kernel's precision; only the voxel's own detector position (cj, ci)
@@ -150,7 +153,7 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
             i1 = (int)fmin((double)(rows - 1), floor(ci + (double)dimax + mi));
         }
         if (j0 > j1 || i0 > i1) continue;
-        const T fr1 = (T)r1, fr2 = (T)r2, fg = (T)g;
+        const T fr1 = r1, fr2 = r2, fg = g;
         const T h0 = (T)(A.h[0] * A.h[0]), h1 = (T)(A.h[1] * A.h[1]), h2 = (T)(A.h[2] * A.h[2]);
         const T len_par = GeneralMath<T>::root(h0 + h1 * fr1 * fr1 + h2 * fr2 * fr2);
         const T dj0 = (T)((double)j0 - cj), di0 = (T)((double)i0 - ci);
