@@ -155,8 +155,9 @@ int tsx_backward(const tsx_plan *plan, float *vol, const float *proj, int additi
                  void *hip_stream);
 
 /* Device scratch bytes that one launch of `op` over `batch` entries needs
- * (padded volume copies for FP, length-weighted projections for BP; 0 when
- * the general kernels run). */
+ * (padded volume copies for the padded FP kernels, length-weighted
+ * projections for BP without detector supersampling; 0 for the general FP
+ * kernel and for supersampled BP). */
 int tsx_workspace_size(const tsx_plan *plan, int op, int64_t batch, int64_t *bytes);
 
 /* Batched variants: `batch` independent volumes / projection sets with the

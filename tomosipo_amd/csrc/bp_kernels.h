@@ -63,7 +63,10 @@ struct GeneralMath<double> {
     static __device__ __forceinline__ double root(double x) { return sqrt(x); }
 };
 
-template <typename T>
+// With PRESCALED (detector supersampling 1 only), proj holds the
+// length-weighted projections l(a, i, j) * y[i, a, j] and the per-pixel ray
+// length is not recomputed.
+template <typename T, bool PRESCALED = false>
 __global__ void __launch_bounds__(BP_BX *BP_BY)
 bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
            int nx, int ny, int nz, T *__restrict__ vol, int64_t vol_stride,
@@ -174,12 +177,16 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
                         const T dq2 = (dj * A2j + di * A2i) * iden;
                         const T w1 = T(1) - fabs(dq1), w2 = T(1) - fabs(dq2);
                         const T w = (w1 > T(0) ? w1 : T(0)) * (w2 > T(0) ? w2 : T(0));
-                        T len = len_par;
-                        if (A.cone) {
-                            const T s1 = dq1 * fg + fr1, s2 = dq2 * fg + fr2;
-                            len = GeneralMath<T>::root(h0 + h1 * s1 * s1 + h2 * s2 * s2);
+                        if constexpr (PRESCALED) {
+                            wsum += w;
+                        } else {
+                            T len = len_par;
+                            if (A.cone) {
+                                const T s1 = dq1 * fg + fr1, s2 = dq2 * fg + fr2;
+                                len = GeneralMath<T>::root(h0 + h1 * s1 * s1 + h2 * s2 * s2);
+                            }
+                            wsum += w * len;
                         }
-                        wsum += w * len;
                     }
                 }
                 acc_a += wsum * yv;

@@ -455,11 +455,19 @@ int64_t bp_workspace_elems(const tsx_plan *p)
     return (int64_t)p->rows * p->n_angles * p->cols;
 }
 
+// The general backprojector takes length-weighted projections (like the
+// separable one) for detector supersampling 1, within the launch limits of
+// the weighting pass.
+bool bp_general_prescaled(const tsx_plan *p, int64_t batch)
+{
+    return !p->bp_separable && p->ss == 1 && (int64_t)p->rows * batch <= 65535;
+}
+
 int64_t workspace_bytes(const tsx_plan *p, int op, int64_t batch)
 {
     if (force_general()) return 0;
     if (op == TSX_OP_FP) return p->fp_padded ? fp_workspace_elems(p) * batch * 4 : 0;
-    return p->bp_separable ? bp_workspace_elems(p) * batch * 4 : 0;
+    return p->bp_separable || bp_general_prescaled(p, batch) ? bp_workspace_elems(p) * batch * 4 : 0;
 }
 
 // Either the caller's workspace (checked for size) or stream-ordered memory.
@@ -625,8 +633,23 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
             return fail(TSX_ERR_UNSUPPORTED, "volume depth times batch exceeds 65535 per launch");
         dim3 grid((unsigned)((p->nx + BP_BX - 1) / BP_BX), (unsigned)((p->ny + BP_BY - 1) / BP_BY),
                   (unsigned)(p->nz * batch));
-        hipLaunchKernelGGL(bp_general<float>, grid, dim3(BP_BX, BP_BY), 0, stream, p->dev, p->n_angles,
-                           p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, proj, ps, ep);
+        if (!force_general() && bp_general_prescaled(p, batch)) {
+            const int64_t sino = (int64_t)p->rows * p->n_angles * p->cols;
+            Scratch owned;
+            void *wptr = nullptr;
+            int rc = get_workspace(p, sino * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
+            if (rc) return rc;
+            float *ly = (float *)wptr;
+            dim3 sgrid((unsigned)((p->cols + 255) / 256), (unsigned)p->n_angles, (unsigned)(p->rows * batch));
+            hipLaunchKernelGGL(scale_by_length, sgrid, dim3(256), 0, stream, p->dev, p->n_angles, p->rows,
+                               p->cols, proj, ps, ly, sino);
+            hipLaunchKernelGGL((bp_general<float, true>), grid, dim3(BP_BX, BP_BY), 0, stream, p->dev,
+                               p->n_angles, p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, ly, sino,
+                               ep);
+        } else {
+            hipLaunchKernelGGL(bp_general<float>, grid, dim3(BP_BX, BP_BY), 0, stream, p->dev, p->n_angles,
+                               p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, proj, ps, ep);
+        }
         TSX_HIP_CHECK(hipGetLastError());
         return TSX_OK;
     }
