@@ -248,26 +248,27 @@ struct BpView {
     Epilogue ep;
 };
 
-// LDS layout of bp_separable: a column of u holds kz = -4 .. BZ - 1 at
-// offsets 0 .. BZ + 3 (kz = 0 at offset UOFF); columns are BZ + 4 floats
-// apart and slices BR * (BZ + 4) + 2 floats apart (= 2 mod 64). Only the
-// cells kz = 0 .. BZ - 1 are read. The splat clamps the lower cell of a row
-// to [-3, BZ], so the weights of rows outside the brick land in cells
-// -3 .. -1 of their own column or in cells BZ, BZ + 1, which are offsets 0
-// and 1 of the next column (kz = -4, -3 there) or the 2-float gap after a
-// slice's last column: cells that are never read. The strides were chosen
-// with a simulation of the kernel's LDS addresses on a cone-beam geometry
-// (dword banks of 64): this slice stride spreads the z-cells that the lanes
+// LDS layout of bp_separable: columns are UCOL = BZ + 2 floats apart and
+// slices SLICE floats apart (SLICE = 36 mod 64); the cell kz of the column in
+// slot pos of slice s is at s * SLICE + pos * UCOL + UOFF + kz. Only the cells
+// kz = 0 .. BZ - 1 are read. The splat clamps the lower cell of a row to
+// [-2, BZ], so the weights of rows outside the brick land in cells -2, -1 of
+// their column or in cells BZ, BZ + 1; a column's cells BZ - 2, BZ - 1 share
+// the next column's (unused) cells -4, -3, and its cells BZ, BZ + 1 the next
+// column's cells -2, -1, so no real cell is written by another column. The
+// strides were chosen with a simulation of the kernel's LDS addresses on a
+// cone-beam geometry (64 dword banks): they spread the z-cells that the lanes
 // of a pass-1 wave (16 columns of 4 slices) and of a pass-2 wave (16 slices)
-// touch at the same time over the banks, while every column stays 8-byte
-// aligned so that pass 2 reads its cells as float2.
+// touch at the same time over the banks (7 % fewer bank cycles than columns
+// BZ + 4 apart), and every column stays 8-byte aligned so that pass 2 reads
+// its cells as float2.
 constexpr int UOFF = 4;  // kz = 0 sits at offset 4 of a column
 
 template <int BZ>
 struct BpShared {
-    static constexpr int UCOL = BZ + 4;
-    static constexpr int SLICE = BR * UCOL + 2;
-    static_assert(UCOL % 2 == 0 && SLICE % 64 == 2, "LDS strides");
+    static constexpr int UCOL = BZ + 2;
+    static constexpr int SLICE = (BR * UCOL + 27) / 64 * 64 + 36;  // first stride >= BR * UCOL + 2, = 36 mod 64
+    static_assert(UCOL % 2 == 0 && SLICE % 64 == 36 && SLICE >= BR * UCOL + 2, "LDS strides");
     float u[BR * SLICE];
     float q2[BR][20];     // Q_c2 - c0 per stored column, increasing; 1e30 if unused
     int jlo[2][BR];       // first detector column of each slice (double-buffered by angle)
@@ -412,8 +413,8 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                     // are read too (inside the buffer, or zero by the range
                     // check); their t lies beyond the brick (t >= BZ when
                     // Bz > 0, t <= -1 when Bz < 0), and the clamp of kf to
-                    // [-3, BZ] sends their weights to cells that pass 2 never
-                    // reads. For t inside [-3, BZ + 1), kf = floor(t) and
+                    // [-2, BZ] sends their weights to cells that pass 2 never
+                    // reads. For t inside [-2, BZ + 1), kf = floor(t) and
                     // w = fract(t) are the exact bilinear split.
                     auto load4 = [&](uint32_t o, float (&yv)[4]) {
 #pragma unroll
@@ -435,7 +436,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const float ti = fmaf((float)r, fB, tb);
-                            kf[r] = min(max(floor_to_int(ti), -3), BZ);
+                            kf[r] = min(max(floor_to_int(ti), -2), BZ);
                             w[r] = __builtin_amdgcn_fractf(ti);
                         }
                         if constexpr (decltype(paired)::value) {
