@@ -359,9 +359,12 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
             continue;
         }
         for (int c = 0; c < chunks; ++c) {
-            // pass 1: thread (s = ty, jj = tx) splats detector column j onto kz
+            // pass 1: thread (s = tx, jj = ty) splats detector column j onto kz
+            // (a wave holds slots 4w .. 4w + 3 of all 16 slices: the empty slots,
+            // the last ones of a slice, gather in the last waves, and a wave's
+            // neighbouring slices read nearly the same detector columns)
             {
-                const int s = ty, jj = tx;
+                const int s = tx, jj = ty;
                 const int m = min(BR, sh.cnt[buf][s] - c * BR);  // valid columns of this chunk
                 const bool valid = jj < m;
                 const int pos = (!valid || sh.inc[buf][s]) ? jj : m - 1 - jj;
