@@ -371,10 +371,12 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                 const int j = sh.jlo[buf][s] + c * BR + jj;
                 float *col = &sh.u[s * S::SLICE + pos * S::UCOL];
                 if (valid) {
-                    // slots without a column are never read (q2 = 1e30)
-                    float2 *c2 = reinterpret_cast<float2 *>(col + UOFF);
+                    // (slots without a column are never read: q2 = 1e30)
+                    auto zero_column = [&]() {
+                        float2 *c2 = reinterpret_cast<float2 *>(col + UOFF);
 #pragma unroll
-                    for (int k = 0; k < BZ / 2; ++k) c2[k] = make_float2(0.0f, 0.0f);
+                        for (int k = 0; k < BZ / 2; ++k) c2[k] = make_float2(0.0f, 0.0f);
+                    };
                     // ray of detector row 0 in column j: slice n is met at the
                     // in-slice point (Az, Q2); row i adds i * Bz along z
                     const double n = (double)(m0 + s), jd = (double)j;
@@ -406,8 +408,12 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                     float lo = fminf(ra, rb), hi = fmaxf(ra, rb);
                     lo -= 1e-3f + 1e-5f * fabsf(lo);
                     hi += 1e-3f + 1e-5f * fabsf(hi);
-                    const int i0 = max(0, (int)ceilf(fmaxf(lo, -1e9f)));
-                    const int i1 = min(rows - 1, (int)floorf(fminf(hi, 1e9f)));
+                    const int r0 = (int)ceilf(fmaxf(lo, -1e9f)), r1 = (int)floorf(fminf(hi, 1e9f));
+                    const int i0 = max(0, r0);
+                    const int i1 = min(rows - 1, r1);
+                    // no row of the range is cut off by the detector's edge: then
+                    // rows less than two cells apart reach every cell of the brick
+                    const bool covered = r0 >= 0 && r1 <= rows - 1 && r0 <= r1;
                     const float t0 = (float)(Arel + (double)i0 * Bz);
                     uint32_t off = (uint32_t)i0 * row_bytes + ((uint32_t)a * (uint32_t)cols + (uint32_t)j) * 4u;
                     float *u = col + UOFF;
@@ -478,11 +484,74 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                             tb = fmaf(8.0f, fB, tb);
                         }
                     };
-                    // the paired order when every column of the wave allows it
-                    if (__builtin_amdgcn_ballot_w64(fabsf(fB) > 1.01f) == __builtin_amdgcn_read_exec())
-                        splat_rows(Flag<true>());
-                    else
-                        splat_rows(Flag<false>());
+                    // Rows 1.01 .. 1.99 cells apart (|Bz|), taken in increasing t:
+                    // a row's lower cell kf is at least the previous row's upper
+                    // cell pk and at most one above it, and a cell takes at most
+                    // two rows. The pending upper weight `carry` stays in a
+                    // register, and each row writes cells (pk, pk + 1) once,
+                    // final except for the pending one, with no LDS read:
+                    //   kf == pk:     (carry + lo, hi)   hi is rewritten later
+                    //   kf == pk + 1: (carry, lo)
+                    // For a covered column every cell of the brick is written, so
+                    // it needs no zeroing. Rows clamped to kf = BZ and the write
+                    // base clamped to BZ touch only cells BZ, BZ + 1.
+                    auto splat_carry = [&]() {
+                        const bool up = fB > 0.0f;  // the sign of the detector's v along z
+                        const float dB = fabsf(fB);
+                        const uint32_t step = up ? row_bytes : 0u - row_bytes;
+                        float tb = up ? t0 : fmaf((float)(i1 - i0), fB, t0);
+                        uint32_t o = up ? off : off + (uint32_t)(i1 - i0) * row_bytes;
+                        int pk = min(max(floor_to_int(tb), -2), BZ);
+                        float carry = 0.0f;
+                        auto load4s = [&](uint32_t oo, float (&yv)[4]) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                                      rs, oo + (uint32_t)r * step, 0, 0));
+                        };
+                        auto carry4 = [&](float tg, const float (&yv)[4]) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const float ti = fmaf((float)r, dB, tg);
+                                const int kf = min(max(floor_to_int(ti), -2), BZ);
+                                const float hiw = __builtin_amdgcn_fractf(ti) * yv[r];
+                                const float low = yv[r] - hiw;
+                                const bool eq = kf == pk;
+                                const int b0 = min(pk, BZ);
+                                u[b0] = carry + (eq ? low : 0.0f);
+                                u[b0 + 1] = eq ? hiw : low;
+                                pk = kf + 1;
+                                carry = hiw;
+                            }
+                        };
+                        float ya[4], yb[4];
+                        const int nr = i1 - i0 + 1;
+                        load4s(o, ya);
+                        for (int i = 0; i < nr; i += 8) {
+                            load4s(o + 4u * step, yb);
+                            __builtin_amdgcn_sched_barrier(0);
+                            carry4(tb, ya);
+                            if (i + 4 >= nr) break;
+                            load4s(o + 8u * step, ya);
+                            __builtin_amdgcn_sched_barrier(0);
+                            carry4(fmaf(4.0f, dB, tb), yb);
+                            o += 8u * step;
+                            tb = fmaf(8.0f, dB, tb);
+                        }
+                        u[min(pk, BZ)] = carry;
+                    };
+                    const uint64_t lanes = __builtin_amdgcn_read_exec();
+                    if (__builtin_amdgcn_ballot_w64(fabsf(fB) > 1.01f && fabsf(fB) < 1.99f) == lanes) {
+                        if (!covered) zero_column();
+                        splat_carry();
+                    } else {
+                        zero_column();
+                        // the paired order when every column of the wave allows it
+                        if (__builtin_amdgcn_ballot_w64(fabsf(fB) > 1.01f) == lanes)
+                            splat_rows(Flag<true>());
+                        else
+                            splat_rows(Flag<false>());
+                    }
                 } else {
                     sh.q2[s][pos] = 1e30f;
                 }
