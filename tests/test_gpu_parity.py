@@ -90,14 +90,31 @@ def test_truncated_detector(kind):
 
 @pytest.mark.parametrize("rows", [64, 39, 38, 24, 18])
 def test_backprojector_row_pitch(rows):
-    """The separable backprojector splats detector rows onto the z-grid in two
-    orders: row by row, or rows two apart together when a row step exceeds
-    1.01 voxels. Parallel beam, 32^3 unit volume, detector 1.2 high: the row
-    step is 38.4 / rows voxels (0.6, 0.985, 1.0105, 1.6, 2.13)."""
+    """The separable backprojector splats detector rows onto the z-grid in
+    three ways: row by row (row step up to 1.01 voxels), with the pending cell
+    carried in a register (1.01 to 1.99), or rows two apart together (from
+    1.99). Parallel beam, 32^3 unit volume, detector 1.2 high: the row step is
+    38.4 / rows voxels (0.6, 0.985, 1.0105, 1.6, 2.13)."""
     vg = ts.volume(shape=32, size=1)
     pg = ts.parallel(angles=24, shape=(rows, 48), size=(1.2, 1.5)).to_vec()
     assert ts.operator(vg, pg).projector.plan(0).info()["bp_kernel"] == "separable"
     x = ts.phantom.ellipsoids(32, count=6, seed=rows)
+    _check_fp_bp(vg, pg, x)
+
+
+@pytest.mark.parametrize("kind", ["parallel", "cone"])
+def test_backprojector_short_detector(kind):
+    """Row step 1.5 voxels (the splat that carries the pending cell in a
+    register), with a detector shorter than the volume along z: the rows of
+    the top and bottom bricks are cut off by the detector's edges, and those
+    columns must still read zero where no row reaches."""
+    vg = ts.volume(shape=32, size=1)
+    if kind == "parallel":
+        pg = ts.parallel(angles=20, shape=(16, 48), size=(0.75, 1.5)).to_vec()
+    else:
+        pg = ts.cone(angles=20, shape=(14, 48), size=(1.0, 2.4), src_orig_dist=4, src_det_dist=6).to_vec()
+    assert ts.operator(vg, pg).projector.plan(0).info()["bp_kernel"] == "separable"
+    x = ts.phantom.ellipsoids(32, count=6, seed=11)
     _check_fp_bp(vg, pg, x)
 
 
