@@ -33,10 +33,6 @@ namespace tsx {
 
 constexpr int BP_BX = 64, BP_BY = 4;
 
-template <bool B>
-struct Flag {
-    static constexpr bool value = B;
-};
 constexpr int BR = 16;  // brick edge (x, y) of bp_separable
 
 __device__ __forceinline__ int sel3(int axis, int x, int y, int z)
@@ -417,74 +413,19 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                     const float t0 = (float)(Arel + (double)i0 * Bz);
                     uint32_t off = (uint32_t)i0 * row_bytes + ((uint32_t)a * (uint32_t)cols + (uint32_t)j) * 4u;
                     float *u = col + UOFF;
-                    // rows in groups of four, the loads of the next group in
-                    // flight while the current one is splatted. Rows past i1
-                    // are read too (inside the buffer, or zero by the range
-                    // check); their t lies beyond the brick (t >= BZ when
-                    // Bz > 0, t <= -1 when Bz < 0), and the clamp of kf to
-                    // [-2, BZ] sends their weights to cells that pass 2 never
-                    // reads. For t inside [-2, BZ + 1), kf = floor(t) and
+                    // The rows are taken in increasing t (in the order of i if
+                    // Bz > 0, reversed otherwise; the sign of Bz is that of the
+                    // detector's v along z, the same for every column), in
+                    // groups of four, the loads of the next group in flight
+                    // while the current one is splatted, and no LDS cell is
+                    // read. Rows past the range are read too (inside the
+                    // buffer, or zero by the range check): their t lies beyond
+                    // the brick (t >= BZ), and the clamp of kf to [-2, BZ]
+                    // sends their weights to cells BZ, BZ + 1, which pass 2
+                    // never reads. For t inside [-2, BZ + 1), kf = floor(t) and
                     // w = fract(t) are the exact bilinear split.
-                    auto load4 = [&](uint32_t o, float (&yv)[4]) {
-#pragma unroll
-                        for (int r = 0; r < 4; ++r)
-                            yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                                  rs, o + (uint32_t)r * row_bytes, 0, 0));
-                    };
-                    // One group of four rows. In order, each row's two cells
-                    // are read, updated and written before the next row's
-                    // (neighbouring rows may share a cell), so every row waits
-                    // for an LDS round trip. With PAIRED, rows 0 and 2, then
-                    // 1 and 3, are read together: rows two apart touch
-                    // disjoint cells when |Bz| > 1.01 (their t differ by more
-                    // than 2; rows clamped to the same edge cell share only
-                    // cells that are never read).
-                    auto splat4 = [&](float tb, const float (&yv)[4], auto paired) {
-                        int kf[4];
-                        float w[4];
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const float ti = fmaf((float)r, fB, tb);
-                            kf[r] = min(max(floor_to_int(ti), -2), BZ);
-                            w[r] = __builtin_amdgcn_fractf(ti);
-                        }
-                        if constexpr (decltype(paired)::value) {
-#pragma unroll
-                            for (int h = 0; h < 2; ++h) {
-                                const int p = h, q = h + 2;
-                                const float p0 = u[kf[p]], p1 = u[kf[p] + 1];
-                                const float q0 = u[kf[q]], q1 = u[kf[q] + 1];
-                                u[kf[p]] = fmaf(1.0f - w[p], yv[p], p0);
-                                u[kf[p] + 1] = fmaf(w[p], yv[p], p1);
-                                u[kf[q]] = fmaf(1.0f - w[q], yv[q], q0);
-                                u[kf[q] + 1] = fmaf(w[q], yv[q], q1);
-                            }
-                        } else {
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                u[kf[r]] = fmaf(1.0f - w[r], yv[r], u[kf[r]]);
-                                u[kf[r] + 1] = fmaf(w[r], yv[r], u[kf[r] + 1]);
-                            }
-                        }
-                    };
-                    auto splat_rows = [&](auto paired) {
-                        float ya[4], yb[4];
-                        float tb = t0;
-                        uint32_t o = off;
-                        load4(o, ya);
-                        for (int i = i0; i <= i1; i += 8) {
-                            load4(o + 4u * row_bytes, yb);
-                            __builtin_amdgcn_sched_barrier(0);  // keep these loads ahead of the splat
-                            splat4(tb, ya, paired);
-                            if (i + 4 > i1) break;
-                            load4(o + 8u * row_bytes, ya);
-                            __builtin_amdgcn_sched_barrier(0);
-                            splat4(fmaf(4.0f, fB, tb), yb, paired);
-                            o += 8u * row_bytes;
-                            tb = fmaf(8.0f, fB, tb);
-                        }
-                    };
-                    // Rows 1.01 .. 1.99 cells apart (|Bz|), taken in increasing t:
+                    //
+                    // Rows 1.01 .. 1.99 cells apart (|Bz|):
                     // a row's lower cell kf is at least the previous row's upper
                     // cell pk and at most one above it, and a cell takes at most
                     // two rows. The pending upper weight `carry` stays in a
@@ -496,7 +437,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                     // it needs no zeroing. Rows clamped to kf = BZ and the write
                     // base clamped to BZ touch only cells BZ, BZ + 1.
                     auto splat_carry = [&]() {
-                        const bool up = fB > 0.0f;  // the sign of the detector's v along z
+                        const bool up = fB > 0.0f;
                         const float dB = fabsf(fB);
                         const uint32_t step = up ? row_bytes : 0u - row_bytes;
                         float tb = up ? t0 : fmaf((float)(i1 - i0), fB, t0);
@@ -540,17 +481,69 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                         }
                         u[min(pk, BZ)] = carry;
                     };
-                    const uint64_t lanes = __builtin_amdgcn_read_exec();
-                    if (__builtin_amdgcn_ballot_w64(fabsf(fB) > 1.01f && fabsf(fB) < 1.99f) == lanes) {
+                    // Any row step: the two cells (p, p + 1) of the last row stay
+                    // pending in registers (c0, c1); with d = kf - p a row adds
+                    // to both (d = 0), finishes p (d = 1) or both (d >= 2), and
+                    // writes (p, p + 1) once with their current values. Rows
+                    // more than 1.99 cells apart leave cells between them that
+                    // no row reaches: those columns are zeroed first, as are
+                    // columns cut off by the detector's edge.
+                    auto splat_carry2 = [&]() {
+                        const bool up = fB > 0.0f;
+                        const float dB = fabsf(fB);
+                        const uint32_t step = up ? row_bytes : 0u - row_bytes;
+                        float tb = up ? t0 : fmaf((float)(i1 - i0), fB, t0);
+                        uint32_t o = up ? off : off + (uint32_t)(i1 - i0) * row_bytes;
+                        int p = min(max(floor_to_int(tb), -2), BZ);
+                        float c0 = 0.0f, c1 = 0.0f;
+                        auto load4s = [&](uint32_t oo, float (&yv)[4]) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                                      rs, oo + (uint32_t)r * step, 0, 0));
+                        };
+                        auto carry4 = [&](float tg, const float (&yv)[4]) {
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const float ti = fmaf((float)r, dB, tg);
+                                const int kf = min(max(floor_to_int(ti), -2), BZ);
+                                const float hiw = __builtin_amdgcn_fractf(ti) * yv[r];
+                                const float low = yv[r] - hiw;
+                                const int d = kf - p;
+                                const bool d0 = d == 0, d1 = d == 1;
+                                const float v0 = d0 ? c0 + low : c0;
+                                const float v1 = c1 + (d0 ? hiw : (d1 ? low : 0.0f));
+                                u[p] = v0;
+                                u[p + 1] = v1;
+                                c0 = d0 ? v0 : (d1 ? v1 : low);
+                                c1 = d0 ? v1 : hiw;
+                                p = kf;
+                            }
+                        };
+                        float ya[4], yb[4];
+                        const int nr = i1 - i0 + 1;
+                        load4s(o, ya);
+                        for (int i = 0; i < nr; i += 8) {
+                            load4s(o + 4u * step, yb);
+                            __builtin_amdgcn_sched_barrier(0);
+                            carry4(tb, ya);
+                            if (i + 4 >= nr) break;
+                            load4s(o + 8u * step, ya);
+                            __builtin_amdgcn_sched_barrier(0);
+                            carry4(fmaf(4.0f, dB, tb), yb);
+                            o += 8u * step;
+                            tb = fmaf(8.0f, dB, tb);
+                        }
+                        u[p] = c0;
+                        u[p + 1] = c1;
+                    };
+                    if (__builtin_amdgcn_ballot_w64(fabsf(fB) > 1.01f && fabsf(fB) < 1.99f) ==
+                        __builtin_amdgcn_read_exec()) {
                         if (!covered) zero_column();
                         splat_carry();
                     } else {
-                        zero_column();
-                        // the paired order when every column of the wave allows it
-                        if (__builtin_amdgcn_ballot_w64(fabsf(fB) > 1.01f) == lanes)
-                            splat_rows(Flag<true>());
-                        else
-                            splat_rows(Flag<false>());
+                        if (!covered || fabsf(fB) >= 1.99f) zero_column();
+                        splat_carry2();
                     }
                 } else {
                     sh.q2[s][pos] = 1e30f;
