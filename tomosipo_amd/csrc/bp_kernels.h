@@ -18,12 +18,14 @@
 //     u[n, j, kz]   = sum_i L(A(j, n) + i B(j, n) - kz) * ly[i, j],
 //   with ly = l * y precomputed. A block owns a brick of 16 x 16 columns by
 //   BZ voxels along z; per angle it builds u for its 16 slices in LDS (pass
-//   1: thread (slice, column) splats its detector column onto the z-grid),
-//   stores the columns of a slice in increasing order of Q_c2, and then every
-//   thread folds exactly the columns with |Q_c2 - kc2| < 1 into the BZ voxels
-//   of its z-column (pass 2; the range is found by counting). The slice
-//   ranges of the next angle are computed during the current one. No
-//   atomics: the result is deterministic.
+//   1: thread (slice, column) splats its detector column onto the z-grid,
+//   rows in increasing z with the pending cells kept in registers, so that
+//   every cell is written without being read back), stores the columns of a
+//   slice in increasing order of Q_c2, and then every thread folds exactly
+//   the columns with |Q_c2 - kc2| < 1 into the BZ voxels of its z-column
+//   (pass 2; the first one found by binary search). The slice ranges of the
+//   next angle are computed during the current one. No atomics: the result
+//   is deterministic.
 #pragma once
 
 #include "tsx_plan.h"
