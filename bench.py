@@ -87,14 +87,14 @@ def cpu_baseline(ts, target_s=12.0):
     t_bp = time.perf_counter() - t0
     na = sub.num_angles
     gups = 2 * N_VOX ** 3 * na / (t_fp + t_bp) / 1e9
-    # C1 (parallel3d 64^3, 96 angles, 64x96) in full: one FP and one BP
-    vg1, pg1 = c1_geometry(ts)
+    # the small parallel scan (64^3, 96 angles, 64x96) in full: one FP and one BP
+    vg1, pg1 = small_parallel_geometry(ts)
     g1 = oracle.oracle_geometry(vg1, pg1)
     x1 = np.zeros(g1.vol_shape)
     x1[16:40, 16:40, 16:40] = 1.0
     t0 = time.perf_counter()
     oracle.bp(g1, oracle.fp(g1, x1))
-    t_c1 = time.perf_counter() - t0
+    t_small = time.perf_counter() - t0
     return {
         "value": round(gups, 4),
         "unit": "GUPS",
@@ -102,18 +102,19 @@ def cpu_baseline(ts, target_s=12.0):
         "kind": "port",
         "sample": (f"instance 0 of the workload, every {step}th angle ({na} of {N_ANGLES}): "
                    f"one FP ({t_fp:.2f} s) and one BP ({t_bp:.2f} s) in FP64"),
-        "c1_full_ms": round(t_c1 * 1e3, 2),
+        "small_parallel_full_ms": round(t_small * 1e3, 2),
     }
 
 
-def c1_geometry(ts):
-    """C1: parallel3d_vec, 64^3 unit cube, 96 angles over pi, 64 x 96 detector of size (1, 1.5)."""
+def small_parallel_geometry(ts):
+    """parallel3d_vec, 64^3 unit cube, 96 angles over pi, 64 x 96 detector of size (1, 1.5)."""
     return ts.volume(shape=64, size=1), ts.parallel(angles=96, shape=(64, 96), size=(1, 1.5)).to_vec()
 
 
-def c1_benchmark(torch, ts, reps=20):
-    """GPU time of one FP and one BP at C1 (single operator calls, launch-bound)."""
-    vg, pg = c1_geometry(ts)
+def small_parallel_benchmark(torch, ts, reps=20):
+    """GPU time of one FP and one BP of the small parallel scan (single operator
+    calls, launch-bound)."""
+    vg, pg = small_parallel_geometry(ts)
     A = ts.operator(vg, pg)
     x = torch.zeros(A.domain_shape, device="cuda")
     x[16:40, 16:40, 16:40] = 1.0
@@ -127,7 +128,7 @@ def c1_benchmark(torch, ts, reps=20):
         A(x, out=y)
         A.T(y, out=b)
     torch.cuda.synchronize()
-    return {"workload": "C1 parallel3d_vec 64^3, 96 angles, 64x96: one FP + one BP",
+    return {"workload": "parallel3d_vec 64^3, 96 angles, 64x96: one FP + one BP",
             "ms": round((time.perf_counter() - t0) * 1e3 / reps, 3)}
 
 
@@ -186,7 +187,7 @@ def fdk_benchmark(torch, ts, reps=3):
             "ms": round(dt * 1e3, 2), "gups": round(N_VOX ** 3 * N_ANGLES / dt / 1e9, 1)}
 
 
-def c4_benchmark(torch, ts, reps=10):
+def autograd_batch_benchmark(torch, ts, reps=10):
     """Autograd step on a batch of 8 volumes 128^3, parallel3d 180 angles, 128x192 detector:
     loss = |f(x) - y|^2 with f = to_autograd(A, num_extra_dims=1), forward + backward."""
     from tomosipo_amd.torch_support import to_autograd
@@ -371,8 +372,8 @@ def main():
             roofline["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
     if args.serving_requests > 0:
         rec["serving"] = serving_benchmark(args.serving_requests)
-    rec["c1"] = c1_benchmark(torch, ts)
-    rec["c4"] = c4_benchmark(torch, ts)
+    rec["small_parallel"] = small_parallel_benchmark(torch, ts)
+    rec["autograd_batch"] = autograd_batch_benchmark(torch, ts)
     if not args.no_sirt:
         rec["sirt"] = sirt_benchmark(torch, ts)
     if not args.no_fdk:

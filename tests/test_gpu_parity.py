@@ -50,7 +50,7 @@ def _check_fp_bp(vg, pg, x, ss=1, tol=TOL):
     assert adj <= tol
 
 
-def test_c1_parallel3d_vec_64():
+def test_parallel3d_vec_64():
     vg = ts.volume(shape=64, size=1)
     pg = ts.parallel(angles=96, shape=(64, 96), size=(1, 1.5)).to_vec()
     x = ts.phantom.box_noise((64, 64, 64), 16, 40, 0.01, seed=1)
@@ -287,7 +287,7 @@ def test_fp64_kernels_match_oracle_and_gradcheck():
     assert to_autograd(A)(xd.detach()).dtype == torch.float32
 
 
-def test_c4_gradient_matches_oracle():
+def test_batched_gradient_matches_oracle():
     """Batch of 8 volumes through to_autograd(num_extra_dims=1): the gradient of
     ||Ax - y||^2 equals the oracle's 2 A^T (Ax - y) within 1e-5 (reduced size)."""
     from tomosipo_amd.torch_support import to_autograd

@@ -14,7 +14,7 @@ from oracle import numpy_twin
 import tomosipo_amd as ts
 
 
-def _c1():
+def _small_parallel():
     vg = ts.volume(shape=64, size=1)
     pg = ts.parallel(angles=96, shape=(64, 96), size=(1, 1.5))
     return vg, pg
@@ -22,7 +22,7 @@ def _c1():
 
 def test_closed_form_theta_zero_and_ninety():
     """Parallel beam, pixels aligned with voxels: y[i, a, j] = h * sum along the ray axis."""
-    vg, pg = _c1()
+    vg, pg = _small_parallel()
     g = oracle.oracle_geometry(vg, pg)
     x = np.random.default_rng(0).random(g.vol_shape)
     y = oracle.fp(g, x)
@@ -85,7 +85,7 @@ def test_numpy_twin_matches_c_oracle(rng):
 
 
 def test_adjoint_exact_fp64(rng):
-    for vg, pg in [_c1(), (ts.volume(shape=(20, 24, 28), size=(1, 1.1, 1.2)),
+    for vg, pg in [_small_parallel(), (ts.volume(shape=(20, 24, 28), size=(1, 1.1, 1.2)),
                           ts.cone(angles=30, shape=(24, 36), size=(1.6, 2.4), cone_angle=0.5))]:
         g = oracle.oracle_geometry(vg, pg)
         x = rng.random(g.vol_shape)
@@ -96,7 +96,7 @@ def test_adjoint_exact_fp64(rng):
 
 
 def test_additive_mode(rng):
-    vg, pg = _c1()
+    vg, pg = _small_parallel()
     g = oracle.oracle_geometry(vg, pg[::8])
     x = rng.random(g.vol_shape)
     y0 = rng.random(g.sino_shape)
@@ -123,7 +123,7 @@ def test_transform_equivariance(seed):
 
 def test_dominant_axis_rule():
     """Parallel beam over [0, pi): y for |cos| >= |sin| (ties x first), x otherwise."""
-    vg, pg = _c1()
+    vg, pg = _small_parallel()
     axes = oracle.axes(oracle.oracle_geometry(vg, pg))
     t = pg.angles
     ray = np.stack([np.sin(t), -np.cos(t)], axis=1)  # (x, y) components
@@ -133,7 +133,7 @@ def test_dominant_axis_rule():
 
 def test_oracle_catches_planted_sign_error():
     """A flipped detector u (a plausible sign mistake) breaks the theta=0 closed form."""
-    vg, pg = _c1()
+    vg, pg = _small_parallel()
     bad = ts.parallel_vec(shape=pg.det_shape, ray_dir=pg.ray_dir, det_pos=pg.det_pos,
                           det_v=pg.det_v, det_u=-pg.det_u)
     x = np.random.default_rng(1).random((64, 64, 64))
