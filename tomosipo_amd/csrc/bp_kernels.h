@@ -594,10 +594,10 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
 }
 
 template <int BZ>
-__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ == 32 ? 4 : 1))) bp_separable(BpView v, int n_blocks)
+__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ == 32 ? 4 : 1))) bp_separable(BpView v, int n_blocks, int shift)
 {
     __shared__ __align__(16) BpShared<BZ> sh;
-    const int logical = xcd_logical(n_blocks);
+    const int logical = xcd_logical(shift);
     if (logical >= n_blocks) return;
     bp_separable_work<BZ>(v, logical, sh);
 }
@@ -606,6 +606,7 @@ struct BpMulti {
     BpView v[MAX_FUSED];
     int begin[MAX_FUSED + 1];
     int n;
+    int shift;  // chunk size of the XCD order (xcd_logical)
 };
 
 template <int BZ>
@@ -613,7 +614,7 @@ __global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ 
 {
     __shared__ __align__(16) BpShared<BZ> sh;
     const int total = m.begin[m.n];
-    const int logical = xcd_logical(total);
+    const int logical = xcd_logical(m.shift);
     if (logical >= total) return;
     int k = 0;
     while (k + 1 < m.n && logical >= m.begin[k + 1]) ++k;

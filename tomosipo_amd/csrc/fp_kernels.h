@@ -527,17 +527,23 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
 }
 
 // XCD-aware order: consecutive hardware block ids go round-robin to the 8
-// XCDs; each XCD gets a contiguous range of logical ids.
-__device__ __forceinline__ int xcd_logical(int n_blocks)
+// XCDs. Logical ids are dealt to the XCDs in chunks of 2^shift consecutive
+// ids, chunk k to XCD k % 8, and each XCD runs its chunks in order. The
+// blocks resident on one XCD at a time then hold a run of consecutive logical
+// ids (neighbouring angles of one detector tile, neighbouring bricks of one
+// row), and every XCD holds a share of every part of the launch. With one
+// contiguous range per XCD instead, the XCDs that hold the detector's edge
+// rows (or the volume's edge slabs) run out of work early.
+__device__ __forceinline__ int xcd_logical(int shift)
 {
-    const int per = (n_blocks + 7) / 8;
-    return (int)(blockIdx.x % 8u) * per + (int)(blockIdx.x / 8u);
+    const int x = (int)(blockIdx.x % 8u), j = (int)(blockIdx.x / 8u);
+    return ((((j >> shift) << 3) + x) << shift) + (j & ((1 << shift) - 1));
 }
 
 template <int SS, int PIPE>
-__global__ void __launch_bounds__(FP_BX *FP_BY) fp_padded(FpView v, int n_blocks)
+__global__ void __launch_bounds__(FP_BX *FP_BY) fp_padded(FpView v, int n_blocks, int shift)
 {
-    const int logical = xcd_logical(n_blocks);
+    const int logical = xcd_logical(shift);
     if (logical >= n_blocks) return;
     fp_padded_work<SS, PIPE>(v, logical);
 }
@@ -550,13 +556,14 @@ struct FpMulti {
     FpView v[MAX_FUSED];
     int begin[MAX_FUSED + 1];
     int n;
+    int shift;  // chunk size of the XCD order (xcd_logical)
 };
 
 template <int PIPE>
 __global__ void __launch_bounds__(FP_BX *FP_BY) fp_padded_multi(FpMulti m)
 {
     const int total = m.begin[m.n];
-    const int logical = xcd_logical(total);
+    const int logical = xcd_logical(m.shift);
     if (logical >= total) return;
     int k = 0;
     while (k + 1 < m.n && logical >= m.begin[k + 1]) ++k;

@@ -470,6 +470,23 @@ int64_t workspace_bytes(const tsx_plan *p, int op, int64_t batch)
     return p->bp_separable || bp_general_prescaled(p, batch) ? bp_workspace_elems(p) * batch * 4 : 0;
 }
 
+// Launches of one block per logical id take the chunked XCD order of
+// xcd_logical: chunks of 2^shift consecutive ids, the largest power of two up
+// to 1024 that still gives every XCD about 64 chunks, and a grid with the same
+// number of chunk slots on every XCD (blocks past the last id return at once).
+// Logical ids stay below MAX_LOGICAL so that the grid fits in an int.
+constexpr int64_t MAX_LOGICAL = 0x7fffffff - 9 * 1024;
+
+unsigned xcd_grid(int64_t n, int *shift)
+{
+    int s = 0;
+    while (s < 10 && ((int64_t)512 << (s + 1)) <= n) ++s;
+    const int64_t c = (int64_t)1 << s;
+    const int64_t chunks = (n + c - 1) / c;
+    *shift = s;
+    return (unsigned)((chunks + 7) / 8 * 8 * c);
+}
+
 // Either the caller's workspace (checked for size) or stream-ordered memory.
 int get_workspace(const tsx_plan *p, int64_t bytes, void *ws, int64_t ws_bytes, hipStream_t stream,
                   Scratch &owned, void **out)
@@ -568,13 +585,14 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     v.proj_stride = ps;
     v.ep = ep;
     const int64_t nb = (int64_t)tiles_u * tiles_v * p->n_angles * batch;
-    if (nb > (int64_t)0x7fffffff - 8) return fail(TSX_ERR_UNSUPPORTED, "forward projection too large for one launch");
+    if (nb > MAX_LOGICAL) return fail(TSX_ERR_UNSUPPORTED, "forward projection too large for one launch");
     const int n_blocks = (int)nb;
-    const unsigned grid = (unsigned)(((n_blocks + 7) / 8) * 8);
+    int shift;
+    const unsigned grid = xcd_grid(n_blocks, &shift);
     const int variant = fp_variant_of(p);
     switch (p->ss) {
 #define TSX_FP_LAUNCH(K, V)                                                                        \
-    hipLaunchKernelGGL((fp_padded<K, V>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream, v, n_blocks)
+    hipLaunchKernelGGL((fp_padded<K, V>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream, v, n_blocks, shift)
 #define TSX_FP_CASE(K)                                                                             \
     case K:                                                                                        \
         if (variant == 4)                                                                          \
@@ -671,13 +689,14 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
     const int BZ = bp_brick_z_of(p);
     BpView v = bp_view(p, BZ, vol, vs, ly, (uint32_t)(sino * 4), (int)batch, ep);
     const int64_t nb = (int64_t)v.bricks_x * v.bricks_y * v.bricks_z * batch;
-    if (nb > (int64_t)0x7fffffff - 8) return fail(TSX_ERR_UNSUPPORTED, "backprojection too large for one launch");
+    if (nb > MAX_LOGICAL) return fail(TSX_ERR_UNSUPPORTED, "backprojection too large for one launch");
     const int n_blocks = (int)nb;
-    const unsigned grid = (unsigned)(((n_blocks + 7) / 8) * 8);
+    int shift;
+    const unsigned grid = xcd_grid(n_blocks, &shift);
     if (BZ == 32)
-        hipLaunchKernelGGL(bp_separable<32>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks);
+        hipLaunchKernelGGL(bp_separable<32>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
     else
-        hipLaunchKernelGGL(bp_separable<16>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks);
+        hipLaunchKernelGGL(bp_separable<16>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
     TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
 }
@@ -764,7 +783,7 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
     int variant = -1;
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
-        const unsigned grid = (unsigned)(((m.begin[m.n] + 7) / 8) * 8);
+        const unsigned grid = xcd_grid(m.begin[m.n], &m.shift);
         if (variant == 4)
             hipLaunchKernelGGL(fp_padded_multi<4>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
         else if (variant == 3)
@@ -813,7 +832,7 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         v.proj_stride = 0;
         v.ep = ep;
         const int64_t blocks = (int64_t)v.tiles_u * v.tiles_v * v.n_angles;
-        if ((int64_t)m.begin[m.n] + blocks > (int64_t)0x7fffffff - 8) {
+        if ((int64_t)m.begin[m.n] + blocks > MAX_LOGICAL) {
             int rc = flush();
             if (rc) return rc;
             m.v[0] = v;
@@ -849,7 +868,7 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
     int64_t woff = 0;
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
-        const unsigned grid = (unsigned)(((m.begin[m.n] + 7) / 8) * 8);
+        const unsigned grid = xcd_grid(m.begin[m.n], &m.shift);
         if (BZ == 32)
             hipLaunchKernelGGL(bp_separable_multi<32>, dim3(grid), dim3(BR, BR), 0, stream, m);
         else
@@ -882,7 +901,7 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
                            p->cols, projs[k], (int64_t)0, ly, sino);
         BpView v = bp_view(p, BZ, vols[k], 0, ly, (uint32_t)(sino * 4), 1, ep);
         const int64_t blocks = (int64_t)v.bricks_x * v.bricks_y * v.bricks_z;
-        if ((int64_t)m.begin[m.n] + blocks > (int64_t)0x7fffffff - 8) {
+        if ((int64_t)m.begin[m.n] + blocks > MAX_LOGICAL) {
             int rc = flush();
             if (rc) return rc;
         }
