@@ -436,8 +436,10 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                     //   kf == pk:     (carry + lo, hi)   hi is rewritten later
                     //   kf == pk + 1: (carry, lo)
                     // For a covered column every cell of the brick is written, so
-                    // it needs no zeroing. Rows clamped to kf = BZ and the write
-                    // base clamped to BZ touch only cells BZ, BZ + 1.
+                    // it needs no zeroing. kf is not clamped here: the rows of the
+                    // range have kf in [-2, BZ], and the rows past it (kf above
+                    // BZ) reach the LDS only through the write base min(pk, BZ),
+                    // so they touch cells BZ, BZ + 1 only.
                     auto splat_carry = [&]() {
                         const bool up = fB > 0.0f;
                         const float dB = fabsf(fB);
@@ -456,7 +458,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
 #pragma unroll
                             for (int r = 0; r < 4; ++r) {
                                 const float ti = fmaf((float)r, dB, tg);
-                                const int kf = min(max(floor_to_int(ti), -2), BZ);
+                                const int kf = floor_to_int(ti);
                                 const float hiw = __builtin_amdgcn_fractf(ti) * yv[r];
                                 const float low = yv[r] - hiw;
                                 const bool eq = kf == pk;
