@@ -457,7 +457,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                         auto carry4 = [&](float tg, const float (&yv)[4]) {
 #pragma unroll
                             for (int r = 0; r < 4; ++r) {
-                                const float ti = fmaf((float)r, dB, tg);
+                                const float ti = r == 0 ? tg : fmaf((float)r, dB, tg);
                                 const int kf = floor_to_int(ti);
                                 const float hiw = __builtin_amdgcn_fractf(ti) * yv[r];
                                 const float low = yv[r] - hiw;
@@ -509,7 +509,7 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
                         auto carry4 = [&](float tg, const float (&yv)[4]) {
 #pragma unroll
                             for (int r = 0; r < 4; ++r) {
-                                const float ti = fmaf((float)r, dB, tg);
+                                const float ti = r == 0 ? tg : fmaf((float)r, dB, tg);
                                 const int kf = min(max(floor_to_int(ti), -2), BZ);
                                 const float hiw = __builtin_amdgcn_fractf(ti) * yv[r];
                                 const float low = yv[r] - hiw;
