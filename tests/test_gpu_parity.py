@@ -335,6 +335,39 @@ def test_fused_multi_instance_equals_single():
         assert torch.equal(b, A.T(y))
 
 
+def test_fused_more_instances_than_one_launch_takes():
+    """20 instances of assorted small shapes: more than one fused launch holds
+    (16), and block counts from a few hundred to tens of thousands, so the
+    launches take different chunk sizes in their order over the XCDs. Fused
+    results equal per-operator calls bitwise; two instances are also checked
+    against the oracle."""
+    from tomosipo_amd.fused import fused_forward, fused_backward
+
+    rng = np.random.default_rng(11)
+    ops, xs, geos = [], [], []
+    for k in range(20):
+        n = int(rng.integers(8, 40))
+        a = int(rng.integers(3, 30))
+        vg = ts.volume(shape=(n, n + 1, n + 3), size=1)
+        if k % 3 == 0:
+            pg = ts.parallel(angles=a, shape=(n, n + 5), size=(1.2, 1.6))
+        else:
+            pg = _cone_vec_helix(a, n + 2, n + 6, 1.4, 2.0, 0.1, 1e-3, seed=k)
+        ops.append(ts.operator(vg, pg))
+        geos.append((vg, pg))
+        xs.append(torch.from_numpy(rng.random((n, n + 1, n + 3), dtype=np.float32)).to(DEV))
+    print("[fused] 20 instances", flush=True)
+    ys = fused_forward(ops, xs)
+    for A, x, y in zip(ops, xs, ys):
+        assert torch.equal(y, A(x))
+    bs = fused_backward(ops, ys)
+    for A, y, b in zip(ops, ys, bs):
+        assert torch.equal(b, A.T(y))
+    for k in (0, 1):
+        g = oracle.oracle_geometry(*geos[k])
+        assert rel_l2(ys[k].cpu().numpy(), oracle.fp(g, xs[k].cpu().numpy())) <= TOL
+
+
 def test_autotune_keeps_parity_and_caches_choices():
     """Autotuning picks kernel variants per plan; results stay within the
     oracle tolerance, a second plan of the same shape takes the cached
