@@ -29,6 +29,18 @@
  *
  * Every function returns TSX_OK (0) or a negative tsx_status; the message of
  * the last failure on the calling thread is returned by tsx_last_error().
+ *
+ * Tracing and failure injection: every entry point that enqueues device work
+ * opens a roctx range named after itself (rocprofv3 --marker-trace shows
+ * them). With TSX_FAIL_LAUNCH set to a comma-separated list of call numbers,
+ * those calls of these entry points fail with TSX_ERR_HIP before enqueueing
+ * anything (calls numbered from 1 while the variable keeps its value; the
+ * count restarts when the value changes) -- for tests of failure handling.
+ *
+ * Launch limits: batches, angle ranges and fused instance lists larger than
+ * one kernel launch takes (65535 per grid dimension, 2^32 work-items per
+ * grid) are split over several launches inside the call; results do not
+ * depend on the split.
  */
 #ifndef TSX_H
 #define TSX_H
@@ -190,6 +202,32 @@ int tsx_multi_forward(const tsx_plan *const *plans, const float *const *vols, fl
 int tsx_multi_backward(const tsx_plan *const *plans, float *const *vols, const float *const *projs,
                        int n, int additive, void *workspace, int64_t workspace_bytes,
                        void *hip_stream);
+
+/* Epilogue of one instance of a fused call: how the projector's result r is
+ * written to that instance's output o.
+ *   TSX_EP_SET:        o = r
+ *   TSX_EP_ADD:        o += r
+ *   TSX_EP_RESIDUAL:   o = w * (y - r)     (forward calls only; y, w like o)
+ *   TSX_EP_SCALED_ADD: o += w * r          (w like o)
+ * The residual and scaled-add forms are the two halves of a SIRT iteration
+ * (see tsx_forward_residual / tsx_backward_scaled_add). */
+enum { TSX_EP_SET = 0, TSX_EP_ADD = 1, TSX_EP_RESIDUAL = 2, TSX_EP_SCALED_ADD = 3 };
+typedef struct {
+    int32_t mode;
+    const float *y;
+    const float *w;
+} tsx_epilogue;
+
+/* As tsx_multi_forward / tsx_multi_backward with a per-instance epilogue, so
+ * that one fused launch carries instances that overwrite, accumulate, or run
+ * a SIRT half-step. No counterpart in the reference; for the request server,
+ * whose fused launches mix plain projections with SIRT iterations. */
+int tsx_multi_forward_ep(const tsx_plan *const *plans, const float *const *vols, float *const *projs,
+                         const tsx_epilogue *eps, int n, void *workspace, int64_t workspace_bytes,
+                         void *hip_stream);
+int tsx_multi_backward_ep(const tsx_plan *const *plans, float *const *vols, const float *const *projs,
+                          const tsx_epilogue *eps, int n, void *workspace, int64_t workspace_bytes,
+                          void *hip_stream);
 
 /* The two halves of one SIRT iteration x += C * A^T(R * (y - A x)) with the
  * elementwise work fused into the projection kernels. They replace the

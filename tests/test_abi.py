@@ -182,3 +182,96 @@ def test_data_container():
     assert np.all(d.data == 1)
     with pytest.raises(AttributeError):
         d.data = np.zeros((4, 4, 4))
+
+
+def test_fused_calls_check_output_lists_and_shapes():
+    """fused_forward / fused_backward reject an output list of the wrong length
+    or with a wrongly shaped tensor before anything reaches the library."""
+    import torch
+    from tomosipo_amd.fused import fused_forward, fused_backward, Epilogue
+
+    A = ts.operator(ts.volume(shape=8, size=1), ts.parallel(angles=4, shape=(8, 12)))
+    B = ts.operator(ts.volume(shape=6, size=1), ts.parallel(angles=3, shape=(6, 9)))
+    xa, xb = torch.zeros(A.domain_shape), torch.zeros(B.domain_shape)
+    with pytest.raises(ValueError, match="one output per operator"):
+        fused_forward([A, B], [xa, xb], out=[torch.zeros(A.range_shape)])
+    with pytest.raises(ValueError, match="output 1"):
+        fused_forward([A, B], [xa, xb], out=[torch.zeros(A.range_shape), torch.zeros(A.range_shape)])
+    with pytest.raises(ValueError, match="input 0"):
+        fused_backward([A], [xa])
+    with pytest.raises(ValueError, match="output 0"):
+        fused_backward([A], [torch.zeros(A.range_shape)], out=[torch.zeros(8, 8, 7)])
+    with pytest.raises(ValueError, match="needs w"):
+        fused_backward([A], [torch.zeros(A.range_shape)], epilogues=[Epilogue("scaled_add")])
+    with pytest.raises(ValueError, match="forward projections"):
+        fused_backward([A], [torch.zeros(A.range_shape)],
+                       epilogues=[Epilogue("residual", y=xa, w=xa)])
+    with pytest.raises(ValueError, match="epilogue 0 y"):
+        fused_forward([A], [xa], epilogues=[Epilogue("residual", y=xa, w=torch.zeros(A.range_shape))])
+
+
+def test_fused_c_abi_rejects_bad_epilogues_and_pointer_lists():
+    lib = _tsx.load()
+    vg, pg = ts.volume(shape=8, size=1), ts.parallel(angles=4, shape=(8, 12))
+    p = _plan(vg, pg)
+    hs = _tsx._ptr_array([p.handle.value])
+    one = _tsx._ptr_array([1])
+    eps = (_tsx.TsxEpilogue * 1)()
+    eps[0].mode = _tsx.TSX_EP_RESIDUAL
+    # host-only plans are rejected first; then NULL epilogue arrays
+    assert lib.tsx_multi_forward_ep(hs, one, one, eps, 1, None, 0, None) == -4
+    assert lib.tsx_multi_forward_ep(hs, one, one, None, 1, None, 0, None) == -1
+    with pytest.raises(ValueError):
+        _tsx.multi_project([p], [1, 2], [1], True, False, None)
+
+
+def test_library_links_the_roctx_tracer():
+    """The entry points open roctx ranges: the library links the roctx runtime."""
+    import subprocess
+
+    out = subprocess.run(["ldd", _tsx.LIB_PATH], capture_output=True, text=True).stdout
+    assert "roctx" in out
+
+
+def test_legacy_projection_argument_errors():
+    """ts.astra.project / forward / backward raise as tomosipo's astra.py:262-268."""
+    vd = ts.data(ts.volume(shape=4))
+    pd = ts.data(ts.parallel(angles=3, shape=4))
+    with pytest.raises(ValueError, match="forward argument"):
+        ts.astra.project(vd, pd)
+    with pytest.raises(ValueError, match="at least one projection dataset"):
+        ts.astra.forward(vd)
+    with pytest.raises(ValueError, match="at least one projection dataset"):
+        ts.astra.backward(pd, pd)
+    with pytest.raises(TypeError):
+        ts.astra.forward(vd, pd, projector=object())
+
+
+def test_dlpack_link_on_host_arrays():
+    """A foreign DLPack array (tests/dlpack_producer.py, standing in for CuPy)
+    gets the DLPack link: shape errors, float32 coercion with a warning, new
+    arrays of the producer's own type sharing memory with their torch view,
+    compatibility with torch tensors and numpy arrays on the same device."""
+    import torch
+    from tests import dlpack_producer as dp
+    from tomosipo_amd.links.dlpack import DLPackLink
+    from tomosipo_amd.links.base import are_compatible
+
+    vg = ts.volume(shape=4)
+    a = dp.Array(torch.zeros(4, 4, 4))
+    lk = ts.link(vg, a)
+    assert isinstance(lk, DLPackLink) and lk.data is a
+    with pytest.raises(ValueError):
+        ts.link(vg, dp.Array(torch.zeros(4, 4, 5)))
+    with pytest.warns(UserWarning):
+        c = ts.link(vg, dp.Array(torch.zeros(4, 4, 4, dtype=torch.float64)))
+    assert isinstance(c.data, dp.Array) and c.tensor.dtype == torch.float32
+    z = lk.new_full((2, 3, 4), 2.0)
+    assert isinstance(z, DLPackLink) and isinstance(z.data, dp.Array)
+    z.tensor[0, 0, 0] = 5.0
+    assert z.data.to_torch()[0, 0, 0].item() == 5.0
+    assert are_compatible(lk, ts.link(vg, torch.zeros(4, 4, 4)))
+    assert are_compatible(lk, ts.link(vg, np.zeros((4, 4, 4), dtype=np.float32)))
+    # numpy arrays and torch tensors keep their own links
+    assert type(ts.link(vg, np.zeros((4, 4, 4), dtype=np.float32))).__name__ == "NumpyLink"
+    assert type(ts.link(vg, torch.zeros(4, 4, 4))).__name__ == "TorchLink"

@@ -336,23 +336,24 @@ def test_fused_multi_instance_equals_single():
 
 
 def test_fused_more_instances_than_one_launch_takes():
-    """20 instances of assorted small shapes: more than one fused launch holds
-    (16), and block counts from a few hundred to tens of thousands, so the
-    launches take different chunk sizes in their order over the XCDs. Fused
-    results equal per-operator calls bitwise; two instances are also checked
-    against the oracle."""
+    """20 instances of assorted shapes: more than one fused launch holds (16).
+    Two of them are 128^3 scans with 200 angles (9600 forward blocks each), so
+    the first fused forward launch has more than 16384 blocks and takes chunks
+    of 4 in its order over the XCDs; the small ones (tens to a few thousand
+    blocks) keep chunks of 1. Fused results equal per-operator calls bitwise,
+    and every instance is checked against the oracle."""
     from tomosipo_amd.fused import fused_forward, fused_backward
 
     rng = np.random.default_rng(11)
     ops, xs, geos = [], [], []
     for k in range(20):
-        n = int(rng.integers(8, 40))
-        a = int(rng.integers(3, 30))
+        n = 128 if k in (3, 7) else int(rng.integers(8, 40))
+        a = 200 if k in (3, 7) else int(rng.integers(3, 30))
         vg = ts.volume(shape=(n, n + 1, n + 3), size=1)
         if k % 3 == 0:
-            pg = ts.parallel(angles=a, shape=(n, n + 5), size=(1.2, 1.6))
+            pg = ts.parallel(angles=a, shape=(n, 3 * n // 2), size=(1.2, 1.6))
         else:
-            pg = _cone_vec_helix(a, n + 2, n + 6, 1.4, 2.0, 0.1, 1e-3, seed=k)
+            pg = _cone_vec_helix(a, n + 2, 3 * n // 2, 1.4, 2.0, 0.1, 1e-3, seed=k)
         ops.append(ts.operator(vg, pg))
         geos.append((vg, pg))
         xs.append(torch.from_numpy(rng.random((n, n + 1, n + 3), dtype=np.float32)).to(DEV))
@@ -363,9 +364,51 @@ def test_fused_more_instances_than_one_launch_takes():
     bs = fused_backward(ops, ys)
     for A, y, b in zip(ops, ys, bs):
         assert torch.equal(b, A.T(y))
-    for k in (0, 1):
+    for k in range(20):
         g = oracle.oracle_geometry(*geos[k])
-        assert rel_l2(ys[k].cpu().numpy(), oracle.fp(g, xs[k].cpu().numpy())) <= TOL
+        assert rel_l2(ys[k].cpu().numpy(), oracle.fp(g, xs[k].cpu().numpy())) <= TOL, k
+        assert rel_l2(bs[k].cpu().numpy(), oracle.bp(g, ys[k].double().cpu().numpy())) <= TOL, k
+
+
+def test_fused_epilogues_equal_single_calls():
+    """Per-instance epilogues in one fused launch: a plain projection, an
+    accumulation and the two SIRT half-steps side by side, each equal to the
+    single-operator call with the same epilogue (tsx_forward_residual,
+    tsx_backward_scaled_add) bitwise."""
+    from tomosipo_amd.fused import fused_forward, fused_backward, Epilogue
+
+    ops = [ts.operator(ts.volume(shape=n, size=1),
+                       ts.cone(angles=20 + n, shape=(n, 3 * n // 2), size=(1.4, 2.1), cone_angle=0.5))
+           for n in (16, 24, 32)]
+    g = torch.Generator(device=DEV)
+    g.manual_seed(8)
+    xs = [torch.rand(A.domain_shape, generator=g, device=DEV) for A in ops]
+    ys = [torch.rand(A.range_shape, generator=g, device=DEV) for A in ops]
+    ws = [torch.rand(A.range_shape, generator=g, device=DEV) for A in ops]
+    acc = torch.rand(ops[1].range_shape, generator=g, device=DEV)
+    out = [torch.empty(ops[0].range_shape, device=DEV), acc.clone(), torch.empty(ops[2].range_shape, device=DEV)]
+    fused_forward(ops, xs, out=out,
+                  epilogues=[Epilogue("set"), Epilogue("add"), Epilogue("residual", y=ys[2], w=ws[2])])
+    assert torch.equal(out[0], ops[0](xs[0]))
+    assert torch.equal(out[1], acc + ops[1](xs[1]))
+    p2 = ops[2].projector.plan(0)
+    ref = torch.empty(ops[2].range_shape, device=DEV)
+    p2.forward_residual(xs[2].data_ptr(), ys[2].data_ptr(), ws[2].data_ptr(), ref.data_ptr(),
+                        torch.cuda.current_stream().cuda_stream,
+                        workspace=torch.empty(p2.workspace_size(0), dtype=torch.uint8, device=DEV).data_ptr(),
+                        workspace_bytes=p2.workspace_size(0))
+    assert torch.equal(out[2], ref)
+    cs = [torch.rand(A.domain_shape, generator=g, device=DEV) for A in ops]
+    vols = [x.clone() for x in xs]
+    fused_backward(ops, ys, out=vols, epilogues=[Epilogue("scaled_add", w=c) for c in cs])
+    for A, x, y, c, v in zip(ops, xs, ys, cs, vols):
+        p = A.projector.plan(0)
+        r = x.clone()
+        nb = p.workspace_size(1)
+        p.backward_scaled_add(r.data_ptr(), y.data_ptr(), c.data_ptr(), torch.cuda.current_stream().cuda_stream,
+                              workspace=torch.empty(max(nb, 1), dtype=torch.uint8, device=DEV).data_ptr(),
+                              workspace_bytes=nb)
+        assert torch.equal(v, r)
 
 
 def test_autotune_keeps_parity_and_caches_choices():

@@ -139,3 +139,123 @@ def test_oracle_catches_planted_sign_error():
     x = np.random.default_rng(1).random((64, 64, 64))
     y = oracle.fp(oracle.oracle_geometry(vg, bad), x)
     assert not np.allclose(y[:, 0, 16:80], x.sum(axis=1) / 64)
+
+
+# ---------------------------------------------------------------------------
+# cone beam: closed forms that fix the source position, the magnification
+# and the detector orientation, with planted mistakes they must catch
+# ---------------------------------------------------------------------------
+def _cone_geometry(theta, sod=3.0, sdd=5.0, rows=33, cols=49, size=(1.65, 2.45)):
+    return ts.cone(angles=[theta], shape=(rows, cols), size=size, src_orig_dist=sod,
+                   src_det_dist=sdd).to_vec()
+
+
+def _cone_chords(pg, vg, pixels):
+    """FP of an all-ones unit-cube volume for the given pixels (i, j)."""
+    g = oracle.oracle_geometry(vg, pg)
+    y = oracle.fp(g, np.ones(g.vol_shape))
+    return np.array([y[i, 0, j] for i, j in pixels])
+
+
+def _chord_closed_form(pg, vg, pixels):
+    """Chord n_m * l of the ray from the source through pixel (i, j): every one
+    of the n_m slices of the stepping axis m is crossed with full bilinear
+    support (the rays stay well inside the cube), l = |h * D| / |D_m|."""
+    rows, cols = pg.det_shape
+    s, d, u, v = pg.src_pos[0], pg.det_pos[0], pg.det_u[0], pg.det_v[0]
+    h = np.array(vg.voxel_size)
+    out = []
+    for i, j in pixels:
+        p = d + (j + 0.5 - cols / 2) * u + (i + 0.5 - rows / 2) * v
+        D = (p - s) / h
+        m = int(np.argmax(np.abs(D)))
+        out.append(vg.shape[m] * np.linalg.norm(h * D) / abs(D[m]))
+    return np.array(out)
+
+
+# the central pixel and pixels up to 4 rows / columns from it: their rays
+# cross every y-slice of the cube with full bilinear support at both angles
+PIXELS = [(16, 24), (16, 28), (12, 24), (20, 20), (13, 27)]
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.4])
+def test_cone_chord_lengths_all_ones(theta):
+    """All-ones unit cube 32^3: FP of a ray equals its chord n_m * l, which for
+    the central ray is 1 / cos(theta) and off-centre grows with the ray's
+    angle to the central ray (set by the source-detector distance)."""
+    vg = ts.volume(shape=32, size=1)
+    pg = _cone_geometry(theta)
+    got = _cone_chords(pg, vg, PIXELS)
+    np.testing.assert_allclose(got, _chord_closed_form(pg, vg, PIXELS), rtol=1e-12)
+    assert got[0] == pytest.approx(1 / np.cos(theta), rel=1e-12)  # the central ray
+    # a planted source distance 10 % off changes the off-centre chords
+    for sod in (2.7, 3.3):
+        bad = _cone_geometry(theta, sod=sod, sdd=5.0 + sod - 3.0)  # detector kept in place
+        assert not np.allclose(_cone_chords(bad, vg, PIXELS)[1:], got[1:], rtol=1e-7)
+
+
+def _footprint_centroid(y):
+    """Weighted centroid (i, j) of a single-angle sinogram y[i, 0, j]."""
+    w = y[:, 0, :]
+    ii, jj = np.meshgrid(np.arange(w.shape[0]), np.arange(w.shape[1]), indexing="ij")
+    return (w * ii).sum() / w.sum(), (w * jj).sum() / w.sum()
+
+
+def _magnified_position(pg, X):
+    """Detector pixel coordinates (i, j) of s + lam (X - s), the central
+    projection of X from the source onto the detector plane."""
+    rows, cols = pg.det_shape
+    s, d, u, v = pg.src_pos[0], pg.det_pos[0], pg.det_u[0], pg.det_v[0]
+    n = np.cross(v, u)
+    lam = np.dot(d - s, n) / np.dot(X - s, n)
+    p = s + lam * (X - s)
+    j = np.dot(p - d, u) / np.dot(u, u) + cols / 2 - 0.5
+    i = np.dot(p - d, v) / np.dot(v, v) + rows / 2 - 0.5
+    return i, j
+
+
+@pytest.mark.parametrize("theta", [0.0, 0.7])
+def test_cone_off_centre_voxel_projects_to_its_magnified_position(theta):
+    """One non-zero voxel off the centre (64^3 unit cube, detector pixels a
+    quarter of the magnified voxel): its footprint's centroid is the central
+    projection of the voxel centre, i = v-coordinate and j = u-coordinate of
+    s + lam (X - s), to 0.05 pixel: the footprint is a tent of about 1.6
+    pixels' half-width centred there, and the centroid of its samples at the
+    pixel centres is off by a few hundredths of a pixel, depending on where
+    the centre falls between pixels. A detector with flipped u, or a source
+    10 % nearer or farther, moves it by pixels."""
+    vg = ts.volume(shape=64, size=1)
+    k = (20, 45, 13)                                  # (z, y, x) voxel index
+    X = (np.array(k) + 0.5) / 64 - 0.5                 # its centre (z, y, x)
+    x = np.zeros((64, 64, 64))
+    x[k] = 1.0
+    pg = _cone_geometry(theta, sod=3.0, sdd=6.0, rows=97, cols=97, size=(1.9, 1.9))
+    y = oracle.fp(oracle.oracle_geometry(vg, pg), x)
+    ci, cj = _footprint_centroid(y)
+    ei, ej = _magnified_position(pg, X)
+    assert abs(ci - ei) < 0.05 and abs(cj - ej) < 0.05, (ci, ei, cj, ej)
+    bads = [ts.cone_vec(shape=pg.det_shape, src_pos=pg.src_pos, det_pos=pg.det_pos, det_v=pg.det_v,
+                        det_u=-pg.det_u)]
+    for sod in (2.7, 3.3):
+        bads.append(_cone_geometry(theta, sod=sod, sdd=6.0 + sod - 3.0, rows=97, cols=97, size=(1.9, 1.9)))
+    for bad in bads:
+        bi, bj = _footprint_centroid(oracle.fp(oracle.oracle_geometry(vg, bad), x))
+        assert max(abs(bi - ei), abs(bj - ej)) > 0.5, (bi, bj, ei, ej)
+
+
+def test_cone_magnification_closed_form():
+    """The footprint centroids of voxels along the source axis scale with the
+    magnification SDD / (SOD + depth) (theta = 0: the source at -SOD along y),
+    to 0.05 pixel as above."""
+    vg = ts.volume(shape=64, size=1)
+    pg = _cone_geometry(0.0, sod=3.0, sdd=6.0, rows=97, cols=97, size=(1.9, 1.9))
+    pitch = 1.9 / 97
+    for ky in (5, 32, 58):
+        k = (40, ky, 50)
+        X = (np.array(k) + 0.5) / 64 - 0.5
+        x = np.zeros((64, 64, 64))
+        x[k] = 1.0
+        ci, cj = _footprint_centroid(oracle.fp(oracle.oracle_geometry(vg, pg), x))
+        M = 6.0 / (3.0 + X[1])                     # depth along +y from the source at y = -3
+        assert (ci - 48) * pitch == pytest.approx(M * X[0], abs=0.05 * pitch)
+        assert (cj - 48) * pitch == pytest.approx(M * X[2], abs=0.05 * pitch)

@@ -17,6 +17,8 @@ HIPCC_FLAGS = [
     # packed FP32 instructions stay off in every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall",
+    # roctx ranges around the entry points (rocprofv3 --marker-trace)
+    "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib",
 ]
 
 

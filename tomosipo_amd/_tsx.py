@@ -24,6 +24,10 @@ TSX_MODE_ADD = 0
 TSX_MODE_SET = 1
 TSX_OP_FP = 0
 TSX_OP_BP = 1
+TSX_EP_SET = 0
+TSX_EP_ADD = 1
+TSX_EP_RESIDUAL = 2
+TSX_EP_SCALED_ADD = 3
 
 _STATUS = {
     -1: "TSX_ERR_INVALID_ARGUMENT",
@@ -39,7 +43,8 @@ EXPORTED = (
     "tsx_workspace_size",
     "tsx_direct_fpbp3d", "tsx_forward", "tsx_backward", "tsx_forward_batched",
     "tsx_backward_batched", "tsx_multi_workspace_size", "tsx_multi_forward",
-    "tsx_multi_backward", "tsx_forward_residual", "tsx_backward_scaled_add",
+    "tsx_multi_backward", "tsx_multi_forward_ep", "tsx_multi_backward_ep",
+    "tsx_forward_residual", "tsx_backward_scaled_add",
     "tsx_forward_batched_f64", "tsx_backward_batched_f64",
     "tsx_last_error", "tsx_version",
 )
@@ -66,6 +71,10 @@ class TsxGeometry(ctypes.Structure):
         ("det_supersampling", ctypes.c_int32),
         ("vox_supersampling", ctypes.c_int32),
     ]
+
+
+class TsxEpilogue(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("y", ctypes.c_void_p), ("w", ctypes.c_void_p)]
 
 
 class TsxPlanInfo(ctypes.Structure):
@@ -120,6 +129,8 @@ def load():
         lib.tsx_multi_workspace_size.argtypes = [P, i32, i32, ctypes.POINTER(i64)]
         lib.tsx_multi_forward.argtypes = [P, P, P, i32, i32, P, i64, P]
         lib.tsx_multi_backward.argtypes = [P, P, P, i32, i32, P, i64, P]
+        lib.tsx_multi_forward_ep.argtypes = [P, P, P, P, i32, P, i64, P]
+        lib.tsx_multi_backward_ep.argtypes = [P, P, P, P, i32, P, i64, P]
         lib.tsx_forward_residual.argtypes = [P, P, i64, P, P, P, i64, i64, P, i64, P]
         lib.tsx_backward_scaled_add.argtypes = [P, P, i64, P, i64, P, i64, P, i64, P]
         lib.tsx_forward_batched_f64.argtypes = [P, P, i64, P, i64, i64, i32, P]
@@ -285,11 +296,27 @@ def multi_workspace_size(plans, op):
 
 
 def multi_project(plans, vol_ptrs, proj_ptrs, forward, additive, stream, workspace=None,
-                  workspace_bytes=0):
-    """One fused launch over many instances (tsx_multi_forward / tsx_multi_backward)."""
+                  workspace_bytes=0, epilogues=None):
+    """One fused launch over many instances (tsx_multi_forward / tsx_multi_backward).
+
+    ``epilogues``: None (all SET, or all ADD with ``additive``), or one
+    ``(mode, y_ptr, w_ptr)`` per instance (tsx_multi_forward_ep / _backward_ep).
+    """
     lib = load()
+    n = len(plans)
+    if len(vol_ptrs) != n or len(proj_ptrs) != n:
+        raise ValueError("one volume and one projection pointer per plan are required")
     hs = _ptr_array([p.handle.value for p in plans])
     vs = _ptr_array(vol_ptrs)
     ps = _ptr_array(proj_ptrs)
-    fn = lib.tsx_multi_forward if forward else lib.tsx_multi_backward
-    _check(fn(hs, vs, ps, len(plans), int(bool(additive)), workspace, int(workspace_bytes), stream))
+    if epilogues is None:
+        fn = lib.tsx_multi_forward if forward else lib.tsx_multi_backward
+        _check(fn(hs, vs, ps, n, int(bool(additive)), workspace, int(workspace_bytes), stream))
+        return
+    if len(epilogues) != n:
+        raise ValueError("one epilogue per plan is required")
+    eps = (TsxEpilogue * max(n, 1))()
+    for k, (mode, y, w) in enumerate(epilogues):
+        eps[k].mode, eps[k].y, eps[k].w = int(mode), y, w
+    fn = lib.tsx_multi_forward_ep if forward else lib.tsx_multi_backward_ep
+    _check(fn(hs, vs, ps, eps, n, workspace, int(workspace_bytes), stream))

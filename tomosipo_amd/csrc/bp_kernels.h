@@ -195,30 +195,36 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
     emit(vol + (int64_t)b * vol_stride, ((int64_t)z * ny + y) * nx + x, b, acc, ep);
 }
 
-// ly[i, a, j] = l(a, i, j) * y[i, a, j] (detector supersampling 1).
+// ly[i, a, j] = l(a, i, j) * y[i, a, j] (detector supersampling 1). Block
+// (x, y) takes columns 256 x .. 256 x + 255 of the detector lines y,
+// y + gridDim.y, ... of the `lines` = batch * rows * n_angles lines (line
+// (b, i, a) is b * rows * n_angles + i * n_angles + a).
 __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_angles, int rows,
                                 int cols, const float *__restrict__ proj, int64_t proj_stride,
-                                float *__restrict__ out, int64_t out_stride)
+                                float *__restrict__ out, int64_t out_stride, int64_t lines)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int a = blockIdx.y;
-    const int i = blockIdx.z % rows;
-    const int b = blockIdx.z / rows;
     if (j >= cols) return;
-    const AngleParams &A = angles[a];
-    float D[3];
+    const int64_t per_entry = (int64_t)rows * n_angles;
+    for (int64_t line = blockIdx.y; line < lines; line += gridDim.y) {
+        const int64_t b = line / per_entry;
+        const int64_t ia = line - b * per_entry;
+        const int i = (int)(ia / n_angles), a = (int)(ia - (int64_t)i * n_angles);
+        const AngleParams &A = angles[a];
+        float D[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const float P = (float)(A.P0[c] + (double)j * A.U[c] + (double)i * A.V[c]);
-        D[c] = A.cone ? P - (float)A.S[c] : (float)A.S[c];
+        for (int c = 0; c < 3; ++c) {
+            const float P = (float)(A.P0[c] + (double)j * A.U[c] + (double)i * A.V[c]);
+            D[c] = A.cone ? P - (float)A.S[c] : (float)A.S[c];
+        }
+        const float inv = 1.0f / D[0];
+        const float a1 = D[1] * inv, a2 = D[2] * inv;
+        const float len = (D[0] == 0.0f) ? 0.0f
+            : sqrtf((float)(A.h[0] * A.h[0]) + (float)(A.h[1] * A.h[1]) * a1 * a1 +
+                    (float)(A.h[2] * A.h[2]) * a2 * a2);
+        const int64_t o = ia * cols + j;
+        out[b * out_stride + o] = len * proj[b * proj_stride + o];
     }
-    const float inv = 1.0f / D[0];
-    const float a1 = D[1] * inv, a2 = D[2] * inv;
-    const float len = (D[0] == 0.0f) ? 0.0f
-        : sqrtf((float)(A.h[0] * A.h[0]) + (float)(A.h[1] * A.h[1]) * a1 * a1 +
-                (float)(A.h[2] * A.h[2]) * a2 * a2);
-    const int64_t o = ((int64_t)i * n_angles + a) * cols + j;
-    out[(int64_t)b * out_stride + o] = len * proj[(int64_t)b * proj_stride + o];
 }
 
 // Detector column where the ray through in-slice point (n, c) (axes m, c2)

@@ -139,14 +139,16 @@ __device__ T fp_ray_general(const AngleParams &A, const T *__restrict__ vol, dou
 
 template <typename T>
 __global__ void __launch_bounds__(FP_BX *FP_BY)
-fp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
+fp_general(const AngleParams *__restrict__ angles, int n_angles, int a0, int rows, int cols, int ss,
            int tiles_u, const T *__restrict__ vol, int64_t vol_stride, T *__restrict__ proj,
            int64_t proj_stride, Epilogue ep)
 {
+    // block (tile, angle a0 + blockIdx.y, batch entry blockIdx.z); the host
+    // splits angle and batch ranges over launches of at most 65535 each
     const int tile = blockIdx.x;
     const int j = (tile % tiles_u) * FP_BX + threadIdx.x;
     const int i = (tile / tiles_u) * FP_BY + threadIdx.y;
-    const int a = blockIdx.y;
+    const int a = a0 + (int)blockIdx.y;
     const int b = blockIdx.z;
     if (j >= cols || i >= rows) return;
     const AngleParams &A = angles[a];
@@ -475,6 +477,7 @@ __device__ __forceinline__ float march_padded_sync(__amdgpu_buffer_rsrc_t rs, co
 struct FpView {
     const AngleParams *angles;
     int n_angles, rows, cols, tiles_u, tiles_v, batch;
+    int a0, na;  // angles a0 .. a0 + na - 1 of the n_angles are projected by this launch
     Layouts L;
     float *proj;
     int64_t proj_stride;
@@ -487,8 +490,8 @@ struct FpView {
 template <int SS, int PIPE>
 __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
 {
-    const int a = logical % v.n_angles;
-    const int rest = logical / v.n_angles;
+    const int a = v.a0 + logical % v.na;
+    const int rest = logical / v.na;
     const int tiles = v.tiles_u * v.tiles_v;
     const int tile = rest % tiles;
     const int b = rest / tiles;

@@ -12,8 +12,10 @@
 //   l = |h * D| / |D_m|,
 // with zero outside the volume; the backprojection is the exact transpose.
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -48,6 +50,53 @@ static int fail(int code, const std::string &msg)
         hipError_t e_ = (expr);                                                          \
         if (e_ != hipSuccess)                                                            \
             return fail(TSX_ERR_HIP, std::string(#expr " failed: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// Every entry point that enqueues device work opens a roctx range named
+// after itself (visible with rocprofv3 --marker-trace) and passes through
+// the fault injector below.
+struct TraceRange {
+    explicit TraceRange(const char *name) { roctxRangePushA(name); }
+    ~TraceRange() { roctxRangePop(); }
+};
+
+// Fault injection for tests of failure handling: with TSX_FAIL_LAUNCH set to
+// a comma-separated list of call numbers, those calls of the entry points
+// that enqueue device work fail with TSX_ERR_HIP before enqueueing anything.
+// Calls are numbered from 1 among the calls made while the variable holds its
+// current value; the count restarts whenever the value changes.
+int injected_failure(const char *entry)
+{
+    static std::mutex mu;
+    static std::string seen;
+    static int64_t count = 0;
+    const char *e = std::getenv("TSX_FAIL_LAUNCH");
+    const std::string val = e ? e : "";
+    std::lock_guard<std::mutex> lk(mu);
+    if (val != seen) {
+        seen = val;
+        count = 0;
+    }
+    if (val.empty()) return TSX_OK;
+    ++count;
+    const char *q = val.c_str();
+    while (*q) {
+        char *end = nullptr;
+        const long long k = std::strtoll(q, &end, 10);
+        if (end == q) break;
+        if (k == count)
+            return fail(TSX_ERR_HIP, std::string(entry) + ": injected failure of call " +
+                                         std::to_string(count) + " (TSX_FAIL_LAUNCH=" + val + ")");
+        q = *end == ',' ? end + 1 : end;
+    }
+    return TSX_OK;
+}
+
+#define TSX_ENTRY(name)                          \
+    TraceRange trace_range_(name);               \
+    do {                                         \
+        int inj_ = injected_failure(name);       \
+        if (inj_) return inj_;                   \
     } while (0)
 
 struct tsx_plan {
@@ -458,9 +507,9 @@ int64_t bp_workspace_elems(const tsx_plan *p)
 // The general backprojector takes length-weighted projections (like the
 // separable one) for detector supersampling 1, within the launch limits of
 // the weighting pass.
-bool bp_general_prescaled(const tsx_plan *p, int64_t batch)
+bool bp_general_prescaled(const tsx_plan *p, int64_t /*batch*/)
 {
-    return !p->bp_separable && p->ss == 1 && (int64_t)p->rows * batch <= 65535;
+    return !p->bp_separable && p->ss == 1;
 }
 
 int64_t workspace_bytes(const tsx_plan *p, int op, int64_t batch)
@@ -476,8 +525,16 @@ int64_t workspace_bytes(const tsx_plan *p, int op, int64_t batch)
 // number of chunk slots on every XCD (blocks past the last id return at once).
 // Single 512^3 instance, forward: 45.4 ms with ~64 chunks per XCD, 41.5 ms
 // with ~256 (the chunks' costs differ, more of them even out the XCDs).
-// Logical ids stay below MAX_LOGICAL so that the grid fits in an int.
+// Logical ids stay below MAX_LOGICAL so that the grid fits in an int, and the
+// grid's work-items (blocks x threads per block) below 2^32, the limit of the
+// dispatch packet: max_blocks() gives the bound for a block size (the grid
+// rounds the ids up by less than 9 x 1024).
 constexpr int64_t MAX_LOGICAL = 0x7fffffff - 9 * 1024;
+
+int64_t max_blocks(int threads_per_block)
+{
+    return std::min(MAX_LOGICAL, (int64_t)(0xffffffffLL / threads_per_block) - 9 * 1024);
+}
 
 unsigned xcd_grid(int64_t n, int *shift)
 {
@@ -487,6 +544,29 @@ unsigned xcd_grid(int64_t n, int *shift)
     const int64_t chunks = (n + c - 1) / c;
     *shift = s;
     return (unsigned)((chunks + 7) / 8 * 8 * c);
+}
+
+// The epilogue of batch entries b0, b0 + 1, ... of a launch that starts at
+// entry b0 of the caller's batch.
+Epilogue ep_from(Epilogue e, int64_t b0)
+{
+    if (e.y) e.y += b0 * e.stride;
+    if (e.w) e.w += b0 * e.stride;
+    return e;
+}
+
+// Grid limits of the launches with one grid dimension per batch entry, angle
+// or slice (65535 per dimension).
+constexpr int64_t GRID_DIM_MAX = 65535;
+
+// The length-weighted projections ly = l * y of `batch` entries.
+void launch_scale_by_length(const tsx_plan *p, const float *proj, int64_t ps, float *ly, int64_t ly_stride,
+                            int64_t batch, hipStream_t stream)
+{
+    const int64_t lines = (int64_t)p->rows * p->n_angles * batch;
+    dim3 grid((unsigned)((p->cols + 255) / 256), (unsigned)std::min(lines, GRID_DIM_MAX));
+    hipLaunchKernelGGL(scale_by_length, grid, dim3(256), 0, stream, p->dev, p->n_angles, p->rows, p->cols,
+                       proj, ps, ly, ly_stride, lines);
 }
 
 // Either the caller's workspace (checked for size) or stream-ordered memory.
@@ -543,6 +623,28 @@ void layout_offsets(const tsx_plan *p, int64_t offs[3], int64_t *per_entry)
     *per_entry = e;
 }
 
+// fp_general over every angle and batch entry, in launches of at most 65535
+// angles and 65535 entries.
+template <typename T>
+int launch_fp_general(const tsx_plan *p, const T *vol, int64_t vs, T *proj, int64_t ps, int64_t batch,
+                      Epilogue ep, hipStream_t stream)
+{
+    const int tiles_u = (p->cols + FP_BX - 1) / FP_BX;
+    const int tiles_v = (p->rows + FP_BY - 1) / FP_BY;
+    for (int64_t b0 = 0; b0 < batch; b0 += GRID_DIM_MAX) {
+        const int64_t nb = std::min(GRID_DIM_MAX, batch - b0);
+        for (int a0 = 0; a0 < p->n_angles; a0 += (int)GRID_DIM_MAX) {
+            const int na = (int)std::min<int64_t>(GRID_DIM_MAX, p->n_angles - a0);
+            dim3 grid((unsigned)(tiles_u * tiles_v), (unsigned)na, (unsigned)nb);
+            hipLaunchKernelGGL(fp_general<T>, grid, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles, a0,
+                               p->rows, p->cols, p->ss, tiles_u, vol + b0 * vs, vs, proj + b0 * ps, ps,
+                               ep_from(ep, b0));
+        }
+    }
+    TSX_HIP_CHECK(hipGetLastError());
+    return TSX_OK;
+}
+
 int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int64_t ps,
               int64_t batch, Epilogue ep, hipStream_t stream, void *wsp = nullptr,
               int64_t ws_bytes = 0)
@@ -550,15 +652,8 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     if (batch <= 0) return TSX_OK;
     const int tiles_u = (p->cols + FP_BX - 1) / FP_BX;
     const int tiles_v = (p->rows + FP_BY - 1) / FP_BY;
-    if (!p->fp_padded || force_general()) {
-        if (p->n_angles > 65535 || batch > 65535)
-            return fail(TSX_ERR_UNSUPPORTED, "more than 65535 angles or batch elements per launch");
-        dim3 grid((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
-        hipLaunchKernelGGL(fp_general<float>, grid, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
-                           p->rows, p->cols, p->ss, tiles_u, vol, vs, proj, ps, ep);
-        TSX_HIP_CHECK(hipGetLastError());
-        return TSX_OK;
-    }
+    if (!p->fp_padded || force_general() || p->ss > 4)
+        return launch_fp_general<float>(p, vol, vs, proj, ps, batch, ep, stream);
     // padded copies of every batch entry for each stepping axis in use
     int64_t per_entry = 0;
     int64_t offs[3] = {0, 0, 0};
@@ -570,29 +665,39 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     float *base = (float *)wptr;
     for (int64_t b = 0; b < batch; ++b) build_layouts(p, vol + b * vs, base + b * per_entry, offs, stream);
     TSX_HIP_CHECK(hipGetLastError());
-    FpView v;
-    v.angles = p->dev;
-    v.n_angles = p->n_angles;
-    v.rows = p->rows;
-    v.cols = p->cols;
-    v.tiles_u = tiles_u;
-    v.tiles_v = tiles_v;
-    v.batch = (int)batch;
-    for (int c = 0; c < 3; ++c) {
-        v.L.ptr[c] = p->axis_count[c] > 0 ? base + offs[c] : nullptr;
-        v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4 * fp_esize(p));
-    }
-    v.L.stride = per_entry;
-    v.proj = proj;
-    v.proj_stride = ps;
-    v.ep = ep;
-    const int64_t nb = (int64_t)tiles_u * tiles_v * p->n_angles * batch;
-    if (nb > MAX_LOGICAL) return fail(TSX_ERR_UNSUPPORTED, "forward projection too large for one launch");
-    const int n_blocks = (int)nb;
-    int shift;
-    const unsigned grid = xcd_grid(n_blocks, &shift);
+    // launches of whole batch entries while they fit the grid limit, else of
+    // angle ranges of one entry
+    const int64_t tiles = (int64_t)tiles_u * tiles_v;
+    const int64_t cap = max_blocks(FP_BX * FP_BY);
+    const int64_t per_b = tiles * p->n_angles;
+    const int64_t bc = std::max<int64_t>(1, cap / per_b);      // entries per launch
+    const int na_max = (int)std::min<int64_t>(p->n_angles, std::max<int64_t>(1, cap / tiles));
     const int variant = fp_variant_of(p);
-    switch (p->ss) {
+    for (int64_t b0 = 0; b0 < batch; b0 += bc) {
+        const int64_t nbat = std::min(bc, batch - b0);
+        for (int a0 = 0; a0 < p->n_angles; a0 += na_max) {
+            FpView v;
+            v.angles = p->dev;
+            v.n_angles = p->n_angles;
+            v.a0 = a0;
+            v.na = std::min(na_max, p->n_angles - a0);
+            v.rows = p->rows;
+            v.cols = p->cols;
+            v.tiles_u = tiles_u;
+            v.tiles_v = tiles_v;
+            v.batch = (int)nbat;
+            for (int c = 0; c < 3; ++c) {
+                v.L.ptr[c] = p->axis_count[c] > 0 ? base + b0 * per_entry + offs[c] : nullptr;
+                v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4 * fp_esize(p));
+            }
+            v.L.stride = per_entry;
+            v.proj = proj + b0 * ps;
+            v.proj_stride = ps;
+            v.ep = ep_from(ep, b0);
+            const int n_blocks = (int)(tiles * v.na * nbat);
+            int shift;
+            const unsigned grid = xcd_grid(n_blocks, &shift);
+            switch (p->ss) {
 #define TSX_FP_LAUNCH(K, V)                                                                        \
     hipLaunchKernelGGL((fp_padded<K, V>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream, v, n_blocks, shift)
 #define TSX_FP_CASE(K)                                                                             \
@@ -604,19 +709,18 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
         else                                                                                       \
             TSX_FP_LAUNCH(K, 2);                                                                   \
         break;
-        TSX_FP_CASE(1)
-        TSX_FP_CASE(2)
-        TSX_FP_CASE(3)
-        TSX_FP_CASE(4)
+                TSX_FP_CASE(1)
+                TSX_FP_CASE(2)
+                TSX_FP_CASE(3)
+                TSX_FP_CASE(4)
 #undef TSX_FP_CASE
 #undef TSX_FP_LAUNCH
-    default: {
-        dim3 g2((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
-        hipLaunchKernelGGL(fp_general<float>, g2, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
-                           p->rows, p->cols, p->ss, tiles_u, vol, vs, proj, ps, ep);
+            default:
+                break;
+            }
+            TSX_HIP_CHECK(hipGetLastError());
+        }
     }
-    }
-    TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
 }
 
@@ -643,63 +747,70 @@ BpView bp_view(const tsx_plan *p, int BZ, float *vol, int64_t vs, const float *l
     return v;
 }
 
+// bp_general over every batch entry, in launches of at most 65535 slices x
+// entries (grid z); `ly` non-null: length-weighted projections (PRESCALED).
+template <typename T, bool PRESCALED>
+int launch_bp_general(const tsx_plan *p, T *vol, int64_t vs, const T *proj, int64_t ps, int64_t batch,
+                      Epilogue ep, hipStream_t stream)
+{
+    if (p->nz > GRID_DIM_MAX) return fail(TSX_ERR_UNSUPPORTED, "volume depth exceeds 65535 slices");
+    const int64_t bc = GRID_DIM_MAX / p->nz;  // entries per launch
+    for (int64_t b0 = 0; b0 < batch; b0 += bc) {
+        const int64_t nb = std::min(bc, batch - b0);
+        dim3 grid((unsigned)((p->nx + BP_BX - 1) / BP_BX), (unsigned)((p->ny + BP_BY - 1) / BP_BY),
+                  (unsigned)(p->nz * nb));
+        hipLaunchKernelGGL((bp_general<T, PRESCALED>), grid, dim3(BP_BX, BP_BY), 0, stream, p->dev,
+                           p->n_angles, p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol + b0 * vs, vs,
+                           proj + b0 * ps, ps, ep_from(ep, b0));
+    }
+    TSX_HIP_CHECK(hipGetLastError());
+    return TSX_OK;
+}
+
 int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int64_t ps,
               int64_t batch, Epilogue ep, hipStream_t stream, void *wsp = nullptr,
               int64_t ws_bytes = 0)
 {
     if (batch <= 0) return TSX_OK;
+    const int64_t sino = (int64_t)p->rows * p->n_angles * p->cols;
     if (!p->bp_separable || force_general()) {
-        if ((int64_t)p->nz * batch > 65535)
-            return fail(TSX_ERR_UNSUPPORTED, "volume depth times batch exceeds 65535 per launch");
-        dim3 grid((unsigned)((p->nx + BP_BX - 1) / BP_BX), (unsigned)((p->ny + BP_BY - 1) / BP_BY),
-                  (unsigned)(p->nz * batch));
         if (!force_general() && bp_general_prescaled(p, batch)) {
-            const int64_t sino = (int64_t)p->rows * p->n_angles * p->cols;
             Scratch owned;
             void *wptr = nullptr;
             int rc = get_workspace(p, sino * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
             if (rc) return rc;
             float *ly = (float *)wptr;
-            dim3 sgrid((unsigned)((p->cols + 255) / 256), (unsigned)p->n_angles, (unsigned)(p->rows * batch));
-            hipLaunchKernelGGL(scale_by_length, sgrid, dim3(256), 0, stream, p->dev, p->n_angles, p->rows,
-                               p->cols, proj, ps, ly, sino);
-            hipLaunchKernelGGL((bp_general<float, true>), grid, dim3(BP_BX, BP_BY), 0, stream, p->dev,
-                               p->n_angles, p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, ly, sino,
-                               ep);
-        } else {
-            hipLaunchKernelGGL(bp_general<float>, grid, dim3(BP_BX, BP_BY), 0, stream, p->dev, p->n_angles,
-                               p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, proj, ps, ep);
+            launch_scale_by_length(p, proj, ps, ly, sino, batch, stream);
+            return launch_bp_general<float, true>(p, vol, vs, ly, sino, batch, ep, stream);
         }
-        TSX_HIP_CHECK(hipGetLastError());
-        return TSX_OK;
+        return launch_bp_general<float, false>(p, vol, vs, proj, ps, batch, ep, stream);
     }
-    const int64_t sino = (int64_t)p->rows * p->n_angles * p->cols;
     if (sino * 4 >= ((int64_t)1 << 32))
         return fail(TSX_ERR_UNSUPPORTED, "projection data of 4 GiB or more per entry");
-    if ((int64_t)p->rows * batch > 65535)
-        return fail(TSX_ERR_UNSUPPORTED, "detector rows times batch exceeds 65535 per launch");
     Scratch owned;
     void *wptr = nullptr;
     int rc = get_workspace(p, sino * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
     if (rc) return rc;
     float *ly = (float *)wptr;
-    {
-        dim3 grid((unsigned)((p->cols + 255) / 256), (unsigned)p->n_angles, (unsigned)(p->rows * batch));
-        hipLaunchKernelGGL(scale_by_length, grid, dim3(256), 0, stream, p->dev, p->n_angles, p->rows,
-                           p->cols, proj, ps, ly, sino);
-    }
+    launch_scale_by_length(p, proj, ps, ly, sino, batch, stream);
     const int BZ = bp_brick_z_of(p);
-    BpView v = bp_view(p, BZ, vol, vs, ly, (uint32_t)(sino * 4), (int)batch, ep);
-    const int64_t nb = (int64_t)v.bricks_x * v.bricks_y * v.bricks_z * batch;
-    if (nb > MAX_LOGICAL) return fail(TSX_ERR_UNSUPPORTED, "backprojection too large for one launch");
-    const int n_blocks = (int)nb;
-    int shift;
-    const unsigned grid = xcd_grid(n_blocks, &shift);
-    if (BZ == 32)
-        hipLaunchKernelGGL(bp_separable<32>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
-    else
-        hipLaunchKernelGGL(bp_separable<16>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
-    TSX_HIP_CHECK(hipGetLastError());
+    BpView v0 = bp_view(p, BZ, vol, vs, ly, (uint32_t)(sino * 4), 1, ep);
+    const int64_t bricks = (int64_t)v0.bricks_x * v0.bricks_y * v0.bricks_z;
+    if (bricks > max_blocks(BR * BR)) return fail(TSX_ERR_UNSUPPORTED, "volume too large for one launch");
+    const int64_t bc = max_blocks(BR * BR) / bricks;  // entries per launch
+    for (int64_t b0 = 0; b0 < batch; b0 += bc) {
+        const int64_t nbat = std::min(bc, batch - b0);
+        BpView v = bp_view(p, BZ, vol + b0 * vs, vs, ly + b0 * sino, (uint32_t)(sino * 4), (int)nbat,
+                           ep_from(ep, b0));
+        const int n_blocks = (int)(bricks * nbat);
+        int shift;
+        const unsigned grid = xcd_grid(n_blocks, &shift);
+        if (BZ == 32)
+            hipLaunchKernelGGL(bp_separable<32>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
+        else
+            hipLaunchKernelGGL(bp_separable<16>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
+        TSX_HIP_CHECK(hipGetLastError());
+    }
     return TSX_OK;
 }
 
@@ -718,25 +829,8 @@ int launch_general(const tsx_plan *p, bool forward, T *vol, int64_t vs, T *proj,
                    int64_t batch, Epilogue ep, hipStream_t stream)
 {
     if (batch <= 0) return TSX_OK;
-    if (forward) {
-        if (p->n_angles > 65535 || batch > 65535)
-            return fail(TSX_ERR_UNSUPPORTED, "more than 65535 angles or batch elements per launch");
-        const int tiles_u = (p->cols + FP_BX - 1) / FP_BX;
-        const int tiles_v = (p->rows + FP_BY - 1) / FP_BY;
-        dim3 grid((unsigned)(tiles_u * tiles_v), (unsigned)p->n_angles, (unsigned)batch);
-        hipLaunchKernelGGL(fp_general<T>, grid, dim3(FP_BX, FP_BY), 0, stream, p->dev, p->n_angles,
-                           p->rows, p->cols, p->ss, tiles_u, (const T *)vol, vs, proj, ps, ep);
-    } else {
-        if ((int64_t)p->nz * batch > 65535)
-            return fail(TSX_ERR_UNSUPPORTED, "volume depth times batch exceeds 65535 per launch");
-        dim3 grid((unsigned)((p->nx + BP_BX - 1) / BP_BX), (unsigned)((p->ny + BP_BY - 1) / BP_BY),
-                  (unsigned)(p->nz * batch));
-        hipLaunchKernelGGL(bp_general<T>, grid, dim3(BP_BX, BP_BY), 0, stream, p->dev, p->n_angles,
-                           p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol, vs, (const T *)proj,
-                           ps, ep);
-    }
-    TSX_HIP_CHECK(hipGetLastError());
-    return TSX_OK;
+    if (forward) return launch_fp_general<T>(p, (const T *)vol, vs, proj, ps, batch, ep, stream);
+    return launch_bp_general<T, false>(p, vol, vs, (const T *)proj, ps, batch, ep, stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -765,8 +859,13 @@ int64_t multi_workspace_bytes(const tsx_plan *const *plans, int n, int op)
     return total;
 }
 
-int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *const *projs, int n,
-             int additive, void *wsp, int64_t ws_bytes, hipStream_t stream)
+// Fused launches: consecutive instances on the fast kernels with the same
+// kernel variant share one launch (up to MAX_FUSED instances, as many blocks
+// as the grid limit takes); an instance on the general kernels, with
+// detector supersampling, or with more blocks than one launch takes, gets its
+// own launch(es). eps[k] is the epilogue of instance k.
+int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *const *projs,
+             const Epilogue *eps, int n, void *wsp, int64_t ws_bytes, hipStream_t stream)
 {
     const int64_t need = multi_workspace_bytes(plans, n, TSX_OP_FP);
     Scratch owned;
@@ -775,13 +874,12 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         int rc = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
         if (rc) return rc;
     }
-    const Epilogue ep = plain(additive);
     const bool fast = !force_general();
+    const int64_t cap = max_blocks(FP_BX * FP_BY);
     FpMulti m;
     m.n = 0;
     m.begin[0] = 0;
     int64_t woff = 0;
-    // one fused launch per run of instances with the same kernel variant
     int variant = -1;
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
@@ -799,15 +897,17 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
     for (int k = 0; k < n; ++k) {
         const tsx_plan *p = plans[k];
         if (!vols[k] || !projs[k]) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
-        if (!(fast && p->fp_padded && p->ss == 1)) {
-            // this instance takes its own launch (general kernel or supersampling)
-            int rc = launch_fp(p, vols[k], 0, projs[k], 0, 1, ep, stream,
-                               wptr ? (char *)wptr + woff : nullptr, align_up(workspace_bytes(p, TSX_OP_FP, 1)));
+        const int64_t wb = align_up(workspace_bytes(p, TSX_OP_FP, 1));
+        const int tiles_u = (p->cols + FP_BX - 1) / FP_BX, tiles_v = (p->rows + FP_BY - 1) / FP_BY;
+        const int64_t blocks = (int64_t)tiles_u * tiles_v * p->n_angles;
+        if (!(fast && p->fp_padded && p->ss == 1) || blocks > cap) {
+            // this instance takes its own launch(es)
+            int rc = launch_fp(p, vols[k], 0, projs[k], 0, 1, eps[k], stream, wptr ? (char *)wptr + woff : nullptr, wb);
             if (rc) return rc;
-            woff += align_up(workspace_bytes(p, TSX_OP_FP, 1));
+            woff += wb;
             continue;
         }
-        if (m.n > 0 && fp_variant_of(p) != variant) {
+        if (m.n > 0 && (fp_variant_of(p) != variant || (int64_t)m.begin[m.n] + blocks > cap)) {
             int rc = flush();
             if (rc) return rc;
         }
@@ -815,15 +915,17 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         int64_t offs[3], per_entry;
         layout_offsets(p, offs, &per_entry);
         float *base = (float *)((char *)wptr + woff);
-        woff += align_up(per_entry * 4);
+        woff += wb;
         build_layouts(p, vols[k], base, offs, stream);
         FpView &v = m.v[m.n];
         v.angles = p->dev;
         v.n_angles = p->n_angles;
+        v.a0 = 0;
+        v.na = p->n_angles;
         v.rows = p->rows;
         v.cols = p->cols;
-        v.tiles_u = (p->cols + FP_BX - 1) / FP_BX;
-        v.tiles_v = (p->rows + FP_BY - 1) / FP_BY;
+        v.tiles_u = tiles_u;
+        v.tiles_v = tiles_v;
         v.batch = 1;
         for (int c = 0; c < 3; ++c) {
             v.L.ptr[c] = p->axis_count[c] > 0 ? base + offs[c] : nullptr;
@@ -832,13 +934,7 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         v.L.stride = 0;
         v.proj = projs[k];
         v.proj_stride = 0;
-        v.ep = ep;
-        const int64_t blocks = (int64_t)v.tiles_u * v.tiles_v * v.n_angles;
-        if ((int64_t)m.begin[m.n] + blocks > MAX_LOGICAL) {
-            int rc = flush();
-            if (rc) return rc;
-            m.v[0] = v;
-        }
+        v.ep = eps[k];
         m.begin[m.n + 1] = m.begin[m.n] + (int)blocks;
         m.n += 1;
         if (m.n == MAX_FUSED) {
@@ -850,8 +946,8 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
     return flush();
 }
 
-int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *const *projs, int n,
-             int additive, void *wsp, int64_t ws_bytes, hipStream_t stream)
+int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *const *projs,
+             const Epilogue *eps, int n, void *wsp, int64_t ws_bytes, hipStream_t stream)
 {
     const int64_t need = multi_workspace_bytes(plans, n, TSX_OP_BP);
     Scratch owned;
@@ -860,9 +956,8 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         int rc = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
         if (rc) return rc;
     }
-    const Epilogue ep = plain(additive);
     const bool fast = !force_general();
-    // one fused launch per run of instances with the same brick depth
+    const int64_t cap = max_blocks(BR * BR);
     int BZ = 32;
     BpMulti m;
     m.n = 0;
@@ -883,31 +978,24 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         const tsx_plan *p = plans[k];
         if (!vols[k] || !projs[k]) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
         const int64_t wb = align_up(workspace_bytes(p, TSX_OP_BP, 1));
-        if (!(fast && p->bp_separable)) {
-            int rc = launch_bp(p, vols[k], 0, projs[k], 0, 1, ep, stream,
-                               wptr ? (char *)wptr + woff : nullptr, wb);
+        const int bz = bp_brick_z_of(p);
+        const int64_t blocks = (int64_t)((p->nx + BR - 1) / BR) * ((p->ny + BR - 1) / BR) * ((p->nz + bz - 1) / bz);
+        if (!(fast && p->bp_separable) || blocks > cap) {
+            int rc = launch_bp(p, vols[k], 0, projs[k], 0, 1, eps[k], stream, wptr ? (char *)wptr + woff : nullptr, wb);
             if (rc) return rc;
             woff += wb;
             continue;
         }
-        if (m.n > 0 && bp_brick_z_of(p) != BZ) {
+        if (m.n > 0 && (bz != BZ || (int64_t)m.begin[m.n] + blocks > cap)) {
             int rc = flush();
             if (rc) return rc;
         }
-        BZ = bp_brick_z_of(p);
+        BZ = bz;
         float *ly = (float *)((char *)wptr + woff);
         woff += wb;
         const int64_t sino = bp_workspace_elems(p);
-        dim3 sgrid((unsigned)((p->cols + 255) / 256), (unsigned)p->n_angles, (unsigned)p->rows);
-        hipLaunchKernelGGL(scale_by_length, sgrid, dim3(256), 0, stream, p->dev, p->n_angles, p->rows,
-                           p->cols, projs[k], (int64_t)0, ly, sino);
-        BpView v = bp_view(p, BZ, vols[k], 0, ly, (uint32_t)(sino * 4), 1, ep);
-        const int64_t blocks = (int64_t)v.bricks_x * v.bricks_y * v.bricks_z;
-        if ((int64_t)m.begin[m.n] + blocks > MAX_LOGICAL) {
-            int rc = flush();
-            if (rc) return rc;
-        }
-        m.v[m.n] = v;
+        launch_scale_by_length(p, projs[k], 0, ly, sino, 1, stream);
+        m.v[m.n] = bp_view(p, BZ, vols[k], 0, ly, (uint32_t)(sino * 4), 1, eps[k]);
         m.begin[m.n + 1] = m.begin[m.n] + (int)blocks;
         m.n += 1;
         if (m.n == MAX_FUSED) {
@@ -1276,6 +1364,7 @@ int tsx_fdk(const tsx_plan *p, float *vol, const float *proj, int additive, void
                                              " detector columns");
     if (p->rows > 65535 || p->n_angles > 65535 || p->nz > 65535)
         return fail(TSX_ERR_UNSUPPORTED, "FDK launch grid limit (65535 rows, angles or slices)");
+    TSX_ENTRY("tsx_fdk");
     DeviceGuard dg(p->device);
     if (!dg.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
     hipStream_t s = (hipStream_t)stream;
@@ -1314,6 +1403,7 @@ int tsx_plan_autotune(tsx_plan *p, void *stream)
 {
     int rc = check_plan(p);
     if (rc) return rc;
+    TraceRange trace("tsx_plan_autotune");
     DeviceGuard dg(p->device);
     if (!dg.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
     return autotune(p, (hipStream_t)stream);
@@ -1342,6 +1432,7 @@ int tsx_forward_batched(const tsx_plan *p, const float *vol, int64_t vs, float *
     int rc = check_plan(p);
     if (rc) return rc;
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+    TSX_ENTRY("tsx_forward_batched");
     DeviceGuard guard(p->device);
     return launch_fp(p, vol, vs, proj, ps, batch, plain(additive), (hipStream_t)stream, workspace,
                      workspace_bytes);
@@ -1354,6 +1445,7 @@ int tsx_backward_batched(const tsx_plan *p, float *vol, int64_t vs, const float 
     int rc = check_plan(p);
     if (rc) return rc;
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+    TSX_ENTRY("tsx_backward_batched");
     DeviceGuard guard(p->device);
     return launch_bp(p, vol, vs, proj, ps, batch, plain(additive), (hipStream_t)stream, workspace,
                      workspace_bytes);
@@ -1365,6 +1457,7 @@ int tsx_forward_batched_f64(const tsx_plan *p, const double *vol, int64_t vs, do
     int rc = check_plan(p);
     if (rc) return rc;
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+    TSX_ENTRY("tsx_forward_batched_f64");
     DeviceGuard guard(p->device);
     return launch_general<double>(p, true, (double *)vol, vs, proj, ps, batch, plain(additive),
                                   (hipStream_t)stream);
@@ -1376,6 +1469,7 @@ int tsx_backward_batched_f64(const tsx_plan *p, double *vol, int64_t vs, const d
     int rc = check_plan(p);
     if (rc) return rc;
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+    TSX_ENTRY("tsx_backward_batched_f64");
     DeviceGuard guard(p->device);
     return launch_general<double>(p, false, vol, vs, (double *)proj, ps, batch, plain(additive),
                                   (hipStream_t)stream);
@@ -1394,6 +1488,7 @@ int tsx_forward_residual(const tsx_plan *p, const float *vol, int64_t vs, const 
     e.y = y;
     e.w = w;
     e.stride = ps;
+    TSX_ENTRY("tsx_forward_residual");
     DeviceGuard guard(p->device);
     return launch_fp(p, vol, vs, proj, ps, batch, e, (hipStream_t)stream, workspace,
                      workspace_bytes);
@@ -1411,6 +1506,7 @@ int tsx_backward_scaled_add(const tsx_plan *p, float *vol, int64_t vs, const flo
     e.y = nullptr;
     e.w = w;
     e.stride = vs;
+    TSX_ENTRY("tsx_backward_scaled_add");
     DeviceGuard guard(p->device);
     return launch_bp(p, vol, vs, proj, ps, batch, e, (hipStream_t)stream, workspace,
                      workspace_bytes);
@@ -1427,26 +1523,102 @@ int tsx_multi_workspace_size(const tsx_plan *const *plans, int n, int op, int64_
     return TSX_OK;
 }
 
-int tsx_multi_forward(const tsx_plan *const *plans, const float *const *vols, float *const *projs,
-                      int n, int additive, void *workspace, int64_t workspace_bytes, void *stream)
+}  // extern "C"
+
+namespace {
+
+// Per-instance epilogues of a fused call: SET / ADD for all when `eps` is
+// NULL, else as given (checked).
+int multi_epilogues(const tsx_epilogue *eps, int n, int additive, int op, std::vector<Epilogue> &out)
+{
+    out.resize(n);
+    for (int k = 0; k < n; ++k) {
+        Epilogue &e = out[k];
+        e.y = nullptr;
+        e.w = nullptr;
+        e.stride = 0;
+        if (!eps) {
+            e.mode = additive ? EP_ADD : EP_SET;
+            continue;
+        }
+        const int mode = eps[k].mode;
+        if (mode == TSX_EP_SET || mode == TSX_EP_ADD) {
+            e.mode = mode == TSX_EP_SET ? EP_SET : EP_ADD;
+        } else if (mode == TSX_EP_RESIDUAL && op == TSX_OP_FP) {
+            if (!eps[k].y || !eps[k].w)
+                return fail(TSX_ERR_INVALID_ARGUMENT, "instance " + std::to_string(k) +
+                                                          ": the residual epilogue needs y and w");
+            e.mode = EP_RESIDUAL;
+            e.y = eps[k].y;
+            e.w = eps[k].w;
+        } else if (mode == TSX_EP_SCALED_ADD) {
+            if (!eps[k].w)
+                return fail(TSX_ERR_INVALID_ARGUMENT, "instance " + std::to_string(k) +
+                                                          ": the scaled-add epilogue needs w");
+            e.mode = EP_SCALED_ADD;
+            e.w = eps[k].w;
+        } else {
+            return fail(TSX_ERR_INVALID_ARGUMENT, "instance " + std::to_string(k) + ": epilogue mode " +
+                                                      std::to_string(mode) + " is not valid for this call");
+        }
+    }
+    return TSX_OK;
+}
+
+int multi_entry(const tsx_plan *const *plans, const void *const *vols, const void *const *projs,
+                const tsx_epilogue *eps, int n, int additive, int op, void *workspace,
+                int64_t workspace_bytes, void *stream)
 {
     int rc = check_multi(plans, n);
     if (rc) return rc;
     if (n == 0) return TSX_OK;
     if (!vols || !projs) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointer arrays must not be NULL");
+    std::vector<Epilogue> ep;
+    rc = multi_epilogues(eps, n, additive, op, ep);
+    if (rc) return rc;
+    TSX_ENTRY(op == TSX_OP_FP ? "tsx_multi_forward" : "tsx_multi_backward");
     DeviceGuard guard(plans[0]->device);
-    return multi_fp(plans, vols, projs, n, additive, workspace, workspace_bytes, (hipStream_t)stream);
+    if (op == TSX_OP_FP)
+        return multi_fp(plans, (const float *const *)vols, (float *const *)projs, ep.data(), n, workspace,
+                        workspace_bytes, (hipStream_t)stream);
+    return multi_bp(plans, (float *const *)vols, (const float *const *)projs, ep.data(), n, workspace,
+                    workspace_bytes, (hipStream_t)stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsx_multi_forward(const tsx_plan *const *plans, const float *const *vols, float *const *projs,
+                      int n, int additive, void *workspace, int64_t workspace_bytes, void *stream)
+{
+    return multi_entry(plans, (const void *const *)vols, (const void *const *)projs, nullptr, n, additive,
+                       TSX_OP_FP, workspace, workspace_bytes, stream);
 }
 
 int tsx_multi_backward(const tsx_plan *const *plans, float *const *vols, const float *const *projs,
                        int n, int additive, void *workspace, int64_t workspace_bytes, void *stream)
 {
-    int rc = check_multi(plans, n);
-    if (rc) return rc;
-    if (n == 0) return TSX_OK;
-    if (!vols || !projs) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointer arrays must not be NULL");
-    DeviceGuard guard(plans[0]->device);
-    return multi_bp(plans, vols, projs, n, additive, workspace, workspace_bytes, (hipStream_t)stream);
+    return multi_entry(plans, (const void *const *)vols, (const void *const *)projs, nullptr, n, additive,
+                       TSX_OP_BP, workspace, workspace_bytes, stream);
+}
+
+int tsx_multi_forward_ep(const tsx_plan *const *plans, const float *const *vols, float *const *projs,
+                         const tsx_epilogue *eps, int n, void *workspace, int64_t workspace_bytes,
+                         void *stream)
+{
+    if (!eps && n > 0) return fail(TSX_ERR_INVALID_ARGUMENT, "epilogues must not be NULL");
+    return multi_entry(plans, (const void *const *)vols, (const void *const *)projs, eps, n, 0, TSX_OP_FP,
+                       workspace, workspace_bytes, stream);
+}
+
+int tsx_multi_backward_ep(const tsx_plan *const *plans, float *const *vols, const float *const *projs,
+                          const tsx_epilogue *eps, int n, void *workspace, int64_t workspace_bytes,
+                          void *stream)
+{
+    if (!eps && n > 0) return fail(TSX_ERR_INVALID_ARGUMENT, "epilogues must not be NULL");
+    return multi_entry(plans, (const void *const *)vols, (const void *const *)projs, eps, n, 0, TSX_OP_BP,
+                       workspace, workspace_bytes, stream);
 }
 
 int tsx_forward(const tsx_plan *p, const float *vol, float *proj, int additive, void *stream)

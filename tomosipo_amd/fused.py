@@ -6,13 +6,40 @@
 results are bit-identical to the per-operator calls. All tensors must be
 float32, contiguous and on the same HIP device. There is no counterpart in
 tomosipo, which applies one operator per call (tomosipo/Operator.py:176-233).
+
+Each instance may carry its own epilogue (``Epilogue``): overwrite,
+accumulate, or one half of a SIRT iteration (``residual``: out = w * (y - A x);
+``scaled_add``: out += w * A.T(y)), so that plain projections and SIRT
+iterations of different requests share a launch.
 """
+import dataclasses
+
 import torch
 
 from . import _tsx
 
 
+@dataclasses.dataclass
+class Epilogue:
+    """How an instance's projection r is written to its output o.
+
+    mode: "set" (o = r), "add" (o += r), "residual" (o = w * (y - r); forward
+    only) or "scaled_add" (o += w * r). ``y`` and ``w`` have the output's shape.
+    """
+    mode: str = "set"
+    y: object = None
+    w: object = None
+
+
+_MODES = {"set": _tsx.TSX_EP_SET, "add": _tsx.TSX_EP_ADD, "residual": _tsx.TSX_EP_RESIDUAL,
+          "scaled_add": _tsx.TSX_EP_SCALED_ADD}
+
+
 def _device_plans(ops, tensors):
+    for k, A in enumerate(ops):
+        if A.projector.voxel_supersampling != 1:
+            raise ValueError(f"operator {k}: fused launches take voxel_supersampling = 1 only "
+                             "(apply operators with voxel supersampling one at a time)")
     dev = tensors[0].device
     for t in tensors:
         if not (isinstance(t, torch.Tensor) and t.is_cuda and t.device == dev):
@@ -23,21 +50,58 @@ def _device_plans(ops, tensors):
     return dev, [A.projector.plan(index) for A in ops]
 
 
-def _run(ops, inputs, out, forward, additive):
+def _check_shapes(tensors, shapes, what):
+    for k, (t, shape) in enumerate(zip(tensors, shapes)):
+        if not isinstance(t, torch.Tensor):
+            raise ValueError(f"{what} {k} must be a torch tensor, got {type(t)}")
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"Expected {what} {k} of shape {tuple(shape)}. Got {tuple(t.shape)}")
+
+
+def _run(ops, inputs, out, forward, additive, epilogues=None):
     if len(ops) != len(inputs):
         raise ValueError("one input per operator is required")
     if len(ops) == 0:
         return []
-    for A, x in zip(ops, inputs):
-        shape = A.domain_shape if forward else A.range_shape
-        if tuple(x.shape) != tuple(shape):
-            raise ValueError(f"Expected input of shape {tuple(shape)}. Got {tuple(x.shape)}")
+    in_shapes = [A.domain_shape if forward else A.range_shape for A in ops]
+    out_shapes = [A.range_shape if forward else A.domain_shape for A in ops]
+    _check_shapes(inputs, in_shapes, "input")
     if out is None:
-        out = [torch.zeros(A.range_shape if forward else A.domain_shape, device=x.device)
-               if additive else
-               torch.empty(A.range_shape if forward else A.domain_shape, device=x.device)
-               for A, x in zip(ops, inputs)]
-    dev, plans = _device_plans(ops, list(inputs) + list(out))
+        zero = additive or (epilogues is not None and any(e.mode in ("add", "scaled_add")
+                                                          for e in epilogues))
+        out = [torch.zeros(s, device=x.device) if zero else torch.empty(s, device=x.device)
+               for s, x in zip(out_shapes, inputs)]
+    else:
+        out = list(out)
+        if len(out) != len(ops):
+            raise ValueError(f"one output per operator is required: got {len(out)} for {len(ops)}")
+        _check_shapes(out, out_shapes, "output")
+    extra = []
+    eps = None
+    if epilogues is not None:
+        epilogues = list(epilogues)
+        if len(epilogues) != len(ops):
+            raise ValueError("one epilogue per operator is required")
+        eps = []
+        for k, e in enumerate(epilogues):
+            if e.mode not in _MODES:
+                raise ValueError(f"epilogue {k}: unknown mode {e.mode!r}")
+            if e.mode == "residual" and not forward:
+                raise ValueError(f"epilogue {k}: the residual form is for forward projections")
+            need = {"residual": ("y", "w"), "scaled_add": ("w",)}.get(e.mode, ())
+            ptrs = {}
+            for name in ("y", "w"):
+                t = getattr(e, name)
+                if name in need:
+                    if t is None:
+                        raise ValueError(f"epilogue {k}: mode {e.mode!r} needs {name}")
+                    _check_shapes([t], [out_shapes[k]], f"epilogue {k} {name}")
+                    extra.append(t)
+                    ptrs[name] = t.data_ptr()
+                else:
+                    ptrs[name] = None
+            eps.append((_MODES[e.mode], ptrs["y"], ptrs["w"]))
+    dev, plans = _device_plans(ops, list(inputs) + list(out) + extra)
     op = _tsx.TSX_OP_FP if forward else _tsx.TSX_OP_BP
     nbytes = _tsx.multi_workspace_size(plans, op)
     with torch.cuda.device_of(inputs[0]):
@@ -45,13 +109,14 @@ def _run(ops, inputs, out, forward, additive):
         stream = torch.cuda.current_stream(dev).cuda_stream
         vols = [x.data_ptr() for x in (inputs if forward else out)]
         projs = [y.data_ptr() for y in (out if forward else inputs)]
-        _tsx.multi_project(plans, vols, projs, forward, additive, stream, ws.data_ptr(), nbytes)
+        _tsx.multi_project(plans, vols, projs, forward, additive, stream, ws.data_ptr(), nbytes,
+                           epilogues=eps)
     return out
 
 
-def fused_forward(operators, volumes, out=None, additive=False):
-    return _run(operators, volumes, out, True, additive)
+def fused_forward(operators, volumes, out=None, additive=False, epilogues=None):
+    return _run(operators, volumes, out, True, additive, epilogues)
 
 
-def fused_backward(operators, projections, out=None, additive=False):
-    return _run(operators, projections, out, False, additive)
+def fused_backward(operators, projections, out=None, additive=False, epilogues=None):
+    return _run(operators, projections, out, False, additive, epilogues)

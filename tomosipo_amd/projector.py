@@ -11,6 +11,15 @@ Replaces tomosipo's ASTRA bridge (``tomosipo/astra.py:80-215``):
   links must be compatible (same device) or ``ValueError``; SET overwrites,
   ADD accumulates.
 
+Voxel supersampling k (``astra.py:94-97``) is defined here so that the
+backprojection stays the exact transpose of the forward projection: the
+volume is refined k times along each axis (each voxel becomes k^3 voxels of
+its value, operator U) and the refined grid is projected, A_k = A_fine U;
+the backprojection is A_k^T = U^T A_fine^T (the refined backprojection summed
+over each voxel's k^3 sub-voxels). ASTRA applies VoxelSuperSampling to its
+voxel-driven backprojector only; A_k x differs from A x only by the finer
+sampling of the same piecewise-constant volume.
+
 Torch tensors on a HIP device go to the kernels by pointer on the current
 stream of their device (asynchronous, like any torch op). numpy arrays and
 CPU tensors are copied to device 0, projected and copied back, which is what
@@ -42,6 +51,13 @@ class Projector:
         self.proj_geom = projection_geometry.to_vec().to_astra()
         self.voxel_supersampling = int(voxel_supersampling)
         self.detector_supersampling = int(detector_supersampling)
+        if self.voxel_supersampling < 1:
+            raise ValueError(f"voxel_supersampling must be >= 1; got {voxel_supersampling}")
+        k = self.voxel_supersampling
+        # the kernels project the k-times refined grid of the same window
+        self.plan_vol_geom = dict(self.vol_geom)
+        for key in ("GridColCount", "GridRowCount", "GridSliceCount"):
+            self.plan_vol_geom[key] = int(self.vol_geom[key]) * k
         self.vol_shape = tuple(volume_geometry.shape)
         rows, cols = projection_geometry.det_shape
         self.proj_shape = (rows, projection_geometry.num_angles, cols)
@@ -55,9 +71,8 @@ class Projector:
         self._lock = threading.Lock()
 
     def _make_plan(self, device):
-        return _tsx.Plan(self.vol_geom, self.proj_geom, device=device,
-                         det_supersampling=self.detector_supersampling,
-                         vox_supersampling=self.voxel_supersampling)
+        return _tsx.Plan(self.plan_vol_geom, self.proj_geom, device=device,
+                         det_supersampling=self.detector_supersampling, vox_supersampling=1)
 
     def plan(self, device):
         """The device plan for ``device`` (created on first use)."""
@@ -97,7 +112,52 @@ def _require_device(torch):
                            "available and there is no CPU fallback")
 
 
+def _refine(x, k):
+    """U: each voxel of the trailing three dims becomes k^3 voxels of its value."""
+    for d in (-3, -2, -1):
+        x = x.repeat_interleave(k, dim=d)
+    return x.contiguous()
+
+
+def _coarsen(x, k, shape):
+    """U^T: the sum over each voxel's k^3 sub-voxels."""
+    nz, ny, nx = shape
+    lead = tuple(x.shape[:-3])
+    return x.reshape(lead + (nz, k, ny, k, nx, k)).sum(dim=(-5, -3, -1))
+
+
 def _run(projector, vol_t, proj_t, forward, additive, batch=1, vol_stride=0, proj_stride=0):
+    """Launch on device tensors (contiguous, same device; float32, or float64 for the
+    FP64 kernels)."""
+    k = projector.voxel_supersampling
+    if k > 1:
+        # through the refined grid: A_k = A_fine U, A_k^T = U^T A_fine^T
+        torch = _torch()
+        shape = projector.vol_shape
+        lead = (batch,) if batch > 1 else ()
+        fine_shape = tuple(n * k for n in shape)
+        fine_numel = int(np.prod(fine_shape))
+        vol_view = vol_t.reshape(-1)[:batch * int(np.prod(shape))].reshape(lead + shape) \
+            if vol_stride in (0, int(np.prod(shape))) else None
+        if vol_view is None:
+            raise ValueError("voxel supersampling needs densely packed batch entries")
+        if forward:
+            fine = _refine(vol_view, k)
+            _run_plan(projector, fine, proj_t, True, additive, batch, fine_numel, proj_stride)
+        else:
+            fine = torch.empty(lead + fine_shape, dtype=vol_t.dtype, device=vol_t.device)
+            _run_plan(projector, fine, proj_t, False, False, batch, fine_numel, proj_stride)
+            coarse = _coarsen(fine, k, shape)
+            with torch.no_grad():
+                if additive:
+                    vol_view += coarse
+                else:
+                    vol_view.copy_(coarse)
+        return
+    _run_plan(projector, vol_t, proj_t, forward, additive, batch, vol_stride, proj_stride)
+
+
+def _run_plan(projector, vol_t, proj_t, forward, additive, batch, vol_stride, proj_stride):
     """Launch on device tensors (contiguous, same device; float32, or float64 for the
     FP64 kernels)."""
     torch = _torch()
@@ -131,7 +191,9 @@ def direct_project(projector, vol_link, proj_link, forward=None, additive=False)
             "compatible. Usually, this indicates that the data are located on different computing "
             "devices. ")
     torch = _torch()
-    vol, proj = vol_link.data, proj_link.data
+    # a link over another library's array (DLPack) is read through its torch view
+    vol = getattr(vol_link, "tensor", vol_link.data)
+    proj = getattr(proj_link, "tensor", proj_link.data)
     on_device = isinstance(vol, torch.Tensor) and vol.is_cuda
     with vol_link.context():
         if on_device:
@@ -162,6 +224,8 @@ def direct_bp(projector, vol_link, proj_link, additive=False):
 
 def _run_fdk(projector, vol_t, proj_t, additive):
     torch = _torch()
+    if projector.voxel_supersampling != 1:
+        raise ValueError("FDK supports voxel_supersampling = 1 only")
     if vol_t.dtype != torch.float32 or proj_t.dtype != torch.float32:
         raise TypeError("FDK takes float32 volume and projections")
     dev = vol_t.device.index if vol_t.device.index is not None else torch.cuda.current_device()
@@ -182,7 +246,8 @@ def direct_fdk(projector, vol_link, proj_link, additive=True):
             "compatible. Usually, this indicates that the data are located on different computing "
             "devices. ")
     torch = _torch()
-    vol, proj = vol_link.data, proj_link.data
+    vol = getattr(vol_link, "tensor", vol_link.data)
+    proj = getattr(proj_link, "tensor", proj_link.data)
     with vol_link.context():
         if isinstance(vol, torch.Tensor) and vol.is_cuda:
             _run_fdk(projector, vol, proj, additive)

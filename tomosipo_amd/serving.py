@@ -6,14 +6,26 @@ server admits a request as soon as it has arrived and its device memory fits
 the HBM budget, keeps its volume and projections resident, and advances every
 admitted request by one projection per tick: all requests whose next step is
 a forward projection go into one fused launch, all whose next step is a
-backprojection into another (``tomosipo_amd.fused``). Requests that arrive
-while others are on the device join at the next tick, so their work overlaps
-on the device. Results are bit-identical to running each request alone,
-because the fused kernels are deterministic and compute exactly what the
-per-operator kernels compute.
+backprojection into another (``tomosipo_amd.fused``). A SIRT step rides in
+those launches with its elementwise work fused into the kernels' epilogues
+(forward: ``r = R * (y - A x)``; backprojection: ``x += C * A.T(r)``), and the
+SIRT weights R = 1 / A(1), C = 1 / A.T(1) are computed once per geometry and
+shared by every request of that geometry. Requests that arrive while others
+are on the device join at the next tick, so their work overlaps on the
+device. Results are bit-identical to running each request alone, because the
+fused kernels are deterministic and compute exactly what the per-operator
+kernels compute.
+
+Failures stay with the request that caused them: a request whose set-up
+fails is reported failed at admission; if a fused launch fails, its requests
+are retried one launch each, and those whose own launch fails again are
+reported failed (``records[rid]["error"]``) and release their memory, while
+the others go on.
 
 ``serve_one_at_a_time`` is the reference point: the same requests in arrival
-order, one operator call per launch, one request at a time.
+order, one request at a time, each processed as tomosipo's own SIRT loop does
+it (``notebooks/sirt_benchmark.py:116-139``): its own R and C, one operator
+call per projection, elementwise torch operations in between.
 
 The reference has no server; its operators are applied one synchronous call
 at a time (tomosipo/Operator.py:176-233, tomosipo/astra.py:147-153).
@@ -72,7 +84,8 @@ def request_geometry(req):
 
 
 def request_bytes(req):
-    """Device bytes a request holds while admitted (data, SIRT state, workspaces)."""
+    """Device bytes a request holds while admitted (data, SIRT state, workspaces).
+    The SIRT weights of its geometry (cached, shared) are counted too."""
     s = req.side
     vol = 4 * s ** 3
     sino = 4 * s * req.n_angles * (3 * s // 2)
@@ -81,7 +94,7 @@ def request_bytes(req):
         return vol + sino + pad
     if req.op == "bp":
         return vol + 2 * sino
-    return 4 * vol + 4 * sino + pad      # x, tmp_x, C, ones | y, tmp_y, R, l*y
+    return 2 * vol + 3 * sino + pad + (vol + sino)  # x, ones | y, r, l*r | C, R
 
 
 def _input_data(req, torch, dev):
@@ -94,10 +107,23 @@ def _input_data(req, torch, dev):
     return x
 
 
-class _Job:
-    """Device state of an admitted request and its remaining steps."""
+def sirt_weights(A, torch, dev):
+    """R = 1 / A(1) and C = 1 / A.T(1), clipped at 1 / epsilon (the SIRT weights
+    of notebooks/sirt_benchmark.py:116-139)."""
+    R = A(torch.ones(A.domain_shape, device=dev))
+    R = torch.clamp(1.0 / R, max=1.0 / epsilon)
+    C = A.T(torch.ones(A.range_shape, device=dev))
+    C = torch.clamp(1.0 / C, max=1.0 / epsilon)
+    return R, C
 
-    def __init__(self, req, A, torch, dev):
+
+class _Job:
+    """Device state of an admitted request and its remaining steps.
+
+    ``weights``: a function returning the (R, C) of the request's geometry, or
+    None to compute them for this request (the one-at-a-time reference)."""
+
+    def __init__(self, req, A, torch, dev, weights=None):
         self.req = req
         self.A = A
         self.steps = collections.deque()
@@ -110,38 +136,38 @@ class _Job:
             self.steps.append("bp")
         else:
             # SIRT on the projections of the request's phantom
-            y = A(data)
-            R = A(torch.ones(A.domain_shape, device=dev))
-            self.R = torch.clamp(1.0 / R, max=1.0 / epsilon)
-            C = A.T(torch.ones(A.range_shape, device=dev))
-            self.C = torch.clamp(1.0 / C, max=1.0 / epsilon)
-            self.y = y
+            self.y = A(data)
+            self.R, self.C = weights() if weights is not None else sirt_weights(A, torch, dev)
             self.x = torch.zeros(A.domain_shape, device=dev)
             self.tmp_y = torch.empty(A.range_shape, device=dev)
-            self.tmp_x = torch.empty(A.domain_shape, device=dev)
+            self.tmp_x = None
             for _ in range(req.iterations):
                 self.steps.append("fp")
                 self.steps.append("bp")
         self.done_event = None
 
-    # what the next fused launch of each kind reads and writes
+    # what the next fused launch of each kind reads and writes, and how
     def fp_io(self):
-        return (self.inp, self.out) if self.req.op == "fp" else (self.x, self.tmp_y)
+        from .fused import Epilogue
+
+        if self.req.op == "fp":
+            return self.inp, self.out, Epilogue("set")
+        return self.x, self.tmp_y, Epilogue("residual", y=self.y, w=self.R)
 
     def bp_io(self):
-        return (self.inp, self.out) if self.req.op == "bp" else (self.tmp_y, self.tmp_x)
+        from .fused import Epilogue
 
-    def after_fp(self, torch):
-        if self.req.op == "sirt":
-            torch.sub(self.y, self.tmp_y, out=self.tmp_y)
-            self.tmp_y.mul_(self.R)
-
-    def after_bp(self, torch):
-        if self.req.op == "sirt":
-            self.x.addcmul_(self.C, self.tmp_x)
+        if self.req.op == "bp":
+            return self.inp, self.out, Epilogue("set")
+        return self.tmp_y, self.x, Epilogue("scaled_add", w=self.C)
 
     def result(self):
         return self.out if self.req.op in ("fp", "bp") else self.x
+
+    def release(self):
+        for name in ("inp", "out", "y", "R", "C", "x", "tmp_y", "tmp_x"):
+            if hasattr(self, name):
+                setattr(self, name, None)
 
 
 class Server:
@@ -156,6 +182,7 @@ class Server:
         self.max_active = max_active
         self.keep_results = keep_results
         self._ops = {}
+        self._weights = {}
 
     def _operator(self, req):
         from .Operator import operator
@@ -168,10 +195,44 @@ class Server:
             self._ops[key] = A
         return A
 
-    def run(self, requests):
-        """Serve ``requests`` in real time; returns per-request records and a summary."""
+    def _sirt_weights(self, req, A):
+        key = (req.kind, req.side, req.n_angles)
+        w = self._weights.get(key)
+        if w is None:
+            w = sirt_weights(A, self.torch, self.dev)
+            self._weights[key] = w
+        return w
+
+    def _launch(self, jobs, forward, records, failed_jobs):
+        """One fused launch over ``jobs``; on failure one launch per job, and the
+        jobs whose own launch fails are moved to ``failed_jobs``."""
+        from ._tsx import TsxError
         from .fused import fused_forward, fused_backward
 
+        fn = fused_forward if forward else fused_backward
+
+        def run(js):
+            io = [j.fp_io() if forward else j.bp_io() for j in js]
+            fn([j.A for j in js], [a for a, _, _ in io], out=[b for _, b, _ in io],
+               epilogues=[e for _, _, e in io])
+
+        try:
+            run(jobs)
+            return list(jobs)
+        except (TsxError, RuntimeError, ValueError):
+            pass
+        ok = []
+        for j in jobs:
+            try:
+                run([j])
+                ok.append(j)
+            except (TsxError, RuntimeError, ValueError) as e:
+                records[j.req.rid]["error"] = str(e)
+                failed_jobs.append(j)
+        return ok
+
+    def run(self, requests):
+        """Serve ``requests`` in real time; returns per-request records and a summary."""
         torch = self.torch
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         for r in pending:
@@ -194,9 +255,16 @@ class Server:
                         deferred += 1
                         break
                     pending.popleft()
-                    used += need
-                    active.append(_Job(r, self._operator(r), torch, self.dev))
                     records[r.rid] = {"arrival": r.arrival, "admitted": now}
+                    try:
+                        A = self._operator(r)
+                        job = _Job(r, A, torch, self.dev, weights=lambda r=r, A=A: self._sirt_weights(r, A))
+                    except Exception as e:  # a request that cannot be set up fails alone
+                        records[r.rid]["error"] = f"{type(e).__name__}: {e}"
+                        records[r.rid]["done"] = time.perf_counter() - t0
+                        continue
+                    used += need
+                    active.append(job)
                 # retire finished jobs whose work has completed on the device
                 still = []
                 for job in finishing:
@@ -206,6 +274,7 @@ class Server:
                         used -= request_bytes(job.req)
                         if self.keep_results:
                             results[job.req.rid] = job.result()
+                        job.release()
                     else:
                         still.append(job)
                 finishing = still
@@ -220,20 +289,24 @@ class Server:
                 while len(ticks) >= 2:
                     ticks.popleft().synchronize()
                 # one tick: one fused forward launch and one fused backprojection launch
+                failed = []
                 fps = [j for j in active if j.steps[0] == "fp"]
                 bps = [j for j in active if j.steps[0] == "bp"]
                 if fps:
-                    io = [j.fp_io() for j in fps]
-                    fused_forward([j.A for j in fps], [a for a, _ in io], out=[b for _, b in io])
-                    for j in fps:
-                        j.after_fp(torch)
+                    for j in self._launch(fps, True, records, failed):
                         j.steps.popleft()
                 if bps:
-                    io = [j.bp_io() for j in bps]
-                    fused_backward([j.A for j in bps], [a for a, _ in io], out=[b for _, b in io])
-                    for j in bps:
-                        j.after_bp(torch)
+                    for j in self._launch(bps, False, records, failed):
                         j.steps.popleft()
+                if failed:
+                    torch.cuda.synchronize()
+                    t = time.perf_counter() - t0
+                    for j in failed:
+                        records[j.req.rid]["done"] = t
+                        used -= request_bytes(j.req)
+                        j.release()
+                    gone = {id(j) for j in failed}
+                    active = [j for j in active if id(j) not in gone]
                 ev = torch.cuda.Event()
                 ev.record()
                 ticks.append(ev)
@@ -269,7 +342,9 @@ def warm_plans(requests, device=0):
 
 
 def serve_one_at_a_time(requests, device=0, keep_results=False):
-    """Reference server: arrival order, one request at a time, one launch per operator call."""
+    """Reference server: arrival order, one request at a time, one launch per
+    operator call, SIRT as tomosipo's own loop (its own R and C; separate
+    elementwise operations)."""
     import torch
 
     from .Operator import operator
@@ -289,16 +364,18 @@ def serve_one_at_a_time(requests, device=0, keep_results=False):
                 ops[key] = operator(vg, pg)
             job = _Job(r, ops[key], torch, dev)
             A = job.A
-            while job.steps:
-                step = job.steps.popleft()
-                if step == "fp":
-                    a, b = job.fp_io()
-                    A(a, out=b)
-                    job.after_fp(torch)
-                else:
-                    a, b = job.bp_io()
-                    A.T(a, out=b)
-                    job.after_bp(torch)
+            if r.op == "fp":
+                A(job.inp, out=job.out)
+            elif r.op == "bp":
+                A.T(job.inp, out=job.out)
+            else:
+                tmp_x = torch.empty(A.domain_shape, device=dev)
+                for _ in range(r.iterations):
+                    A(job.x, out=job.tmp_y)
+                    torch.sub(job.y, job.tmp_y, out=job.tmp_y)
+                    job.tmp_y.mul_(job.R)
+                    A.T(job.tmp_y, out=tmp_x)
+                    job.x.addcmul_(job.C, tmp_x)
             torch.cuda.synchronize()
             records[r.rid] = {"arrival": r.arrival, "admitted": start,
                               "done": time.perf_counter() - t0}
@@ -307,13 +384,24 @@ def serve_one_at_a_time(requests, device=0, keep_results=False):
     return _summary(records, results, 0)
 
 
+def request_work(req):
+    """Voxel-angle updates of a request (FP or BP: N^3 n_angles; SIRT: per
+    iteration one FP and one BP, plus the three projections of its set-up)."""
+    va = req.side ** 3 * req.n_angles
+    if req.op in ("fp", "bp"):
+        return va
+    return va * (2 * req.iterations + 3)
+
+
 def _summary(records, results, deferred):
-    lat = np.array([v["done"] - v["arrival"] for v in records.values()])
+    done = {k: v for k, v in records.items() if "error" not in v}
+    lat = np.array([v["done"] - v["arrival"] for v in done.values()]) if done else np.zeros(1)
     first = min(v["arrival"] for v in records.values())
     last = max(v["done"] for v in records.values())
     return {
-        "completed": len(records),
-        "req_per_s": len(records) / (last - first),
+        "completed": len(done),
+        "failed": len(records) - len(done),
+        "req_per_s": len(done) / (last - first),
         "p50_latency_s": float(np.percentile(lat, 50)),
         "p99_latency_s": float(np.percentile(lat, 99)),
         "makespan_s": last - first,

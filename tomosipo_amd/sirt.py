@@ -24,6 +24,8 @@ class SIRT:
     def __init__(self, A, y, x0=None, *, fused=True):
         if not isinstance(A, Operator):
             raise TypeError("SIRT needs a tomosipo_amd Operator")
+        if A.projector.voxel_supersampling != 1:
+            raise ValueError("the fused SIRT takes voxel_supersampling = 1 only")
         if not (isinstance(y, torch.Tensor) and y.is_cuda):
             raise ValueError("y must be a torch tensor on a HIP device")
         from . import epsilon
