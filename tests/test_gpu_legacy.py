@@ -102,7 +102,7 @@ def test_supersampling_options():
         assert rel_l2(pd.data, oracle.fp(g, x.astype(np.float64))) <= 1e-5
         vd = ts.data(vg, np.zeros((10, 10, 10), dtype=np.float32))
         ts.astra.backward(vd, ts.data(pg, y.copy()), detector_supersampling=dss, voxel_supersampling=vss)
-        fine = ts.volume(shape=10 * vss, size=vg.size, pos=vg.pos)
+        fine = ts.volume(shape=10 * vss, size=vg.size, pos=tuple(vg.pos[0]))
         gf = oracle.oracle_geometry(fine, pg, det_supersampling=dss)
         bf = oracle.bp(gf, y.astype(np.float64))
         ref = bf.reshape(10, vss, 10, vss, 10, vss).sum(axis=(1, 3, 5))
