@@ -255,6 +255,24 @@ int tsx_backward_batched_f64(const tsx_plan *plan, double *vol, int64_t vol_stri
                              const double *proj, int64_t proj_stride, int64_t batch, int additive,
                              void *hip_stream);
 
+/* Bytes of the static device scratch buffer of tsx_direct_fpbp3d_noalloc. */
+#define TSX_NOALLOC_SCRATCH_BYTES (256ll << 20)
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. Same arguments, kernels and results as
+ * tsx_direct_fpbp3d, but the kernels' workspace (tsx_workspace_size(plan,
+ * op, 1) bytes) is taken from a static device buffer of
+ * TSX_NOALLOC_SCRATCH_BYTES that the library holds on every device it runs on
+ * until the process ends; the call allocates and frees no device memory.
+ * Plans whose workspace exceeds that buffer fail with TSX_ERR_UNSUPPORTED
+ * (e.g. a forward projection of a 256^3 volume with two stepping axes).
+ * Every call on a device shares that device's buffer: calls that overlap on
+ * one device (several host threads, or several streams in flight) race, so
+ * the caller serialises them. */
+int tsx_direct_fpbp3d_noalloc(const tsx_plan *plan, float *vol, float *proj, int mode, int op,
+                              void *hip_stream);
+
 /* Message describing the last failure on this thread ("" if none). */
 const char *tsx_last_error(void);
 

@@ -105,6 +105,9 @@ def test_compute_on_host_plan_reports_no_device():
     assert p.info()["tuned"] is False
     lib = _tsx.load()
     assert lib.tsx_direct_fpbp3d(p.handle, None, None, 1, 7, None) != 0
+    # the no-allocation entry point: host plans fail before any device work
+    assert lib.tsx_direct_fpbp3d_noalloc(p.handle, 1, 1, 1, 0, None) == -4
+    assert lib.tsx_direct_fpbp3d_noalloc(None, 1, 1, 1, 0, None) == -1
 
 
 def test_null_arguments():

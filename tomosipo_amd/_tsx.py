@@ -46,8 +46,13 @@ EXPORTED = (
     "tsx_multi_backward", "tsx_multi_forward_ep", "tsx_multi_backward_ep",
     "tsx_forward_residual", "tsx_backward_scaled_add",
     "tsx_forward_batched_f64", "tsx_backward_batched_f64",
+    "tsx_direct_fpbp3d_noalloc",
     "tsx_last_error", "tsx_version",
 )
+
+# Bytes of the static device scratch buffer of tsx_direct_fpbp3d_noalloc
+# (TSX_NOALLOC_SCRATCH_BYTES).
+NOALLOC_SCRATCH_BYTES = 256 << 20
 
 
 class TsxError(RuntimeError):
@@ -135,6 +140,7 @@ def load():
         lib.tsx_backward_scaled_add.argtypes = [P, P, i64, P, i64, P, i64, P, i64, P]
         lib.tsx_forward_batched_f64.argtypes = [P, P, i64, P, i64, i64, i32, P]
         lib.tsx_backward_batched_f64.argtypes = [P, P, i64, P, i64, i64, i32, P]
+        lib.tsx_direct_fpbp3d_noalloc.argtypes = [P, P, P, i32, i32, P]
         for name in EXPORTED:
             if name not in ("tsx_last_error", "tsx_version"):
                 getattr(lib, name).restype = ctypes.c_int

@@ -1642,3 +1642,73 @@ int tsx_direct_fpbp3d(const tsx_plan *p, float *vol, float *proj, int mode, int 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// No-allocation variant of tsx_direct_fpbp3d. The kernels' workspace (padded
+// volume layouts for the forward projector, length-weighted projections for
+// the backprojector) lives in `scratch`, a static device array that every
+// GPU running this library holds until the process ends. Its host-side name
+// is only a stand-in: host code uses the device address that
+// hipGetSymbolAddress returns, once per device with that device current.
+// Every byte of the workspace that a kernel reads is written earlier in the
+// same call (the layout builders write the zero borders too, the weighting
+// pass every projection element), so nothing is cleared between calls.
+// Calls that overlap on one device share the array and race.
+static __device__ __align__(256) unsigned char scratch[TSX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+
+// Device address of `scratch` on `device` (which must be current).
+int scratch_address(int device, void **out)
+{
+    static std::mutex mu;
+    static void *addr[MAX_DEVICES] = {nullptr};
+    if (device < 0 || device >= MAX_DEVICES)
+        return fail(TSX_ERR_UNSUPPORTED, "device index " + std::to_string(device) + " is beyond " +
+                                             std::to_string(MAX_DEVICES - 1));
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(TSX_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        addr[device] = ptr;
+    }
+    *out = addr[device];
+    return TSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsx_direct_fpbp3d_noalloc(const tsx_plan *p, float *vol, float *proj, int mode, int op, void *stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (mode != TSX_MODE_SET && mode != TSX_MODE_ADD)
+        return fail(TSX_ERR_INVALID_ARGUMENT, "mode must be TSX_MODE_SET (1) or TSX_MODE_ADD (0)");
+    if (op != TSX_OP_FP && op != TSX_OP_BP)
+        return fail(TSX_ERR_INVALID_ARGUMENT, "op must be TSX_OP_FP (0) or TSX_OP_BP (1)");
+    if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+    const int64_t need = workspace_bytes(p, op, 1);
+    if (need > TSX_NOALLOC_SCRATCH_BYTES)
+        return fail(TSX_ERR_UNSUPPORTED, "tsx_direct_fpbp3d_noalloc: the plan needs " + std::to_string(need) +
+                                             " bytes of workspace, more than the " +
+                                             std::to_string((int64_t)TSX_NOALLOC_SCRATCH_BYTES) +
+                                             " bytes of the static scratch buffer");
+    TSX_ENTRY("tsx_direct_fpbp3d_noalloc");
+    DeviceGuard guard(p->device);
+    if (!guard.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
+    void *ws = nullptr;
+    rc = scratch_address(p->device, &ws);
+    if (rc) return rc;
+    const Epilogue ep = plain(mode == TSX_MODE_ADD);
+    if (op == TSX_OP_FP)
+        return launch_fp(p, vol, 0, proj, 0, 1, ep, (hipStream_t)stream, ws, TSX_NOALLOC_SCRATCH_BYTES);
+    return launch_bp(p, vol, 0, proj, 0, 1, ep, (hipStream_t)stream, ws, TSX_NOALLOC_SCRATCH_BYTES);
+}
+
+}  // extern "C"
