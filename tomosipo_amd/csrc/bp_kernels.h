@@ -9,11 +9,11 @@
 //   voxel-centre ray) and sums over the pixels in the bounding box. Any
 //   geometry, any detector supersampling.
 //
-// bp_separable: for geometries whose detector v axis is parallel to the
-//   in-slice axis c1 = z and whose stepping axis is x or y (every circular,
-//   helical or jittered-source scan with an upright detector). Then Q_c2 of a
-//   ray depends only on its detector column j, and Q_z is affine in the row i,
-//   so per slice n
+// bp_separable (kernels bp_wave, bp_wave_multi): for geometries whose
+//   detector v axis is parallel to the in-slice axis c1 = z and whose
+//   stepping axis is x or y (every circular, helical or jittered-source scan
+//   with an upright detector). Then Q_c2 of a ray depends only on its
+//   detector column j, and Q_z is affine in the row i, so per slice n
 //     x[n, kz, kc2] = sum_j L(Q_c2(j; n) - kc2) * u[n, j, kz],
 //     u[n, j, kz]   = sum_i L(A(j, n) + i B(j, n) - kz) * ly[i, j],
 //   with ly = l * y precomputed. A block owns a brick of 16 x 16 columns by
@@ -23,9 +23,9 @@
 //   every cell is written without being read back), stores the columns of a
 //   slice in increasing order of Q_c2, and then every thread folds exactly
 //   the columns with |Q_c2 - kc2| < 1 into the BZ voxels of its z-column
-//   (pass 2; the first one found by binary search). The slice ranges of the
-//   next angle are computed during the current one. No atomics: the result
-//   is deterministic.
+//   (pass 2; the first one found by binary search). Each wave owns four
+//   slices in both passes, so the angle loop synchronises waves only with
+//   themselves (see bp_wave_work). No atomics: the result is deterministic.
 #pragma once
 
 #include "tsx_plan.h"
@@ -244,6 +244,7 @@ __device__ __forceinline__ double column_of(const AP &A, double n, double c)
 // One instance of the separable backprojector as seen by a block.
 struct BpView {
     const AngleParams *angles;
+    const AngleParams *sorted;  // the angles ordered by stepping axis, original index in `pad`
     int n_angles, rows, cols, nx, ny, nz, bricks_x, bricks_y, bricks_z, batch;
     float *vol;
     int64_t vol_stride;
@@ -252,20 +253,19 @@ struct BpView {
     Epilogue ep;
 };
 
-// LDS layout of bp_separable: columns are UCOL = BZ + 2 floats apart and
-// slices SLICE floats apart (SLICE = 36 mod 64); the cell kz of the column in
-// slot pos of slice s is at s * SLICE + pos * UCOL + UOFF + kz. Only the cells
-// kz = 0 .. BZ - 1 are read. The splat clamps the lower cell of a row to
-// [-2, BZ], so the weights of rows outside the brick land in cells -2, -1 of
-// their column or in cells BZ, BZ + 1; a column's cells BZ - 2, BZ - 1 share
-// the next column's (unused) cells -4, -3, and its cells BZ, BZ + 1 the next
-// column's cells -2, -1, so no real cell is written by another column. The
-// strides were chosen with a simulation of the kernel's LDS addresses on a
-// cone-beam geometry (64 dword banks): they spread the z-cells that the lanes
-// of a pass-1 wave (16 columns of 4 slices) and of a pass-2 wave (16 slices)
-// touch at the same time over the banks (7 % fewer bank cycles than columns
-// BZ + 4 apart), and every column stays 8-byte aligned so that pass 2 reads
-// its cells as float2.
+// LDS layout of the separable backprojector: columns are UCOL = BZ + 2
+// floats apart and slices SLICE floats apart (SLICE = 36 mod 64); the cell kz
+// of the column in slot pos of slice s is at s * SLICE + pos * UCOL + UOFF +
+// kz. Only the cells kz = 0 .. BZ - 1 are read. The splat clamps the lower
+// cell of a row to [-2, BZ], so the weights of rows outside the brick land in
+// cells -2, -1 of their column or in cells BZ, BZ + 1; a column's cells
+// BZ - 2, BZ - 1 share the next column's (unused) cells -4, -3, and its cells
+// BZ, BZ + 1 the next column's cells -2, -1, so no real cell is written by
+// another column. The strides were chosen with a simulation of the LDS
+// addresses of an earlier, block-synchronous form of the kernel on a
+// cone-beam geometry (64 dword banks; 7 % fewer bank cycles than columns
+// BZ + 4 apart) and kept; every column stays 8-byte aligned so that pass 2
+// reads its cells as float2.
 constexpr int UOFF = 4;  // kz = 0 sits at offset 4 of a column
 
 template <int BZ>
@@ -273,14 +273,12 @@ struct BpShared {
     static constexpr int UCOL = BZ + 2;
     static constexpr int SLICE = (BR * UCOL + 27) / 64 * 64 + 36;  // first stride >= BR * UCOL + 2, = 36 mod 64
     static_assert(UCOL % 2 == 0 && SLICE % 64 == 36 && SLICE >= BR * UCOL + 2, "LDS strides");
+    static constexpr int TROW = BR + 1;      // staging of the accumulator transpose:
+    static constexpr int TCELL = BR * TROW;  // [kz][row][col], rows BR + 1 apart (in u)
+    static_assert(BZ * TCELL <= BR * SLICE, "the transpose staging fits in u");
     float u[BR * SLICE];
-    float q2[BR][20];     // Q_c2 - c0 per stored column, increasing; 1e30 if unused
-    int jlo[2][BR];       // first detector column of each slice (double-buffered by angle)
-    int cnt[2][BR];       // number of columns of each slice
-    int inc[2][BR];       // 1 if Q_c2 increases with the detector column
-    int chunks[2];        // max over the slices of ceil(cnt / BR)
+    float q2[BR][20];  // Q_c2 - c0 per stored column, increasing; 1e30 if unused
 };
-
 
 // Slice range of slice t of a brick (first column, count, direction).
 template <typename AP>
@@ -300,26 +298,253 @@ __device__ __forceinline__ void bp_slice_range(const AP &A, int bx, int by, int 
     inc = ja <= jb;
 }
 
-// Slice ranges of one angle for the 16 slices of a brick, computed by lanes
-// t = 0 .. BR - 1 of the first wave (all active together), which also publish
-// the number of column chunks.
+// Pass 1 of the separable backprojector for one detector column j of one
+// slice n_slice: the bilinear transpose along z of the column's rows onto the
+// BZ cells of the brick (starting at z0), written to col[UOFF + kz], and the
+// column's in-slice position Q_c2 - c0 to q2. `a` is the angle's index in the
+// projections (their layout is [row][angle][col]).
 template <int BZ, typename AP>
-__device__ __forceinline__ void bp_ranges(const AP &A, int bx, int by, int cols, int t,
-                                          BpShared<BZ> &sh, int buf)
+__device__ __forceinline__ void bp_column_splat(const AP &A, __amdgpu_buffer_rsrc_t rs, uint32_t row_bytes,
+                                                int a, int cols, int rows, int j, int n_slice, int c0,
+                                                int z0, float *col, float &q2)
 {
-    int jlo, cnt, inc;
-    bp_slice_range(A, bx, by, cols, t, jlo, cnt, inc);
-    sh.jlo[buf][t] = jlo;
-    sh.cnt[buf][t] = cnt;
-    sh.inc[buf][t] = inc;
-    int ch = (cnt + BR - 1) / BR;
+    // (slots without a column are never read: q2 = 1e30)
+    auto zero_column = [&]() {
+        float2 *c2 = reinterpret_cast<float2 *>(col + UOFF);
 #pragma unroll
-    for (int o = BR / 2; o > 0; o >>= 1) ch = max(ch, __shfl_xor(ch, o, BR));
-    if (t == 0) sh.chunks[buf] = ch;
+        for (int k = 0; k < BZ / 2; ++k) c2[k] = make_float2(0.0f, 0.0f);
+    };
+    // ray of detector row 0 in column j: slice n is met at the
+    // in-slice point (Az, Q2); row i adds i * Bz along z
+    const double n = (double)n_slice, jd = (double)j;
+    double Az, Bz, Q2;
+    if (A.cone) {
+        const double Dm = (A.P0[0] - A.S[0]) + jd * A.U[0];
+        const double Dz = (A.P0[1] - A.S[1]) + jd * A.U[1];
+        const double D2 = (A.P0[2] - A.S[2]) + jd * A.U[2];
+        const double tau = (n - A.S[0]) * rcp64(Dm);
+        Az = A.S[1] + tau * Dz;
+        Bz = tau * A.V[1];
+        Q2 = A.S[2] + tau * D2;
+    } else {
+        const double Pm = A.P0[0] + jd * A.U[0];
+        const double tau = (n - Pm) * rcp64(A.S[0]);
+        Az = (A.P0[1] + jd * A.U[1]) + tau * A.S[1];
+        Bz = A.V[1];
+        Q2 = (A.P0[2] + jd * A.U[2]) + tau * A.S[2];
+    }
+    q2 = (float)(Q2 - (double)c0);
+    // rows with t_i = Arel + i * Bz in (-1, BZ), bounded in FP32
+    // with a margin that exceeds its rounding: a row just outside
+    // is harmless (its t lies beyond the brick, see the splat
+    // below), and no row inside is lost
+    const double Arel = Az - (double)z0;
+    const float fB = (float)Bz;
+    const float fA = (float)Arel, iB = __builtin_amdgcn_rcpf(fB);
+    const float ra = (-1.0f - fA) * iB, rb = ((float)BZ - fA) * iB;
+    float lo = fminf(ra, rb), hi = fmaxf(ra, rb);
+    lo -= 1e-3f + 1e-5f * fabsf(lo);
+    hi += 1e-3f + 1e-5f * fabsf(hi);
+    const int r0 = (int)ceilf(fmaxf(lo, -1e9f)), r1 = (int)floorf(fminf(hi, 1e9f));
+    const int i0 = max(0, r0);
+    const int i1 = min(rows - 1, r1);
+    // no row of the range is cut off by the detector's edge: then
+    // rows less than two cells apart reach every cell of the brick
+    const bool covered = r0 >= 0 && r1 <= rows - 1 && r0 <= r1;
+    const float t0 = (float)(Arel + (double)i0 * Bz);
+    uint32_t off = (uint32_t)i0 * row_bytes + ((uint32_t)a * (uint32_t)cols + (uint32_t)j) * 4u;
+    float *u = col + UOFF;
+    // The rows are taken in increasing t (in the order of i if
+    // Bz > 0, reversed otherwise; the sign of Bz is that of the
+    // detector's v along z, the same for every column), in
+    // groups of four, the loads of the next group in flight
+    // while the current one is splatted, and no LDS cell is
+    // read. Rows past the range are read too (inside the
+    // buffer, or zero by the range check): their t lies beyond
+    // the brick (t >= BZ), and the clamp of kf to [-2, BZ]
+    // sends their weights to cells BZ, BZ + 1, which pass 2
+    // never reads. For t inside [-2, BZ + 1), kf = floor(t) and
+    // w = fract(t) are the exact bilinear split.
+    //
+    // Rows 1.01 .. 1.99 cells apart (|Bz|):
+    // a row's lower cell kf is at least the previous row's upper
+    // cell pk and at most one above it, and a cell takes at most
+    // two rows. The pending upper weight `carry` stays in a
+    // register, and each row writes cells (pk, pk + 1) once,
+    // final except for the pending one, with no LDS read:
+    //   kf == pk:     (carry + lo, hi)   hi is rewritten later
+    //   kf == pk + 1: (carry, lo)
+    // For a covered column every cell of the brick is written, so
+    // it needs no zeroing. kf is not clamped here: the rows of the
+    // range have kf in [-2, BZ], and the rows past it (kf above
+    // BZ) reach the LDS only through the write base min(pk, BZ),
+    // so they touch cells BZ, BZ + 1 only.
+    auto splat_carry = [&]() {
+        const bool up = fB > 0.0f;
+        const float dB = fabsf(fB);
+        const uint32_t step = up ? row_bytes : 0u - row_bytes;
+        float tb = up ? t0 : fmaf((float)(i1 - i0), fB, t0);
+        uint32_t o = up ? off : off + (uint32_t)(i1 - i0) * row_bytes;
+        int pk = min(max(floor_to_int(tb), -2), BZ);
+        float carry = 0.0f;
+        auto load4s = [&](uint32_t oo, float (&yv)[4]) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                      rs, oo + (uint32_t)r * step, 0, 0));
+        };
+        auto carry4 = [&](float tg, const float (&yv)[4]) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float ti = r == 0 ? tg : fmaf((float)r, dB, tg);
+                const int kf = floor_to_int(ti);
+                const float hiw = __builtin_amdgcn_fractf(ti) * yv[r];
+                const float low = yv[r] - hiw;
+                const bool eq = kf == pk;
+                const int b0 = min(pk, BZ);
+                u[b0] = carry + (eq ? low : 0.0f);
+                u[b0 + 1] = eq ? hiw : low;
+                pk = kf + 1;
+                carry = hiw;
+            }
+        };
+        float ya[4], yb[4];
+        const int nr = i1 - i0 + 1;
+        load4s(o, ya);
+        for (int i = 0; i < nr; i += 8) {
+            load4s(o + 4u * step, yb);
+            __builtin_amdgcn_sched_barrier(0);
+            carry4(tb, ya);
+            if (i + 4 >= nr) break;
+            load4s(o + 8u * step, ya);
+            __builtin_amdgcn_sched_barrier(0);
+            carry4(fmaf(4.0f, dB, tb), yb);
+            o += 8u * step;
+            tb = fmaf(8.0f, dB, tb);
+        }
+        u[min(pk, BZ)] = carry;
+    };
+    // Any row step: the two cells (p, p + 1) of the last row stay
+    // pending in registers (c0, c1); with d = kf - p a row adds
+    // to both (d = 0), finishes p (d = 1) or both (d >= 2), and
+    // writes (p, p + 1) once with their current values. Rows
+    // more than 1.99 cells apart leave cells between them that
+    // no row reaches: those columns are zeroed first, as are
+    // columns cut off by the detector's edge.
+    auto splat_carry2 = [&]() {
+        const bool up = fB > 0.0f;
+        const float dB = fabsf(fB);
+        const uint32_t step = up ? row_bytes : 0u - row_bytes;
+        float tb = up ? t0 : fmaf((float)(i1 - i0), fB, t0);
+        uint32_t o = up ? off : off + (uint32_t)(i1 - i0) * row_bytes;
+        int p = min(max(floor_to_int(tb), -2), BZ);
+        float c0 = 0.0f, c1 = 0.0f;
+        auto load4s = [&](uint32_t oo, float (&yv)[4]) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                                      rs, oo + (uint32_t)r * step, 0, 0));
+        };
+        auto carry4 = [&](float tg, const float (&yv)[4]) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float ti = r == 0 ? tg : fmaf((float)r, dB, tg);
+                const int kf = min(max(floor_to_int(ti), -2), BZ);
+                const float hiw = __builtin_amdgcn_fractf(ti) * yv[r];
+                const float low = yv[r] - hiw;
+                const int d = kf - p;
+                const bool d0 = d == 0, d1 = d == 1;
+                const float v0 = d0 ? c0 + low : c0;
+                const float v1 = c1 + (d0 ? hiw : (d1 ? low : 0.0f));
+                u[p] = v0;
+                u[p + 1] = v1;
+                c0 = d0 ? v0 : (d1 ? v1 : low);
+                c1 = d0 ? v1 : hiw;
+                p = kf;
+            }
+        };
+        float ya[4], yb[4];
+        const int nr = i1 - i0 + 1;
+        load4s(o, ya);
+        for (int i = 0; i < nr; i += 8) {
+            load4s(o + 4u * step, yb);
+            __builtin_amdgcn_sched_barrier(0);
+            carry4(tb, ya);
+            if (i + 4 >= nr) break;
+            load4s(o + 8u * step, ya);
+            __builtin_amdgcn_sched_barrier(0);
+            carry4(fmaf(4.0f, dB, tb), yb);
+            o += 8u * step;
+            tb = fmaf(8.0f, dB, tb);
+        }
+        u[p] = c0;
+        u[p + 1] = c1;
+    };
+    if (__builtin_amdgcn_ballot_w64(fabsf(fB) > 1.01f && fabsf(fB) < 1.99f) ==
+        __builtin_amdgcn_read_exec()) {
+        if (!covered) zero_column();
+        splat_carry();
+    } else {
+        if (!covered || fabsf(fB) >= 1.99f) zero_column();
+        splat_carry2();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// The separable backprojector with wave-local slices. Thread (tx, ty) of the
+// 16 x 16 block works on slice s = ty of the brick in both passes (pass 1:
+// detector column slot tx; pass 2: in-slice voxel column kc = tx), so each
+// wave (slices 4w .. 4w + 3) exchanges LDS data only with itself and the
+// angle loop has no block barrier: a wave finishes its pass 2 of one angle
+// and starts the next while the others are still at work. The voxel column a
+// thread accumulates is (x, y) = (c0 + tx, m0 + ty) for y-stepping angles and
+// (m0 + ty, c0 + tx) for x-stepping ones, so the angles are taken from the
+// plan's table ordered by stepping axis (x first) and the accumulators are
+// moved between threads (tx, ty) and (ty, tx) through LDS where the axis
+// changes (once per launch) and at the end if the last angle steps along x.
+// Each wave computes the slice ranges of its own four slices for the next
+// angle (in registers) and runs as many column chunks as those slices need.
+
+// Orders the LDS accesses of one wave around this point. LDS operations of a
+// wave complete in issue order, so the fences (wavefront scope) only keep the
+// compiler from moving LDS reads and writes across it.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// acc of thread (tx, ty) <- acc of thread (ty, tx), through LDS (block-wide;
+// called by every thread of the block at the same point of the angle loop).
+template <int BZ>
+__device__ __forceinline__ void bp_wave_transpose(float (&acc)[BZ], float *stage, int tx, int ty)
+{
+    using S = BpShared<BZ>;
+    __syncthreads();  // every wave is done with u
+#pragma unroll
+    for (int k = 0; k < BZ; ++k) stage[k * S::TCELL + ty * S::TROW + tx] = acc[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < BZ; ++k) acc[k] = stage[k * S::TCELL + tx * S::TROW + ty];
+    __syncthreads();  // u is written again by pass 1
+}
+
+// Slice range of slice s (the lane's own) and the number of column chunks of
+// the wave (the largest over its four slices, lanes 16 and 32 apart).
+template <typename AP>
+__device__ __forceinline__ void bp_wave_range(const AP &A, int bx, int by, int cols, int s, int &jlo, int &cnt,
+                                              int &inc, int &chunks)
+{
+    bp_slice_range(A, bx, by, cols, s, jlo, cnt, inc);
+    int ch = (cnt + BR - 1) / BR;
+    ch = max(ch, __shfl_xor(ch, 16));
+    ch = max(ch, __shfl_xor(ch, 32));
+    chunks = __builtin_amdgcn_readfirstlane(ch);
 }
 
 template <int BZ>
-__device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, BpShared<BZ> &sh)
+__device__ __forceinline__ void bp_wave_work(const BpView &v, int logical, BpShared<BZ> &sh)
 {
     using S = BpShared<BZ>;
     const int n_angles = v.n_angles, rows = v.rows, cols = v.cols;
@@ -328,15 +553,12 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
     const int bz = (logical / (v.bricks_x * v.bricks_y)) % v.bricks_z;
     const int b = logical / (v.bricks_x * v.bricks_y * v.bricks_z);
     if (b >= v.batch) return;
-    // The angle parameters do not change during the launch. Read through a
-    // constant-address-space pointer, the uniform ones become scalar loads
-    // into SGPRs; through a generic pointer the barriers of the angle loop
-    // (memory fences) make the compiler read them with vector loads into
-    // VGPRs (7 % slower at 32-deep bricks, 22 % at 16-deep ones).
+    // angle parameters through a constant-address-space pointer: the uniform
+    // ones become scalar loads
     typedef const __attribute__((address_space(4))) AngleParams ConstAngle;
-    ConstAngle *__restrict__ angles = (ConstAngle *)v.angles;
+    ConstAngle *__restrict__ angles = (ConstAngle *)v.sorted;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x = bx * BR + tx, y = by * BR + ty, z0 = bz * BZ;
+    const int z0 = bz * BZ;
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)uniform_ptr(v.ly + (int64_t)b * ((int64_t)v.ly_bytes / 4)), (short)0,
         (int)__builtin_amdgcn_readfirstlane(v.ly_bytes), 0x00020000);
@@ -344,271 +566,85 @@ __device__ __forceinline__ void bp_separable_work(const BpView &v, int logical, 
     float acc[BZ];
 #pragma unroll
     for (int k = 0; k < BZ; ++k) acc[k] = 0.0f;
-    // slice ranges (16 lanes of the first wave, one slice each) of angle 0
-    const int t = ty * BR + tx;
-    if (t < BR) bp_ranges(angles[0], bx, by, cols, t, sh, 0);
-    __syncthreads();
-    for (int a = 0; a < n_angles; ++a) {
-        const int buf = a & 1;
-        ConstAngle &A = angles[a];
-        const bool mx = A.m == 0;
+    int mode = angles[0].m;  // stepping axis whose voxel mapping acc follows
+    int jlo, cnt, inc, chunks;
+    bp_wave_range(angles[0], bx, by, cols, ty, jlo, cnt, inc, chunks);
+    for (int k = 0; k < n_angles; ++k) {
+        ConstAngle &A = angles[k];
+        const int am = A.m;
+        if (am != mode) {
+            bp_wave_transpose<BZ>(acc, sh.u, tx, ty);
+            mode = am;
+        }
+        const int cur_jlo = jlo, cur_cnt = cnt, cur_inc = inc, cur_chunks = chunks;
+        if (k + 1 < n_angles) bp_wave_range(angles[k + 1], bx, by, cols, ty, jlo, cnt, inc, chunks);
+        const bool mx = am == 0;
         const int m0 = mx ? bx * BR : by * BR;
         const int c0 = mx ? by * BR : bx * BR;
-        const int chunks = sh.chunks[buf];
-        if (chunks == 0) {
-            // no detector column of this angle meets the brick: only the next
-            // angle's slice ranges are due (uniform branch over the block)
-            if (t < BR && a + 1 < n_angles) bp_ranges(angles[a + 1], bx, by, cols, t, sh, buf ^ 1);
-            __syncthreads();
-            continue;
-        }
-        for (int c = 0; c < chunks; ++c) {
-            // pass 1: thread (s = tx, jj = ty) splats detector column j onto kz
-            // (a wave holds slots 4w .. 4w + 3 of all 16 slices: the empty slots,
-            // the last ones of a slice, gather in the last waves, and a wave's
-            // neighbouring slices read nearly the same detector columns)
+        const int a = A.pad;  // index of the angle in the projections
+        for (int c = 0; c < cur_chunks; ++c) {
+            // pass 1: thread (slice ty, slot tx) splats detector column j onto kz
             {
-                const int s = tx, jj = ty;
-                const int m = min(BR, sh.cnt[buf][s] - c * BR);  // valid columns of this chunk
-                const bool valid = jj < m;
-                const int pos = (!valid || sh.inc[buf][s]) ? jj : m - 1 - jj;
-                const int j = sh.jlo[buf][s] + c * BR + jj;
-                float *col = &sh.u[s * S::SLICE + pos * S::UCOL];
-                if (valid) {
-                    // (slots without a column are never read: q2 = 1e30)
-                    auto zero_column = [&]() {
-                        float2 *c2 = reinterpret_cast<float2 *>(col + UOFF);
-#pragma unroll
-                        for (int k = 0; k < BZ / 2; ++k) c2[k] = make_float2(0.0f, 0.0f);
-                    };
-                    // ray of detector row 0 in column j: slice n is met at the
-                    // in-slice point (Az, Q2); row i adds i * Bz along z
-                    const double n = (double)(m0 + s), jd = (double)j;
-                    double Az, Bz, Q2;
-                    if (A.cone) {
-                        const double Dm = (A.P0[0] - A.S[0]) + jd * A.U[0];
-                        const double Dz = (A.P0[1] - A.S[1]) + jd * A.U[1];
-                        const double D2 = (A.P0[2] - A.S[2]) + jd * A.U[2];
-                        const double tau = (n - A.S[0]) * rcp64(Dm);
-                        Az = A.S[1] + tau * Dz;
-                        Bz = tau * A.V[1];
-                        Q2 = A.S[2] + tau * D2;
-                    } else {
-                        const double Pm = A.P0[0] + jd * A.U[0];
-                        const double tau = (n - Pm) * rcp64(A.S[0]);
-                        Az = (A.P0[1] + jd * A.U[1]) + tau * A.S[1];
-                        Bz = A.V[1];
-                        Q2 = (A.P0[2] + jd * A.U[2]) + tau * A.S[2];
-                    }
-                    sh.q2[s][pos] = (float)(Q2 - (double)c0);
-                    // rows with t_i = Arel + i * Bz in (-1, BZ), bounded in FP32
-                    // with a margin that exceeds its rounding: a row just outside
-                    // is harmless (its t lies beyond the brick, see the splat
-                    // below), and no row inside is lost
-                    const double Arel = Az - (double)z0;
-                    const float fB = (float)Bz;
-                    const float fA = (float)Arel, iB = __builtin_amdgcn_rcpf(fB);
-                    const float ra = (-1.0f - fA) * iB, rb = ((float)BZ - fA) * iB;
-                    float lo = fminf(ra, rb), hi = fmaxf(ra, rb);
-                    lo -= 1e-3f + 1e-5f * fabsf(lo);
-                    hi += 1e-3f + 1e-5f * fabsf(hi);
-                    const int r0 = (int)ceilf(fmaxf(lo, -1e9f)), r1 = (int)floorf(fminf(hi, 1e9f));
-                    const int i0 = max(0, r0);
-                    const int i1 = min(rows - 1, r1);
-                    // no row of the range is cut off by the detector's edge: then
-                    // rows less than two cells apart reach every cell of the brick
-                    const bool covered = r0 >= 0 && r1 <= rows - 1 && r0 <= r1;
-                    const float t0 = (float)(Arel + (double)i0 * Bz);
-                    uint32_t off = (uint32_t)i0 * row_bytes + ((uint32_t)a * (uint32_t)cols + (uint32_t)j) * 4u;
-                    float *u = col + UOFF;
-                    // The rows are taken in increasing t (in the order of i if
-                    // Bz > 0, reversed otherwise; the sign of Bz is that of the
-                    // detector's v along z, the same for every column), in
-                    // groups of four, the loads of the next group in flight
-                    // while the current one is splatted, and no LDS cell is
-                    // read. Rows past the range are read too (inside the
-                    // buffer, or zero by the range check): their t lies beyond
-                    // the brick (t >= BZ), and the clamp of kf to [-2, BZ]
-                    // sends their weights to cells BZ, BZ + 1, which pass 2
-                    // never reads. For t inside [-2, BZ + 1), kf = floor(t) and
-                    // w = fract(t) are the exact bilinear split.
-                    //
-                    // Rows 1.01 .. 1.99 cells apart (|Bz|):
-                    // a row's lower cell kf is at least the previous row's upper
-                    // cell pk and at most one above it, and a cell takes at most
-                    // two rows. The pending upper weight `carry` stays in a
-                    // register, and each row writes cells (pk, pk + 1) once,
-                    // final except for the pending one, with no LDS read:
-                    //   kf == pk:     (carry + lo, hi)   hi is rewritten later
-                    //   kf == pk + 1: (carry, lo)
-                    // For a covered column every cell of the brick is written, so
-                    // it needs no zeroing. kf is not clamped here: the rows of the
-                    // range have kf in [-2, BZ], and the rows past it (kf above
-                    // BZ) reach the LDS only through the write base min(pk, BZ),
-                    // so they touch cells BZ, BZ + 1 only.
-                    auto splat_carry = [&]() {
-                        const bool up = fB > 0.0f;
-                        const float dB = fabsf(fB);
-                        const uint32_t step = up ? row_bytes : 0u - row_bytes;
-                        float tb = up ? t0 : fmaf((float)(i1 - i0), fB, t0);
-                        uint32_t o = up ? off : off + (uint32_t)(i1 - i0) * row_bytes;
-                        int pk = min(max(floor_to_int(tb), -2), BZ);
-                        float carry = 0.0f;
-                        auto load4s = [&](uint32_t oo, float (&yv)[4]) {
-#pragma unroll
-                            for (int r = 0; r < 4; ++r)
-                                yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                                      rs, oo + (uint32_t)r * step, 0, 0));
-                        };
-                        auto carry4 = [&](float tg, const float (&yv)[4]) {
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const float ti = r == 0 ? tg : fmaf((float)r, dB, tg);
-                                const int kf = floor_to_int(ti);
-                                const float hiw = __builtin_amdgcn_fractf(ti) * yv[r];
-                                const float low = yv[r] - hiw;
-                                const bool eq = kf == pk;
-                                const int b0 = min(pk, BZ);
-                                u[b0] = carry + (eq ? low : 0.0f);
-                                u[b0 + 1] = eq ? hiw : low;
-                                pk = kf + 1;
-                                carry = hiw;
-                            }
-                        };
-                        float ya[4], yb[4];
-                        const int nr = i1 - i0 + 1;
-                        load4s(o, ya);
-                        for (int i = 0; i < nr; i += 8) {
-                            load4s(o + 4u * step, yb);
-                            __builtin_amdgcn_sched_barrier(0);
-                            carry4(tb, ya);
-                            if (i + 4 >= nr) break;
-                            load4s(o + 8u * step, ya);
-                            __builtin_amdgcn_sched_barrier(0);
-                            carry4(fmaf(4.0f, dB, tb), yb);
-                            o += 8u * step;
-                            tb = fmaf(8.0f, dB, tb);
-                        }
-                        u[min(pk, BZ)] = carry;
-                    };
-                    // Any row step: the two cells (p, p + 1) of the last row stay
-                    // pending in registers (c0, c1); with d = kf - p a row adds
-                    // to both (d = 0), finishes p (d = 1) or both (d >= 2), and
-                    // writes (p, p + 1) once with their current values. Rows
-                    // more than 1.99 cells apart leave cells between them that
-                    // no row reaches: those columns are zeroed first, as are
-                    // columns cut off by the detector's edge.
-                    auto splat_carry2 = [&]() {
-                        const bool up = fB > 0.0f;
-                        const float dB = fabsf(fB);
-                        const uint32_t step = up ? row_bytes : 0u - row_bytes;
-                        float tb = up ? t0 : fmaf((float)(i1 - i0), fB, t0);
-                        uint32_t o = up ? off : off + (uint32_t)(i1 - i0) * row_bytes;
-                        int p = min(max(floor_to_int(tb), -2), BZ);
-                        float c0 = 0.0f, c1 = 0.0f;
-                        auto load4s = [&](uint32_t oo, float (&yv)[4]) {
-#pragma unroll
-                            for (int r = 0; r < 4; ++r)
-                                yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                                      rs, oo + (uint32_t)r * step, 0, 0));
-                        };
-                        auto carry4 = [&](float tg, const float (&yv)[4]) {
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) {
-                                const float ti = r == 0 ? tg : fmaf((float)r, dB, tg);
-                                const int kf = min(max(floor_to_int(ti), -2), BZ);
-                                const float hiw = __builtin_amdgcn_fractf(ti) * yv[r];
-                                const float low = yv[r] - hiw;
-                                const int d = kf - p;
-                                const bool d0 = d == 0, d1 = d == 1;
-                                const float v0 = d0 ? c0 + low : c0;
-                                const float v1 = c1 + (d0 ? hiw : (d1 ? low : 0.0f));
-                                u[p] = v0;
-                                u[p + 1] = v1;
-                                c0 = d0 ? v0 : (d1 ? v1 : low);
-                                c1 = d0 ? v1 : hiw;
-                                p = kf;
-                            }
-                        };
-                        float ya[4], yb[4];
-                        const int nr = i1 - i0 + 1;
-                        load4s(o, ya);
-                        for (int i = 0; i < nr; i += 8) {
-                            load4s(o + 4u * step, yb);
-                            __builtin_amdgcn_sched_barrier(0);
-                            carry4(tb, ya);
-                            if (i + 4 >= nr) break;
-                            load4s(o + 8u * step, ya);
-                            __builtin_amdgcn_sched_barrier(0);
-                            carry4(fmaf(4.0f, dB, tb), yb);
-                            o += 8u * step;
-                            tb = fmaf(8.0f, dB, tb);
-                        }
-                        u[p] = c0;
-                        u[p + 1] = c1;
-                    };
-                    if (__builtin_amdgcn_ballot_w64(fabsf(fB) > 1.01f && fabsf(fB) < 1.99f) ==
-                        __builtin_amdgcn_read_exec()) {
-                        if (!covered) zero_column();
-                        splat_carry();
-                    } else {
-                        if (!covered || fabsf(fB) >= 1.99f) zero_column();
-                        splat_carry2();
-                    }
-                } else {
-                    sh.q2[s][pos] = 1e30f;
-                }
+                const int m = min(BR, cur_cnt - c * BR);  // valid columns of this chunk
+                const bool valid = tx < m;
+                const int pos = (!valid || cur_inc) ? tx : m - 1 - tx;
+                const int j = cur_jlo + c * BR + tx;
+                float *col = &sh.u[ty * S::SLICE + pos * S::UCOL];
+                if (valid)
+                    bp_column_splat<BZ>(A, rs, row_bytes, a, cols, rows, j, m0 + ty, c0, z0, col, sh.q2[ty][pos]);
+                else
+                    sh.q2[ty][pos] = 1e30f;  // (slots without a column are never read)
             }
-            // slice ranges of the next angle (the buffer was last read in the
-            // previous angle, before its final barrier)
-            if (c == 0 && t < BR && a + 1 < n_angles) bp_ranges(angles[a + 1], bx, by, cols, t, sh, buf ^ 1);
-            __syncthreads();
+            wave_lds_sync();
             // pass 2: the stored columns with |Q_c2 - kc| < 1 follow the first
-            // one with Q_c2 > kc - 1 (binary search over the increasing q2[s][.])
+            // one with Q_c2 > kc - 1 (binary search over the increasing q2[ty][.])
             {
-                const int s = mx ? tx : ty;
-                const float kc = (float)(mx ? ty : tx);
-                const float *qs = sh.q2[s];
+                const float kc = (float)tx;
+                const float *qs = sh.q2[ty];
                 int pos = 0;
 #pragma unroll
                 for (int step = BR / 2; step > 0; step >>= 1)
                     pos += (qs[pos + step - 1] <= kc - 1.0f) ? step : 0;
                 pos += (qs[pos] <= kc - 1.0f) ? 1 : 0;  // all BR entries below
-                const float *us = &sh.u[s * S::SLICE + UOFF];
+                const float *us = &sh.u[ty * S::SLICE + UOFF];
                 for (; pos < BR; ++pos) {
                     const float d = qs[pos] - kc;
                     if (!(d < 1.0f)) break;
                     const float w = 1.0f - fabsf(d);
                     const float2 *uc = reinterpret_cast<const float2 *>(us + pos * S::UCOL);
 #pragma unroll
-                    for (int k = 0; k < BZ / 2; ++k) {
-                        const float2 uv = uc[k];
-                        acc[2 * k] = fmaf(w, uv.x, acc[2 * k]);
-                        acc[2 * k + 1] = fmaf(w, uv.y, acc[2 * k + 1]);
+                    for (int q = 0; q < BZ / 2; ++q) {
+                        const float2 uv = uc[q];
+                        acc[2 * q] = fmaf(w, uv.x, acc[2 * q]);
+                        acc[2 * q + 1] = fmaf(w, uv.y, acc[2 * q + 1]);
                     }
                 }
             }
-            __syncthreads();  // u and q2 are rewritten by the next chunk or angle
+            wave_lds_sync();  // u and q2 are rewritten by the next chunk or angle
         }
     }
+    if (mode == 0) bp_wave_transpose<BZ>(acc, sh.u, tx, ty);
+    // thread (tx, ty) now holds voxel column (x, y) = (bx * BR + tx, by * BR + ty)
+    const int x = bx * BR + tx, y = by * BR + ty;
     if (x < v.nx && y < v.ny) {
         float *out = v.vol + (int64_t)b * v.vol_stride;
 #pragma unroll
-        for (int k = 0; k < BZ; ++k) {
-            const int z = z0 + k;
-            if (z < v.nz) emit(out, ((int64_t)z * v.ny + y) * v.nx + x, b, acc[k], v.ep);
+        for (int q = 0; q < BZ; ++q) {
+            const int z = z0 + q;
+            if (z < v.nz) emit(out, ((int64_t)z * v.ny + y) * v.nx + x, b, acc[q], v.ep);
         }
     }
 }
 
 template <int BZ>
-__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ == 32 ? 4 : 1))) bp_separable(BpView v, int n_blocks, int shift)
+__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ == 32 ? 4 : 1))) bp_wave(BpView v, int n_blocks, int shift)
 {
     __shared__ __align__(16) BpShared<BZ> sh;
     const int logical = xcd_logical(shift);
     if (logical >= n_blocks) return;
-    bp_separable_work<BZ>(v, logical, sh);
+    bp_wave_work<BZ>(v, logical, sh);
 }
+
 
 struct BpMulti {
     BpView v[MAX_FUSED];
@@ -618,7 +654,7 @@ struct BpMulti {
 };
 
 template <int BZ>
-__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ == 32 ? 4 : 1))) bp_separable_multi(BpMulti m)
+__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ == 32 ? 4 : 1))) bp_wave_multi(BpMulti m)
 {
     __shared__ __align__(16) BpShared<BZ> sh;
     const int total = m.begin[m.n];
@@ -626,7 +662,7 @@ __global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ 
     if (logical >= total) return;
     int k = 0;
     while (k + 1 < m.n && logical >= m.begin[k + 1]) ++k;
-    bp_separable_work<BZ>(m.v[k], logical - m.begin[k], sh);
+    bp_wave_work<BZ>(m.v[k], logical - m.begin[k], sh);
 }
 
 }  // namespace tsx

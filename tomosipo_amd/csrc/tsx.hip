@@ -106,6 +106,9 @@ struct tsx_plan {
     int32_t nx = 0, ny = 0, nz = 0;
     int32_t ss = 1;  // detector supersampling
     std::vector<AngleParams> host;
+    // device: the n_angles entries of `host`, then the same entries ordered by
+    // stepping axis (x, y, z; stable), each with its index in `host` in `pad`
+    // (the angle order of the wave-local separable backprojector)
     AngleParams *dev = nullptr;
     int32_t axis_count[3] = {0, 0, 0};
     // padded per-axis layouts for the forward projector (elements per layout)
@@ -212,6 +215,20 @@ void build_fdk(const tsx_geometry *g, tsx_plan *p)
         F.half_dbeta = 0.5 * db;
         F.inv_du = 1.0 / std::sqrt(dot(U, U));
     }
+}
+
+// The device angle table of a plan: the angles in order, then ordered by
+// stepping axis with their original index in `pad`.
+std::vector<AngleParams> device_angle_table(const std::vector<AngleParams> &host)
+{
+    std::vector<AngleParams> t(host);
+    for (int c = 0; c < 3; ++c)
+        for (size_t a = 0; a < host.size(); ++a)
+            if (host[a].m == c) {
+                t.push_back(host[a]);
+                t.back().pad = (int32_t)a;
+            }
+    return t;
 }
 
 int build_angles(const tsx_geometry *g, tsx_plan *p)
@@ -729,6 +746,7 @@ BpView bp_view(const tsx_plan *p, int BZ, float *vol, int64_t vs, const float *l
 {
     BpView v;
     v.angles = p->dev;
+    v.sorted = p->dev + p->n_angles;
     v.n_angles = p->n_angles;
     v.rows = p->rows;
     v.cols = p->cols;
@@ -806,9 +824,9 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
         int shift;
         const unsigned grid = xcd_grid(n_blocks, &shift);
         if (BZ == 32)
-            hipLaunchKernelGGL(bp_separable<32>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
+            hipLaunchKernelGGL(bp_wave<32>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
         else
-            hipLaunchKernelGGL(bp_separable<16>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
+            hipLaunchKernelGGL(bp_wave<16>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
         TSX_HIP_CHECK(hipGetLastError());
     }
     return TSX_OK;
@@ -967,9 +985,9 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         if (m.n == 0) return TSX_OK;
         const unsigned grid = xcd_grid(m.begin[m.n], &m.shift);
         if (BZ == 32)
-            hipLaunchKernelGGL(bp_separable_multi<32>, dim3(grid), dim3(BR, BR), 0, stream, m);
+            hipLaunchKernelGGL(bp_wave_multi<32>, dim3(grid), dim3(BR, BR), 0, stream, m);
         else
-            hipLaunchKernelGGL(bp_separable_multi<16>, dim3(grid), dim3(BR, BR), 0, stream, m);
+            hipLaunchKernelGGL(bp_wave_multi<16>, dim3(grid), dim3(BR, BR), 0, stream, m);
         TSX_HIP_CHECK(hipGetLastError());
         m.n = 0;
         return TSX_OK;
@@ -1118,7 +1136,8 @@ int autotune(tsx_plan *p, hipStream_t stream)
     const size_t vol_bytes = (size_t)p->nx * p->ny * p->nz * 4;
     const size_t proj_bytes = (size_t)p->rows * sub.n_angles * p->cols * 4;
     const size_t band_bytes = (size_t)std::min(p->rows, 64) * p->n_angles * p->cols * 4;
-    if (hipMalloc(&t.angles, sizeof(AngleParams) * sub.host.size()) != hipSuccess ||
+    const std::vector<AngleParams> sub_table = device_angle_table(sub.host);
+    if (hipMalloc(&t.angles, sizeof(AngleParams) * sub_table.size()) != hipSuccess ||
         hipMalloc(&t.band_angles, sizeof(AngleParams) * p->host.size()) != hipSuccess ||
         hipMalloc(&t.vol, vol_bytes) != hipSuccess || hipMalloc(&t.proj, proj_bytes) != hipSuccess ||
         hipMalloc(&t.band_proj, band_bytes) != hipSuccess) {
@@ -1126,7 +1145,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
         return fail(TSX_ERR_OUT_OF_MEMORY, "autotune: no device memory for the trial buffers");
     }
     sub.dev = t.angles;
-    TSX_HIP_CHECK(hipMemcpyAsync(t.angles, sub.host.data(), sizeof(AngleParams) * sub.host.size(),
+    TSX_HIP_CHECK(hipMemcpyAsync(t.angles, sub_table.data(), sizeof(AngleParams) * sub_table.size(),
                                  hipMemcpyHostToDevice, stream));
     TSX_HIP_CHECK(hipMemsetAsync(t.vol, 0, vol_bytes, stream));
     TSX_HIP_CHECK(hipMemsetAsync(t.proj, 0, proj_bytes, stream));
@@ -1281,13 +1300,14 @@ int tsx_plan_create(const tsx_geometry *g, int device, tsx_plan **out)
             delete p;
             return fail(TSX_ERR_HIP, "hipSetDevice failed");
         }
-        size_t bytes = sizeof(AngleParams) * p->host.size();
+        const std::vector<AngleParams> table = device_angle_table(p->host);
+        size_t bytes = sizeof(AngleParams) * table.size();
         e = hipMalloc((void **)&p->dev, bytes);
         if (e != hipSuccess) {
             delete p;
             return fail(TSX_ERR_OUT_OF_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
         }
-        e = hipMemcpy(p->dev, p->host.data(), bytes, hipMemcpyHostToDevice);
+        e = hipMemcpy(p->dev, table.data(), bytes, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             (void)hipFree(p->dev);
             delete p;
@@ -1332,7 +1352,7 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->vol_shape[0] = p->nz;
     info->vol_shape[1] = p->ny;
     info->vol_shape[2] = p->nx;
-    info->plan_bytes = p->dev ? (int64_t)(sizeof(AngleParams) * p->host.size()) : 0;
+    info->plan_bytes = p->dev ? (int64_t)(sizeof(AngleParams) * 2 * p->host.size()) : 0;
     for (int c = 0; c < 3; ++c) info->axis_count[c] = p->axis_count[c];
     info->fp_kernel = p->fp_padded && !force_general() ? 1 : 0;
     info->bp_kernel = p->bp_separable && !force_general() ? 1 : 0;
