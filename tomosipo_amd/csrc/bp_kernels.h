@@ -246,6 +246,8 @@ struct BpView {
     const AngleParams *angles;
     const AngleParams *sorted;  // the angles ordered by stepping axis, original index in `pad`
     int n_angles, rows, cols, nx, ny, nz, bricks_x, bricks_y, bricks_z, batch;
+    const uint32_t *rt;  // slice-range table (bp_range_table), rt_stride entries per angle
+    int64_t rt_stride;
     float *vol;
     int64_t vol_stride;
     const float *ly;    // length-weighted projections, batch entries ly_bytes apart
@@ -296,6 +298,42 @@ __device__ __forceinline__ void bp_slice_range(const AP &A, int bx, int by, int 
     jlo = lo;
     cnt = max(0, hi - lo + 1);
     inc = ja <= jb;
+}
+
+// Slice-range table of the separable backprojector: for every angle k of the
+// axis-ordered table, every column-block cb of 16 voxels along the in-slice
+// axis c2 and every slice n along the stepping axis (rounded up to whole
+// bricks), entry k * stride + cb * nm + n (nm = slices rounded up to 16)
+// holds the detector columns that meet the brick's slice n, as
+// jlo | cnt << 16 | inc << 31 (first column, count, 1 if Q_c2 increases with
+// the column; needs fewer than 32768 detector columns). Depends only on the
+// geometry: built once per plan, so the kernel reads one word per lane and
+// angle instead of evaluating the FP64 column map twice.
+constexpr int BP_RT_MAX_COLS = 32767;
+
+// Entries per angle: bricks along x times bricks along y times BR slices
+// (the same for x- and y-stepping angles).
+__host__ __device__ inline int64_t bp_rt_stride(int nx, int ny)
+{
+    return (int64_t)((nx + BR - 1) / BR) * ((ny + BR - 1) / BR) * BR;
+}
+
+__global__ void bp_range_table(const AngleParams *__restrict__ sorted, int cols, int nx, int ny, int64_t stride,
+                               uint32_t *__restrict__ out)
+{
+    const int k = blockIdx.y;
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= stride) return;
+    const AngleParams &A = sorted[k];
+    const bool mx = A.m == 0;
+    const int nm = mx ? (nx + BR - 1) / BR * BR : (ny + BR - 1) / BR * BR;
+    const int ncb = mx ? (ny + BR - 1) / BR : (nx + BR - 1) / BR;
+    if (e >= (int64_t)nm * ncb) return;
+    const int cb = (int)(e / nm), n = (int)(e % nm);
+    const int bx = mx ? n / BR : cb, by = mx ? cb : n / BR;
+    int jlo, cnt, inc;
+    bp_slice_range(A, bx, by, cols, n % BR, jlo, cnt, inc);
+    out[(int64_t)k * stride + e] = (uint32_t)jlo | ((uint32_t)cnt << 16) | ((uint32_t)inc << 31);
 }
 
 // Pass 1 of the separable backprojector for one detector column j of one
@@ -530,13 +568,22 @@ __device__ __forceinline__ void bp_wave_transpose(float (&acc)[BZ], float *stage
     __syncthreads();  // u is written again by pass 1
 }
 
-// Slice range of slice s (the lane's own) and the number of column chunks of
-// the wave (the largest over its four slices, lanes 16 and 32 apart).
-template <typename AP>
-__device__ __forceinline__ void bp_wave_range(const AP &A, int bx, int by, int cols, int s, int &jlo, int &cnt,
-                                              int &inc, int &chunks)
+// The lane's entry of the slice-range table for angle k (its slice ty of the
+// block's brick; the c2 block and slice index follow the angle's axis).
+__device__ __forceinline__ uint32_t bp_wave_entry(const BpView &v, int k, bool mx, int bx, int by, int ty)
 {
-    bp_slice_range(A, bx, by, cols, s, jlo, cnt, inc);
+    const int64_t e = mx ? (int64_t)by * (v.bricks_x * BR) + bx * BR + ty
+                         : (int64_t)bx * (v.bricks_y * BR) + by * BR + ty;
+    return v.rt[(int64_t)k * v.rt_stride + e];
+}
+
+// Decodes a table entry; chunks = the wave's number of column chunks (the
+// largest over its four slices, lanes 16 and 32 apart).
+__device__ __forceinline__ void bp_wave_decode(uint32_t w, int &jlo, int &cnt, int &inc, int &chunks)
+{
+    jlo = (int)(w & 0xffffu);
+    cnt = (int)((w >> 16) & 0x7fffu);
+    inc = (int)(w >> 31);
     int ch = (cnt + BR - 1) / BR;
     ch = max(ch, __shfl_xor(ch, 16));
     ch = max(ch, __shfl_xor(ch, 32));
@@ -567,8 +614,8 @@ __device__ __forceinline__ void bp_wave_work(const BpView &v, int logical, BpSha
 #pragma unroll
     for (int k = 0; k < BZ; ++k) acc[k] = 0.0f;
     int mode = angles[0].m;  // stepping axis whose voxel mapping acc follows
-    int jlo, cnt, inc, chunks;
-    bp_wave_range(angles[0], bx, by, cols, ty, jlo, cnt, inc, chunks);
+    // slice ranges from the table, read one angle ahead
+    uint32_t next = bp_wave_entry(v, 0, mode == 0, bx, by, ty);
     for (int k = 0; k < n_angles; ++k) {
         ConstAngle &A = angles[k];
         const int am = A.m;
@@ -576,8 +623,9 @@ __device__ __forceinline__ void bp_wave_work(const BpView &v, int logical, BpSha
             bp_wave_transpose<BZ>(acc, sh.u, tx, ty);
             mode = am;
         }
-        const int cur_jlo = jlo, cur_cnt = cnt, cur_inc = inc, cur_chunks = chunks;
-        if (k + 1 < n_angles) bp_wave_range(angles[k + 1], bx, by, cols, ty, jlo, cnt, inc, chunks);
+        int cur_jlo, cur_cnt, cur_inc, cur_chunks;
+        bp_wave_decode(next, cur_jlo, cur_cnt, cur_inc, cur_chunks);
+        if (k + 1 < n_angles) next = bp_wave_entry(v, k + 1, angles[k + 1].m == 0, bx, by, ty);
         const bool mx = am == 0;
         const int m0 = mx ? bx * BR : by * BR;
         const int c0 = mx ? by * BR : bx * BR;

@@ -110,6 +110,9 @@ struct tsx_plan {
     // stepping axis (x, y, z; stable), each with its index in `host` in `pad`
     // (the angle order of the wave-local separable backprojector)
     AngleParams *dev = nullptr;
+    // separable backprojector: slice-range table (bp_range_table), device
+    uint32_t *bp_rt = nullptr;
+    int64_t bp_rt_stride = 0;  // entries per angle
     int32_t axis_count[3] = {0, 0, 0};
     // padded per-axis layouts for the forward projector (elements per layout)
     int64_t layout_elems[3] = {0, 0, 0};
@@ -369,6 +372,10 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
     // separable backprojection: upright detector (v along z), stepping along x or y
     p->bp_separable = g->det_supersampling == 1;
     if (!p->bp_separable) p->bp_reason = "detector supersampling";
+    if (g->det_col_count > BP_RT_MAX_COLS) {
+        p->bp_separable = false;
+        p->bp_reason = "more than " + std::to_string(BP_RT_MAX_COLS) + " detector columns";
+    }
     if ((int64_t)g->det_row_count * g->n_angles * g->det_col_count * 4 >= ((int64_t)1 << 32)) {
         p->bp_separable = false;
         p->bp_reason = "projection data of 4 GiB or more";
@@ -586,6 +593,26 @@ void launch_scale_by_length(const tsx_plan *p, const float *proj, int64_t ps, fl
                        proj, ps, ly, ly_stride, lines);
 }
 
+// Allocates and fills the slice-range table of a separable plan whose device
+// angle table is in place (work enqueued on `stream`).
+int build_range_table(tsx_plan *p, hipStream_t stream, uint32_t **out)
+{
+    *out = nullptr;
+    if (!p->bp_separable) return TSX_OK;
+    const int64_t stride = bp_rt_stride(p->nx, p->ny);
+    const hipError_t e = hipMalloc((void **)out, (size_t)(stride * p->n_angles * 4));
+    if (e != hipSuccess) {
+        *out = nullptr;
+        return fail(TSX_ERR_OUT_OF_MEMORY, std::string("slice-range table: ") + hipGetErrorString(e));
+    }
+    p->bp_rt_stride = stride;
+    dim3 grid((unsigned)((stride + 255) / 256), (unsigned)p->n_angles);
+    hipLaunchKernelGGL(bp_range_table, grid, dim3(256), 0, stream, p->dev + p->n_angles, p->cols, p->nx, p->ny,
+                       stride, *out);
+    TSX_HIP_CHECK(hipGetLastError());
+    return TSX_OK;
+}
+
 // Either the caller's workspace (checked for size) or stream-ordered memory.
 int get_workspace(const tsx_plan *p, int64_t bytes, void *ws, int64_t ws_bytes, hipStream_t stream,
                   Scratch &owned, void **out)
@@ -747,6 +774,8 @@ BpView bp_view(const tsx_plan *p, int BZ, float *vol, int64_t vs, const float *l
     BpView v;
     v.angles = p->dev;
     v.sorted = p->dev + p->n_angles;
+    v.rt = p->bp_rt;
+    v.rt_stride = p->bp_rt_stride;
     v.n_angles = p->n_angles;
     v.rows = p->rows;
     v.cols = p->cols;
@@ -1078,8 +1107,10 @@ struct TuneScratch {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     void *vol = nullptr, *proj = nullptr, *band_proj = nullptr;
     AngleParams *angles = nullptr, *band_angles = nullptr;
+    uint32_t *rt = nullptr;
     ~TuneScratch()
     {
+        if (rt) (void)hipFree(rt);
         if (band_proj) (void)hipFree(band_proj);
         if (band_angles) (void)hipFree(band_angles);
         if (e0) (void)hipEventDestroy(e0);
@@ -1147,6 +1178,11 @@ int autotune(tsx_plan *p, hipStream_t stream)
     sub.dev = t.angles;
     TSX_HIP_CHECK(hipMemcpyAsync(t.angles, sub_table.data(), sizeof(AngleParams) * sub_table.size(),
                                  hipMemcpyHostToDevice, stream));
+    {
+        int rc = build_range_table(&sub, stream, &t.rt);
+        if (rc) return rc;
+        sub.bp_rt = t.rt;
+    }
     TSX_HIP_CHECK(hipMemsetAsync(t.vol, 0, vol_bytes, stream));
     TSX_HIP_CHECK(hipMemsetAsync(t.proj, 0, proj_bytes, stream));
     TSX_HIP_CHECK(hipEventCreate(&t.e0));
@@ -1324,6 +1360,16 @@ int tsx_plan_create(const tsx_geometry *g, int device, tsx_plan **out)
                 return fail(TSX_ERR_OUT_OF_MEMORY, std::string("FDK tables: ") + hipGetErrorString(e));
             }
         }
+        int rc = build_range_table(p, nullptr, &p->bp_rt);
+        if (rc == TSX_OK && hipStreamSynchronize(nullptr) != hipSuccess)
+            rc = fail(TSX_ERR_HIP, "slice-range table: synchronisation failed");
+        if (rc) {
+            if (p->bp_rt) (void)hipFree(p->bp_rt);
+            if (p->fdk_dev) (void)hipFree(p->fdk_dev);
+            (void)hipFree(p->dev);
+            delete p;
+            return rc;
+        }
         p->device = device;
     }
     *out = p;
@@ -1337,6 +1383,7 @@ int tsx_plan_destroy(tsx_plan *p)
         DeviceGuard guard(p->device);
         (void)hipFree(p->dev);
         if (p->fdk_dev) (void)hipFree(p->fdk_dev);
+        if (p->bp_rt) (void)hipFree(p->bp_rt);
     }
     delete p;
     return TSX_OK;
@@ -1352,7 +1399,9 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->vol_shape[0] = p->nz;
     info->vol_shape[1] = p->ny;
     info->vol_shape[2] = p->nx;
-    info->plan_bytes = p->dev ? (int64_t)(sizeof(AngleParams) * 2 * p->host.size()) : 0;
+    info->plan_bytes = p->dev ? (int64_t)(sizeof(AngleParams) * 2 * p->host.size()) +
+                                    (p->bp_rt ? p->bp_rt_stride * p->n_angles * 4 : 0)
+                              : 0;
     for (int c = 0; c < 3; ++c) info->axis_count[c] = p->axis_count[c];
     info->fp_kernel = p->fp_padded && !force_general() ? 1 : 0;
     info->bp_kernel = p->bp_separable && !force_general() ? 1 : 0;
