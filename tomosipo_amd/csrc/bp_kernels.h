@@ -199,6 +199,22 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
 // (x, y) takes columns 256 x .. 256 x + 255 of the detector lines y,
 // y + gridDim.y, ... of the `lines` = batch * rows * n_angles lines (line
 // (b, i, a) is b * rows * n_angles + i * n_angles + a).
+// World length per slice step of the ray through pixel (i, j) of angle A.
+__device__ __forceinline__ float ray_length(const AngleParams &A, int i, int j)
+{
+    float D[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float P = (float)(A.P0[c] + (double)j * A.U[c] + (double)i * A.V[c]);
+        D[c] = A.cone ? P - (float)A.S[c] : (float)A.S[c];
+    }
+    const float inv = 1.0f / D[0];
+    const float a1 = D[1] * inv, a2 = D[2] * inv;
+    return (D[0] == 0.0f) ? 0.0f
+        : sqrtf((float)(A.h[0] * A.h[0]) + (float)(A.h[1] * A.h[1]) * a1 * a1 +
+                (float)(A.h[2] * A.h[2]) * a2 * a2);
+}
+
 __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_angles, int rows,
                                 int cols, const float *__restrict__ proj, int64_t proj_stride,
                                 float *__restrict__ out, int64_t out_stride, int64_t lines)
@@ -210,20 +226,55 @@ __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_an
         const int64_t b = line / per_entry;
         const int64_t ia = line - b * per_entry;
         const int i = (int)(ia / n_angles), a = (int)(ia - (int64_t)i * n_angles);
-        const AngleParams &A = angles[a];
-        float D[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const float P = (float)(A.P0[c] + (double)j * A.U[c] + (double)i * A.V[c]);
-            D[c] = A.cone ? P - (float)A.S[c] : (float)A.S[c];
-        }
-        const float inv = 1.0f / D[0];
-        const float a1 = D[1] * inv, a2 = D[2] * inv;
-        const float len = (D[0] == 0.0f) ? 0.0f
-            : sqrtf((float)(A.h[0] * A.h[0]) + (float)(A.h[1] * A.h[1]) * a1 * a1 +
-                    (float)(A.h[2] * A.h[2]) * a2 * a2);
+        const float len = ray_length(angles[a], i, j);
         const int64_t o = ia * cols + j;
         out[b * out_stride + o] = len * proj[b * proj_stride + o];
+    }
+}
+
+// Length-weighted projections for the separable backprojector, transposed so
+// that the rows of one detector column are contiguous:
+//   out[b][a][j][RPAD + i] = l(a, i, j) * proj[b][i][a][j],
+// each column padded with RPAD zero rows on both sides (rows -RPAD .. -1 and
+// rows .. rows + RPAD - 1), so that loads of four consecutive rows starting
+// anywhere in [-RPAD, rows - 1] stay inside the column. Block (x, y): tile of
+// 32 rows x 32 columns (x = row tile + tiles_i * column tile) of the lines y,
+// y + gridDim.y, ... of the n_angles * batch (entry, angle) pairs, through LDS.
+constexpr int RPAD = 3;
+
+__global__ void __launch_bounds__(256)
+scale_by_length_t(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols,
+                  const float *__restrict__ proj, int64_t proj_stride, float *__restrict__ out,
+                  int64_t out_stride, int64_t lines)
+{
+    __shared__ float tile[32][33];
+    const int tiles_i = (rows + 31) / 32;
+    const int i0 = (int)(blockIdx.x % tiles_i) * 32, j0 = (int)(blockIdx.x / tiles_i) * 32;
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    const int rp = rows + 2 * RPAD;
+    for (int64_t line = blockIdx.y; line < lines; line += gridDim.y) {
+        const int64_t b = line / n_angles;
+        const int a = (int)(line - b * n_angles);
+        const AngleParams &A = angles[a];
+        __syncthreads();  // the tile of the previous line has been written out
+        for (int r = ty; r < 32; r += 8) {
+            const int i = i0 + r, j = j0 + tx;
+            float v = 0.0f;
+            if (i < rows && j < cols)
+                v = ray_length(A, i, j) * proj[b * proj_stride + ((int64_t)i * n_angles + a) * cols + j];
+            tile[r][tx] = v;
+        }
+        __syncthreads();
+        for (int c = ty; c < 32; c += 8) {
+            const int j = j0 + c;
+            if (j >= cols) continue;
+            float *col = out + b * out_stride + ((int64_t)a * cols + j) * rp;
+            if (i0 + tx < rows) col[RPAD + i0 + tx] = tile[tx][c];
+            if (tx < RPAD) {
+                if (i0 == 0) col[tx] = 0.0f;
+                if (i0 + 32 >= rows) col[RPAD + rows + tx] = 0.0f;
+            }
+        }
     }
 }
 
@@ -336,15 +387,155 @@ __global__ void bp_range_table(const AngleParams *__restrict__ sorted, int cols,
     out[(int64_t)k * stride + e] = (uint32_t)jlo | ((uint32_t)cnt << 16) | ((uint32_t)inc << 31);
 }
 
+// Rows ig .. ig + 3 of a column of the transposed length-weighted
+// projections (byte offset colbase + 4 ig), in increasing t: in the order of
+// the rows when UP, reversed otherwise.
+template <bool UP>
+__device__ __forceinline__ void bp_rows4(__amdgpu_buffer_rsrc_t rs, uint32_t off, float (&y)[4])
+{
+    const float4 v = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+    y[0] = UP ? v.x : v.w;
+    y[1] = UP ? v.y : v.z;
+    y[2] = UP ? v.z : v.y;
+    y[3] = UP ? v.w : v.x;
+}
+
+// Splat of the rows i0 .. i1 of one column onto the cells u[kz] (the bilinear
+// transpose along z), rows 1.01 .. 1.99 cells apart (|Bz|): a row's lower
+// cell kf is at least the previous row's upper cell pk and at most one above
+// it, and a cell takes at most two rows. The pending upper weight `carry`
+// stays in a register, and each row writes cells (pk, pk + 1) once, final
+// except for the pending one, with no LDS read:
+//   kf == pk:     (carry + lo, hi)   hi is rewritten later
+//   kf == pk + 1: (carry, lo)
+// For t inside [-2, BZ + 1), kf = floor(t) and w = fract(t) are the exact
+// bilinear split. The rows of the range have t in (-1 - e, BZ + e) (e the
+// small margin of the row range), so kf is in [-2, BZ] and only the last row
+// of the range can have kf = BZ: every row of a full group of four writes
+// cells up to BZ + 1 without a clamp. The last, partial group also takes up
+// to three rows past the range (their t lies beyond the brick, t >= BZ); it
+// writes through the base min(pk, BZ), which sends those rows to cells BZ,
+// BZ + 1 (the next column's unused cells -2, -1), never read by pass 2.
+// Group g starts at row i0 + 4g (UP) or i1 - 3 - 4g, never below i0 - 3: in
+// the column's zero padding at worst. The load of the group after the last
+// may fall beyond the column (its values are not used).
+template <int BZ, bool UP, bool CLAMP>
+__device__ __forceinline__ void bp_carry4(float tg, float dB, const float (&y)[4], float *u, int &pk, float &carry)
+{
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float ti = r == 0 ? tg : fmaf((float)r, dB, tg);
+        const int kf = floor_to_int(ti);
+        const float hiw = __builtin_amdgcn_fractf(ti) * y[r];
+        const float low = y[r] - hiw;
+        const bool eq = kf == pk;
+        const int b0 = CLAMP ? min(pk, BZ) : pk;
+        u[b0] = carry + (eq ? low : 0.0f);
+        u[b0 + 1] = eq ? hiw : low;
+        pk = kf + 1;
+        carry = hiw;
+    }
+}
+
+template <int BZ, bool UP>
+__device__ __forceinline__ void bp_splat_carry(__amdgpu_buffer_rsrc_t rs, uint32_t colbase, int i0, int i1, float t0,
+                                               float fB, float *u)
+{
+    const float dB = fabsf(fB);
+    const int nr = i1 - i0 + 1;
+    float tb = UP ? t0 : fmaf((float)(i1 - i0), fB, t0);  // t of the first row taken
+    int pk = min(max(floor_to_int(tb), -2), BZ);
+    float carry = 0.0f;
+    if (nr > 0) {
+        // two groups per iteration, loaded alternately into ya and yb, each
+        // load in flight while the other group is splatted
+        const uint32_t step = UP ? 16u : 0u - 16u;
+        uint32_t o = colbase + (uint32_t)(UP ? i0 : i1 - 3) * 4u;
+        float ya[4], yb[4];
+        bp_rows4<UP>(rs, o, ya);
+        const int nfull = nr >> 2;
+        int g = 0;
+        for (; g + 1 < nfull; g += 2) {
+            bp_rows4<UP>(rs, o + step, yb);
+            __builtin_amdgcn_sched_barrier(0);
+            bp_carry4<BZ, UP, false>(tb, dB, ya, u, pk, carry);
+            bp_rows4<UP>(rs, o + 2u * step, ya);
+            __builtin_amdgcn_sched_barrier(0);
+            bp_carry4<BZ, UP, false>(fmaf(4.0f, dB, tb), dB, yb, u, pk, carry);
+            o += 2u * step;
+            tb = fmaf(8.0f, dB, tb);
+        }
+        // ya holds group g: the last full group (nfull odd) or the partial one
+        if (g < nfull) {
+            bp_rows4<UP>(rs, o + step, yb);
+            __builtin_amdgcn_sched_barrier(0);
+            bp_carry4<BZ, UP, false>(tb, dB, ya, u, pk, carry);
+            if (nr & 3) bp_carry4<BZ, UP, true>(fmaf(4.0f, dB, tb), dB, yb, u, pk, carry);
+        } else if (nr & 3) {
+            bp_carry4<BZ, UP, true>(tb, dB, ya, u, pk, carry);
+        }
+    }
+    u[min(pk, BZ)] = carry;
+}
+
+// Any row step: the two cells (p, p + 1) of the last row stay pending in
+// registers (c0, c1); with d = kf - p a row adds to both (d = 0), finishes p
+// (d = 1) or both (d >= 2), and writes (p, p + 1) once with their current
+// values; kf is clamped to [-2, BZ] for every row. Rows more than 1.99 cells
+// apart leave cells between them that no row reaches: the caller zeroes those
+// columns first, as it does columns cut off by the detector's edge.
+template <int BZ, bool UP>
+__device__ __forceinline__ void bp_splat_carry2(__amdgpu_buffer_rsrc_t rs, uint32_t colbase, int i0, int i1, float t0,
+                                                float fB, float *u)
+{
+    const float dB = fabsf(fB);
+    const int nr = i1 - i0 + 1;
+    float tb = UP ? t0 : fmaf((float)(i1 - i0), fB, t0);
+    int p = min(max(floor_to_int(tb), -2), BZ);
+    float c0 = 0.0f, c1 = 0.0f;
+    if (nr > 0) {
+        const uint32_t step = UP ? 16u : 0u - 16u;
+        uint32_t o = colbase + (uint32_t)(UP ? i0 : i1 - 3) * 4u;
+        float cur[4], nxt[4];
+        bp_rows4<UP>(rs, o, cur);
+        for (int i = 0; i < nr; i += 4) {
+            o += step;
+            bp_rows4<UP>(rs, o, nxt);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float ti = r == 0 ? tb : fmaf((float)r, dB, tb);
+                const int kf = min(max(floor_to_int(ti), -2), BZ);
+                const float hiw = __builtin_amdgcn_fractf(ti) * cur[r];
+                const float low = cur[r] - hiw;
+                const int d = kf - p;
+                const bool d0 = d == 0, d1 = d == 1;
+                const float v0 = d0 ? c0 + low : c0;
+                const float v1 = c1 + (d0 ? hiw : (d1 ? low : 0.0f));
+                u[p] = v0;
+                u[p + 1] = v1;
+                c0 = d0 ? v0 : (d1 ? v1 : low);
+                c1 = d0 ? v1 : hiw;
+                p = kf;
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) cur[r] = nxt[r];
+            tb = fmaf(4.0f, dB, tb);
+        }
+    }
+    u[p] = c0;
+    u[p + 1] = c1;
+}
+
 // Pass 1 of the separable backprojector for one detector column j of one
 // slice n_slice: the bilinear transpose along z of the column's rows onto the
 // BZ cells of the brick (starting at z0), written to col[UOFF + kz], and the
 // column's in-slice position Q_c2 - c0 to q2. `a` is the angle's index in the
-// projections (their layout is [row][angle][col]).
+// projections; rs addresses their transposed, padded, length-weighted copy
+// (scale_by_length_t).
 template <int BZ, typename AP>
-__device__ __forceinline__ void bp_column_splat(const AP &A, __amdgpu_buffer_rsrc_t rs, uint32_t row_bytes,
-                                                int a, int cols, int rows, int j, int n_slice, int c0,
-                                                int z0, float *col, float &q2)
+__device__ __forceinline__ void bp_column_splat(const AP &A, __amdgpu_buffer_rsrc_t rs, int a, int cols, int rows,
+                                                int j, int n_slice, int c0, int z0, float *col, float &q2)
 {
     // (slots without a column are never read: q2 = 1e30)
     auto zero_column = [&]() {
@@ -390,141 +581,27 @@ __device__ __forceinline__ void bp_column_splat(const AP &A, __amdgpu_buffer_rsr
     // rows less than two cells apart reach every cell of the brick
     const bool covered = r0 >= 0 && r1 <= rows - 1 && r0 <= r1;
     const float t0 = (float)(Arel + (double)i0 * Bz);
-    uint32_t off = (uint32_t)i0 * row_bytes + ((uint32_t)a * (uint32_t)cols + (uint32_t)j) * 4u;
     float *u = col + UOFF;
-    // The rows are taken in increasing t (in the order of i if
-    // Bz > 0, reversed otherwise; the sign of Bz is that of the
-    // detector's v along z, the same for every column), in
-    // groups of four, the loads of the next group in flight
-    // while the current one is splatted, and no LDS cell is
-    // read. Rows past the range are read too (inside the
-    // buffer, or zero by the range check): their t lies beyond
-    // the brick (t >= BZ), and the clamp of kf to [-2, BZ]
-    // sends their weights to cells BZ, BZ + 1, which pass 2
-    // never reads. For t inside [-2, BZ + 1), kf = floor(t) and
-    // w = fract(t) are the exact bilinear split.
-    //
-    // Rows 1.01 .. 1.99 cells apart (|Bz|):
-    // a row's lower cell kf is at least the previous row's upper
-    // cell pk and at most one above it, and a cell takes at most
-    // two rows. The pending upper weight `carry` stays in a
-    // register, and each row writes cells (pk, pk + 1) once,
-    // final except for the pending one, with no LDS read:
-    //   kf == pk:     (carry + lo, hi)   hi is rewritten later
-    //   kf == pk + 1: (carry, lo)
-    // For a covered column every cell of the brick is written, so
-    // it needs no zeroing. kf is not clamped here: the rows of the
-    // range have kf in [-2, BZ], and the rows past it (kf above
-    // BZ) reach the LDS only through the write base min(pk, BZ),
-    // so they touch cells BZ, BZ + 1 only.
-    auto splat_carry = [&]() {
-        const bool up = fB > 0.0f;
-        const float dB = fabsf(fB);
-        const uint32_t step = up ? row_bytes : 0u - row_bytes;
-        float tb = up ? t0 : fmaf((float)(i1 - i0), fB, t0);
-        uint32_t o = up ? off : off + (uint32_t)(i1 - i0) * row_bytes;
-        int pk = min(max(floor_to_int(tb), -2), BZ);
-        float carry = 0.0f;
-        auto load4s = [&](uint32_t oo, float (&yv)[4]) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                      rs, oo + (uint32_t)r * step, 0, 0));
-        };
-        auto carry4 = [&](float tg, const float (&yv)[4]) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float ti = r == 0 ? tg : fmaf((float)r, dB, tg);
-                const int kf = floor_to_int(ti);
-                const float hiw = __builtin_amdgcn_fractf(ti) * yv[r];
-                const float low = yv[r] - hiw;
-                const bool eq = kf == pk;
-                const int b0 = min(pk, BZ);
-                u[b0] = carry + (eq ? low : 0.0f);
-                u[b0 + 1] = eq ? hiw : low;
-                pk = kf + 1;
-                carry = hiw;
-            }
-        };
-        float ya[4], yb[4];
-        const int nr = i1 - i0 + 1;
-        load4s(o, ya);
-        for (int i = 0; i < nr; i += 8) {
-            load4s(o + 4u * step, yb);
-            __builtin_amdgcn_sched_barrier(0);
-            carry4(tb, ya);
-            if (i + 4 >= nr) break;
-            load4s(o + 8u * step, ya);
-            __builtin_amdgcn_sched_barrier(0);
-            carry4(fmaf(4.0f, dB, tb), yb);
-            o += 8u * step;
-            tb = fmaf(8.0f, dB, tb);
-        }
-        u[min(pk, BZ)] = carry;
-    };
-    // Any row step: the two cells (p, p + 1) of the last row stay
-    // pending in registers (c0, c1); with d = kf - p a row adds
-    // to both (d = 0), finishes p (d = 1) or both (d >= 2), and
-    // writes (p, p + 1) once with their current values. Rows
-    // more than 1.99 cells apart leave cells between them that
-    // no row reaches: those columns are zeroed first, as are
-    // columns cut off by the detector's edge.
-    auto splat_carry2 = [&]() {
-        const bool up = fB > 0.0f;
-        const float dB = fabsf(fB);
-        const uint32_t step = up ? row_bytes : 0u - row_bytes;
-        float tb = up ? t0 : fmaf((float)(i1 - i0), fB, t0);
-        uint32_t o = up ? off : off + (uint32_t)(i1 - i0) * row_bytes;
-        int p = min(max(floor_to_int(tb), -2), BZ);
-        float c0 = 0.0f, c1 = 0.0f;
-        auto load4s = [&](uint32_t oo, float (&yv)[4]) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                yv[r] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-                                                      rs, oo + (uint32_t)r * step, 0, 0));
-        };
-        auto carry4 = [&](float tg, const float (&yv)[4]) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const float ti = r == 0 ? tg : fmaf((float)r, dB, tg);
-                const int kf = min(max(floor_to_int(ti), -2), BZ);
-                const float hiw = __builtin_amdgcn_fractf(ti) * yv[r];
-                const float low = yv[r] - hiw;
-                const int d = kf - p;
-                const bool d0 = d == 0, d1 = d == 1;
-                const float v0 = d0 ? c0 + low : c0;
-                const float v1 = c1 + (d0 ? hiw : (d1 ? low : 0.0f));
-                u[p] = v0;
-                u[p + 1] = v1;
-                c0 = d0 ? v0 : (d1 ? v1 : low);
-                c1 = d0 ? v1 : hiw;
-                p = kf;
-            }
-        };
-        float ya[4], yb[4];
-        const int nr = i1 - i0 + 1;
-        load4s(o, ya);
-        for (int i = 0; i < nr; i += 8) {
-            load4s(o + 4u * step, yb);
-            __builtin_amdgcn_sched_barrier(0);
-            carry4(tb, ya);
-            if (i + 4 >= nr) break;
-            load4s(o + 8u * step, ya);
-            __builtin_amdgcn_sched_barrier(0);
-            carry4(fmaf(4.0f, dB, tb), yb);
-            o += 8u * step;
-            tb = fmaf(8.0f, dB, tb);
-        }
-        u[p] = c0;
-        u[p + 1] = c1;
-    };
-    if (__builtin_amdgcn_ballot_w64(fabsf(fB) > 1.01f && fabsf(fB) < 1.99f) ==
-        __builtin_amdgcn_read_exec()) {
+    // The rows are taken in increasing t (in the order of i if Bz > 0,
+    // reversed otherwise; the sign of Bz is that of the detector's v along z,
+    // the same for every column: UP is uniform), in groups of four
+    // consecutive rows read as one 16-byte load from the column's rows in the
+    // transposed, padded layout of scale_by_length_t, the next group in
+    // flight while the current one is splatted, and no LDS cell is read.
+    const uint32_t colbase =
+        (((uint32_t)a * (uint32_t)cols + (uint32_t)j) * (uint32_t)(rows + 2 * RPAD) + (uint32_t)RPAD) * 4u;
+    if (__builtin_amdgcn_ballot_w64(fabsf(fB) > 1.01f && fabsf(fB) < 1.99f) == __builtin_amdgcn_read_exec()) {
         if (!covered) zero_column();
-        splat_carry();
+        if (A.V[1] > 0.0)
+            bp_splat_carry<BZ, true>(rs, colbase, i0, i1, t0, fB, u);
+        else
+            bp_splat_carry<BZ, false>(rs, colbase, i0, i1, t0, fB, u);
     } else {
         if (!covered || fabsf(fB) >= 1.99f) zero_column();
-        splat_carry2();
+        if (A.V[1] > 0.0)
+            bp_splat_carry2<BZ, true>(rs, colbase, i0, i1, t0, fB, u);
+        else
+            bp_splat_carry2<BZ, false>(rs, colbase, i0, i1, t0, fB, u);
     }
 }
 
@@ -609,7 +686,6 @@ __device__ __forceinline__ void bp_wave_work(const BpView &v, int logical, BpSha
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         (void *)uniform_ptr(v.ly + (int64_t)b * ((int64_t)v.ly_bytes / 4)), (short)0,
         (int)__builtin_amdgcn_readfirstlane(v.ly_bytes), 0x00020000);
-    const uint32_t row_bytes = (uint32_t)n_angles * (uint32_t)cols * 4u;
     float acc[BZ];
 #pragma unroll
     for (int k = 0; k < BZ; ++k) acc[k] = 0.0f;
@@ -639,7 +715,7 @@ __device__ __forceinline__ void bp_wave_work(const BpView &v, int logical, BpSha
                 const int j = cur_jlo + c * BR + tx;
                 float *col = &sh.u[ty * S::SLICE + pos * S::UCOL];
                 if (valid)
-                    bp_column_splat<BZ>(A, rs, row_bytes, a, cols, rows, j, m0 + ty, c0, z0, col, sh.q2[ty][pos]);
+                    bp_column_splat<BZ>(A, rs, a, cols, rows, j, m0 + ty, c0, z0, col, sh.q2[ty][pos]);
                 else
                     sh.q2[ty][pos] = 1e30f;  // (slots without a column are never read)
             }

@@ -376,7 +376,7 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
         p->bp_separable = false;
         p->bp_reason = "more than " + std::to_string(BP_RT_MAX_COLS) + " detector columns";
     }
-    if ((int64_t)g->det_row_count * g->n_angles * g->det_col_count * 4 >= ((int64_t)1 << 32)) {
+    if ((int64_t)(g->det_row_count + 2 * RPAD) * g->n_angles * g->det_col_count * 4 >= ((int64_t)1 << 32)) {
         p->bp_separable = false;
         p->bp_reason = "projection data of 4 GiB or more";
     }
@@ -523,8 +523,12 @@ int64_t fp_workspace_elems(const tsx_plan *p)
     return per_entry;
 }
 
+// Length-weighted projections of one batch entry: transposed and padded
+// ([angle][col][RPAD + row], scale_by_length_t) for the separable kernel,
+// in the projections' own layout for the general one.
 int64_t bp_workspace_elems(const tsx_plan *p)
 {
+    if (p->bp_separable) return (int64_t)p->n_angles * p->cols * (p->rows + 2 * RPAD);
     return (int64_t)p->rows * p->n_angles * p->cols;
 }
 
@@ -611,6 +615,17 @@ int build_range_table(tsx_plan *p, hipStream_t stream, uint32_t **out)
                        stride, *out);
     TSX_HIP_CHECK(hipGetLastError());
     return TSX_OK;
+}
+
+// The transposed, padded length-weighted projections of `batch` entries.
+void launch_scale_by_length_t(const tsx_plan *p, const float *proj, int64_t ps, float *ly, int64_t ly_stride,
+                              int64_t batch, hipStream_t stream)
+{
+    const int64_t lines = (int64_t)p->n_angles * batch;
+    const unsigned tiles = (unsigned)(((p->rows + 31) / 32) * ((p->cols + 31) / 32));
+    dim3 grid(tiles, (unsigned)std::min(lines, GRID_DIM_MAX));
+    hipLaunchKernelGGL(scale_by_length_t, grid, dim3(32, 8), 0, stream, p->dev, p->n_angles, p->rows, p->cols,
+                       proj, ps, ly, ly_stride, lines);
 }
 
 // Either the caller's workspace (checked for size) or stream-ordered memory.
@@ -832,22 +847,23 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
         }
         return launch_bp_general<float, false>(p, vol, vs, proj, ps, batch, ep, stream);
     }
-    if (sino * 4 >= ((int64_t)1 << 32))
+    const int64_t sino_t = bp_workspace_elems(p);  // transposed, padded
+    if (sino_t * 4 >= ((int64_t)1 << 32))
         return fail(TSX_ERR_UNSUPPORTED, "projection data of 4 GiB or more per entry");
     Scratch owned;
     void *wptr = nullptr;
-    int rc = get_workspace(p, sino * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
+    int rc = get_workspace(p, sino_t * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
     if (rc) return rc;
     float *ly = (float *)wptr;
-    launch_scale_by_length(p, proj, ps, ly, sino, batch, stream);
+    launch_scale_by_length_t(p, proj, ps, ly, sino_t, batch, stream);
     const int BZ = bp_brick_z_of(p);
-    BpView v0 = bp_view(p, BZ, vol, vs, ly, (uint32_t)(sino * 4), 1, ep);
+    BpView v0 = bp_view(p, BZ, vol, vs, ly, (uint32_t)(sino_t * 4), 1, ep);
     const int64_t bricks = (int64_t)v0.bricks_x * v0.bricks_y * v0.bricks_z;
     if (bricks > max_blocks(BR * BR)) return fail(TSX_ERR_UNSUPPORTED, "volume too large for one launch");
     const int64_t bc = max_blocks(BR * BR) / bricks;  // entries per launch
     for (int64_t b0 = 0; b0 < batch; b0 += bc) {
         const int64_t nbat = std::min(bc, batch - b0);
-        BpView v = bp_view(p, BZ, vol + b0 * vs, vs, ly + b0 * sino, (uint32_t)(sino * 4), (int)nbat,
+        BpView v = bp_view(p, BZ, vol + b0 * vs, vs, ly + b0 * sino_t, (uint32_t)(sino_t * 4), (int)nbat,
                            ep_from(ep, b0));
         const int n_blocks = (int)(bricks * nbat);
         int shift;
@@ -1040,9 +1056,9 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         BZ = bz;
         float *ly = (float *)((char *)wptr + woff);
         woff += wb;
-        const int64_t sino = bp_workspace_elems(p);
-        launch_scale_by_length(p, projs[k], 0, ly, sino, 1, stream);
-        m.v[m.n] = bp_view(p, BZ, vols[k], 0, ly, (uint32_t)(sino * 4), 1, eps[k]);
+        const int64_t sino_t = bp_workspace_elems(p);
+        launch_scale_by_length_t(p, projs[k], 0, ly, sino_t, 1, stream);
+        m.v[m.n] = bp_view(p, BZ, vols[k], 0, ly, (uint32_t)(sino_t * 4), 1, eps[k]);
         m.begin[m.n + 1] = m.begin[m.n] + (int)blocks;
         m.n += 1;
         if (m.n == MAX_FUSED) {
