@@ -37,9 +37,16 @@ SIZE = (np.sqrt(2), 1.5 * np.sqrt(2))
 # stepping axis: samples per voxel-angle = rows * cols * N / N^3.
 FLOP_PER_SAMPLE = 14.0
 GATHER_B_PER_SAMPLE = 16.0
+# MI355X vector FP32: 157.3 TFLOP/s published (packed FP32, two flops per lane
+# per instruction slot); this build disables packed FP32, so the issue-rate
+# ceiling of its kernels is half of that.
 FP32_PEAK_TFLOPS = 157.3
-LDS_PEAK_TBS = 78.6
+FP32_NONPACKED_PEAK_TFLOPS = 78.6
 HBM_PEAK_TBS = 8.0
+# Aggregate LDS read bandwidth of the MI355X, measured with a 16-byte-per-lane
+# LDS read kernel (profiles/lds_bandwidth.txt); None until measured.
+LDS_MEASURED_PROFILE = os.path.join(ROOT, "profiles", "lds_bandwidth.json")
+LDS_ASSUMED_TBS = 78.6  # 256 CU x 128 B/clk x 2.4 GHz (CDNA3-class LDS)
 
 
 def instance_geometry(i, ts):
@@ -87,6 +94,8 @@ def cpu_baseline(ts, target_s=12.0):
     t_bp = time.perf_counter() - t0
     na = sub.num_angles
     gups = 2 * N_VOX ** 3 * na / (t_fp + t_bp) / 1e9
+    gups_fp = N_VOX ** 3 * na / t_fp / 1e9
+    gups_bp = N_VOX ** 3 * na / t_bp / 1e9
     # the small parallel scan (64^3, 96 angles, 64x96) in full: one FP and one BP
     vg1, pg1 = small_parallel_geometry(ts)
     g1 = oracle.oracle_geometry(vg1, pg1)
@@ -100,10 +109,45 @@ def cpu_baseline(ts, target_s=12.0):
         "unit": "GUPS",
         "cores": oracle.num_threads(),
         "kind": "port",
+        "gups_fp": round(gups_fp, 4),
+        "gups_bp": round(gups_bp, 4),
+        "host": host_cpu(),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
         "sample": (f"instance 0 of the workload, every {step}th angle ({na} of {N_ANGLES}): "
                    f"one FP ({t_fp:.2f} s) and one BP ({t_bp:.2f} s) in FP64"),
         "small_parallel_full_ms": round(t_small * 1e3, 2),
     }
+
+
+def host_cpu():
+    """lscpu's model name and socket / core / thread counts of this host."""
+    import subprocess
+
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except (OSError, subprocess.SubprocessError):
+        out = ""
+    for line in out.splitlines():
+        key, _, val = line.partition(":")
+        key, val = key.strip(), val.strip()
+        if key in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+            info[key] = val
+    info["os_cpu_count"] = os.cpu_count()
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+    return info
+
+
+def lds_peak():
+    """(TB/s, source) of the aggregate LDS bandwidth used as the gather peak."""
+    if os.path.exists(LDS_MEASURED_PROFILE):
+        with open(LDS_MEASURED_PROFILE) as f:
+            m = json.load(f)
+        return float(m["lds_read_TBps"]), "measured (profiles/lds_bandwidth.json)"
+    return LDS_ASSUMED_TBS, "assumed 256 CU x 128 B/clk x 2.4 GHz"
 
 
 def small_parallel_geometry(ts):
@@ -320,16 +364,23 @@ def main():
         ("forward", fp_ms * args.instances)
     achieved_tf = samples * FLOP_PER_SAMPLE / (dom_ms * 1e-3) / 1e12
     gather_tbs = samples * GATHER_B_PER_SAMPLE / (dom_ms * 1e-3) / 1e12
+    lds_tbs, lds_source = lds_peak()
     roofline = {
-        "bound": "mfma",
+        # the projectors are gathers with FP32 vector arithmetic and no matrix
+        # instructions: the compute bound is the vector ALU, not MFMA
+        "bound": "valu",
         "achieved": round(achieved_tf, 3),
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved_tf / FP32_PEAK_TFLOPS, 4),
+        "peak_nonpacked": FP32_NONPACKED_PEAK_TFLOPS,
+        "frac_nonpacked": round(achieved_tf / FP32_NONPACKED_PEAK_TFLOPS, 4),
         "traffic": None,
         "kernel": f"fused {dom} over {args.instances} instances (incl. its pre-pass)",
         "kernel_ms": round(dom_ms, 3),
-        "basis": "FP32 vector flops, 14 per bilinear sample (no MFMA: projection is a gather)",
+        "basis": ("FP32 vector flops, 14 per bilinear sample, every detector ray x every slice of "
+                  "its stepping axis (no MFMA: projection is a gather); peak = published packed-FP32 "
+                  "rate, peak_nonpacked = the rate without packed FP32, which this build does not use"),
     }
     rec = {
         "metric": "GUPS (voxels x angles / s), FP+BP, 512^3 cone_vec, 720 angles, 512x768 detector",
@@ -355,9 +406,10 @@ def main():
         },
         "roofline": roofline,
         "roofline_gather": {
-            "achieved": round(gather_tbs, 3), "peak": LDS_PEAK_TBS, "unit": "TB/s",
-            "frac": round(gather_tbs / LDS_PEAK_TBS, 4),
-            "basis": "16 B of bilinear corners per sample against aggregate LDS bandwidth",
+            "achieved": round(gather_tbs, 3), "peak": round(lds_tbs, 2), "unit": "TB/s",
+            "frac": round(gather_tbs / lds_tbs, 4),
+            "basis": "16 B of bilinear corners per sample against aggregate LDS read bandwidth",
+            "peak_source": lds_source,
         },
     }
     # HBM bytes per launch of the dominant kernel from the committed rocprofv3 counter
@@ -369,7 +421,11 @@ def main():
         key = "bp_separable_bytes_per_instance" if dom == "backprojection" else "fp_padded_bytes_per_instance"
         if key in t:
             roofline["traffic"] = float(t[key]) * args.instances
-            roofline["traffic_unit"] = "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
+            roofline["traffic_unit"] = "bytes per launch"
+            roofline["traffic_source"] = ("read from profiles/hbm_traffic.json: rocprofv3 --pmc "
+                                          "FETCH_SIZE + WRITE_SIZE of a one-instance run, copy-"
+                                          f"calibrated, times {args.instances} instances; not "
+                                          "measured in this run")
     if args.serving_requests > 0:
         rec["serving"] = serving_benchmark(args.serving_requests)
     rec["small_parallel"] = small_parallel_benchmark(torch, ts)

@@ -40,5 +40,24 @@ def build(verbose=False):
     return LIB
 
 
+PEAKS_SRC = os.path.join(CSRC, "peaks.hip")
+PEAKS_BIN = os.path.join(CSRC, "peaks")
+
+
+def build_peaks():
+    """Compile the peak-rate microbenchmarks (csrc/peaks.hip) into ``csrc/peaks``."""
+    flags = [f for f in HIPCC_FLAGS if f not in ("-shared", "-fPIC") and not f.startswith(("-L", "-l", "-Wl"))]
+    cmd = [hipcc(), *flags, "-o", PEAKS_BIN, PEAKS_SRC]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{' '.join(cmd)}\n{res.stderr}")
+    return PEAKS_BIN
+
+
 if __name__ == "__main__":
-    print(build(verbose=True))
+    import sys
+
+    if len(sys.argv) > 1 and sys.argv[1] == "peaks":
+        print(build_peaks())
+    else:
+        print(build(verbose=True))
