@@ -176,26 +176,64 @@ def small_parallel_benchmark(torch, ts, reps=20):
             "ms": round((time.perf_counter() - t0) * 1e3 / reps, 3)}
 
 
-def _serve_both(reqs, warm):
+def progress(msg):
+    """One line on stderr per finished part of the run."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def _serve_both(reqs, warm, gups_basis):
+    """Both servers on the same stream, with a capacity ceiling: the stream's
+    total work (voxel-angle updates, request_work) at ``gups_basis`` GUPS (the
+    FP+BP rate of the fused 512^3 launches measured in this run, the projectors'
+    best sustained rate; smaller requests run slower, so the ceiling is an upper
+    bound), and no request served before it arrives."""
     from tomosipo_amd import serving
 
-    # tune every request shape once (a one-off cost per shape in a long-running
-    # server), then warm both servers' plans and kernels on a few requests
-    serving.warm_plans(reqs)
-    serving.Server().run(warm)
-    serving.serve_one_at_a_time(warm)
-    batched = serving.Server(hbm_budget_bytes=24 << 30).run(reqs)
-    single = serving.serve_one_at_a_time(reqs)
+    # create and tune every request shape once (a long-running server keeps its
+    # plans), shared by both servers; then warm both servers on a few requests
+    ops = {}
+    n_shapes = serving.warm_plans(list(reqs) + list(warm), operators=ops)
+    progress(f"serving: {n_shapes} request shapes planned")
+    serving.Server(operators=ops).run(warm)
+    serving.serve_one_at_a_time(warm, operators=ops)
+    batched = serving.Server(hbm_budget_bytes=24 << 30, operators=ops).run(reqs)
+    progress(f"serving: continuous batching {batched['req_per_s']:.1f} req/s")
+    single = serving.serve_one_at_a_time(reqs, operators=ops)
+    progress(f"serving: one request per launch {single['req_per_s']:.1f} req/s")
     keys = ("req_per_s", "p50_latency_s", "p99_latency_s", "makespan_s")
+    # least work that serves the stream: each request's own projections (a SIRT
+    # request: its data term A x plus two per iteration), and the SIRT weights
+    # R = 1 / A(1), C = 1 / A.T(1) once per distinct SIRT geometry
+    work = 0.0
+    sirt_keys = set()
+    for r in reqs:
+        va = float(r.side) ** 3 * r.n_angles
+        if r.op == "sirt":
+            work += va * (2 * r.iterations + 1)
+            if serving.request_key(r) not in sirt_keys:
+                sirt_keys.add(serving.request_key(r))
+                work += 2 * va
+        else:
+            work += va
+    span = max(r.arrival for r in reqs) - min(r.arrival for r in reqs)
+    min_makespan = max(span, work / (gups_basis * 1e9))
+    ceiling = len(reqs) / min_makespan
     return {
         "continuous_batching": {k: round(batched[k], 4) for k in keys} |
-        {"deferred_admissions": batched["deferred_admissions"]},
+        {"deferred_admissions": batched["deferred_admissions"], "failed": batched["failed"]},
         "one_request_per_launch": {k: round(single[k], 4) for k in keys},
         "req_per_s_speedup": round(batched["req_per_s"] / single["req_per_s"], 3),
+        "capacity_ceiling": {
+            "work_voxel_angles": work, "gups_basis": round(gups_basis, 1),
+            "min_makespan_s": round(min_makespan, 3), "req_per_s": round(ceiling, 3),
+            "continuous_batching_frac": round(batched["req_per_s"] / ceiling, 3),
+            "one_request_per_launch_frac": round(single["req_per_s"] / ceiling, 3),
+            "two_x_one_request_per_launch_exceeds_ceiling": bool(2 * single["req_per_s"] > ceiling),
+        },
     }
 
 
-def serving_benchmark(n_requests):
+def serving_benchmark(n_requests, gups_basis):
     """Mixed request stream (Poisson 200 req/s; sides 64-256; 60-720 angles; cone or
     parallel; FP, BP or 20-iteration SIRT): continuous batching vs one request per launch.
     A second stream, a burst of small FP/BP requests (sides 64-128, 60-180 angles, all
@@ -205,12 +243,12 @@ def serving_benchmark(n_requests):
 
     out = {"stream": f"{n_requests} requests, Poisson 200 req/s, seed 7"}
     out |= _serve_both(serving.make_stream(n_requests, rate=200.0, seed=7),
-                       serving.make_stream(6, rate=1000.0, seed=8, sirt_iterations=1))
+                       serving.make_stream(6, rate=1000.0, seed=8, sirt_iterations=1), gups_basis)
     small = dict(sides=(64, 96, 128), angles=(60, 180), ops=("fp", "bp"))
     out["small_requests"] = {"stream": f"{3 * n_requests} FP/BP requests, sides 64-128, 60-180 "
                                        "angles, Poisson 1e5 req/s (a burst), seed 9"}
     out["small_requests"] |= _serve_both(serving.make_stream(3 * n_requests, rate=1e5, seed=9, **small),
-                                         serving.make_stream(6, rate=1000.0, seed=10, **small))
+                                         serving.make_stream(6, rate=1000.0, seed=10, **small), gups_basis)
     return out
 
 
@@ -300,7 +338,7 @@ def main():
     ap.add_argument("--instances", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--serving-requests", type=int, default=120,
+    ap.add_argument("--serving-requests", type=int, default=1000,
                     help="requests in the mixed serving stream (0 skips the serving run)")
     ap.add_argument("--no-sirt", action="store_true", help="skip the 256^3 SIRT timing")
     ap.add_argument("--no-fdk", action="store_true", help="skip the 512^3 FDK timing")
@@ -426,20 +464,25 @@ def main():
                                           "FETCH_SIZE + WRITE_SIZE of a one-instance run, copy-"
                                           f"calibrated, times {args.instances} instances; not "
                                           "measured in this run")
+    progress(f"throughput workload: {gups:.1f} GUPS FP+BP")
     if args.serving_requests > 0:
-        rec["serving"] = serving_benchmark(args.serving_requests)
+        rec["serving"] = serving_benchmark(args.serving_requests, gups)
     rec["small_parallel"] = small_parallel_benchmark(torch, ts)
     rec["autograd_batch"] = autograd_batch_benchmark(torch, ts)
+    progress("small parallel scan and autograd batch done")
     if not args.no_sirt:
         rec["sirt"] = sirt_benchmark(torch, ts)
+        progress("SIRT done")
     if not args.no_fdk:
         rec["fdk"] = fdk_benchmark(torch, ts)
+        progress("FDK done")
     if args.calibrate_copy:
         src = torch.empty(256 * 1024 * 1024, device=dev)
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
     if not args.no_cpu:
+        progress("CPU baseline (FP64 oracle) starting")
         rec["cpu_baseline"] = cpu_baseline(ts, args.cpu_seconds)
         rec["cpu_baseline"]["gpu_over_cpu"] = round(gups / rec["cpu_baseline"]["value"], 1)
     print(json.dumps(rec), flush=True)

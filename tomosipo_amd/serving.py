@@ -89,12 +89,13 @@ def request_bytes(req):
     s = req.side
     vol = 4 * s ** 3
     sino = 4 * s * req.n_angles * (3 * s // 2)
-    pad = 4 * (s + 4) ** 2 * s * 2       # padded volume copies of the forward projector
+    pad = 8 * (s + 4) ** 2 * s * 2       # row-paired padded volume copies of the forward projector
+    ly = 4 * (s + 6) * req.n_angles * (3 * s // 2)  # length-weighted projections (padded rows)
     if req.op == "fp":
         return vol + sino + pad
     if req.op == "bp":
-        return vol + 2 * sino
-    return 2 * vol + 3 * sino + pad + (vol + sino)  # x, ones | y, r, l*r | C, R
+        return vol + sino + ly
+    return 2 * vol + 2 * sino + ly + pad + (vol + sino)  # x, ones | y, r, l*r | C, R
 
 
 def _input_data(req, torch, dev):
@@ -170,10 +171,32 @@ class _Job:
                 setattr(self, name, None)
 
 
-class Server:
-    """Continuous batching with HBM-budget admission on one HIP device."""
+def request_key(req):
+    """Geometry key of a request: requests with equal keys share an operator."""
+    return (req.kind, req.side, req.n_angles)
 
-    def __init__(self, hbm_budget_bytes=16 << 30, device=0, max_active=64, keep_results=False):
+
+def request_operator(req, operators):
+    """The operator of ``req`` from the cache ``operators`` (a dict keyed by
+    request_key), created and cached on first use."""
+    from .Operator import operator
+
+    key = request_key(req)
+    A = operators.get(key)
+    if A is None:
+        A = operator(*request_geometry(req))
+        operators[key] = A
+    return A
+
+
+class Server:
+    """Continuous batching with HBM-budget admission on one HIP device.
+
+    ``operators``: a dict of operators by request_key shared with other
+    servers (e.g. filled by warm_plans), so that a long-running server keeps
+    the plans of the shapes it has seen; None gives the server its own."""
+
+    def __init__(self, hbm_budget_bytes=16 << 30, device=0, max_active=64, keep_results=False, operators=None):
         import torch
 
         self.torch = torch
@@ -181,22 +204,14 @@ class Server:
         self.budget = int(hbm_budget_bytes)
         self.max_active = max_active
         self.keep_results = keep_results
-        self._ops = {}
+        self._ops = {} if operators is None else operators
         self._weights = {}
 
     def _operator(self, req):
-        from .Operator import operator
-
-        key = (req.kind, req.side, req.n_angles)
-        A = self._ops.get(key)
-        if A is None:
-            vg, pg = request_geometry(req)
-            A = operator(vg, pg)
-            self._ops[key] = A
-        return A
+        return request_operator(req, self._ops)
 
     def _sirt_weights(self, req, A):
-        key = (req.kind, req.side, req.n_angles)
+        key = request_key(req)
         w = self._weights.get(key)
         if w is None:
             w = sirt_weights(A, self.torch, self.dev)
@@ -322,34 +337,35 @@ class Server:
         return _summary(records, results, deferred)
 
 
-def warm_plans(requests, device=0):
-    """Create the device plan of every distinct request shape in ``requests``,
-    which autotunes the large ones once (the library caches the choices per
-    shape), so that no served request pays for the tuning of its shape."""
+def warm_plans(requests, device=0, operators=None):
+    """Create the operator and device plan of every distinct request shape in
+    ``requests``, which autotunes the large ones once (the library caches the
+    choices per shape), so that no served request pays for creating or tuning
+    the plan of its shape. The operators go into ``operators`` (a dict keyed by
+    request_key, to be passed to the servers) when given. Returns the number
+    of distinct shapes."""
     import torch
 
-    from .Operator import operator
-
+    ops = {} if operators is None else operators
     seen = set()
     with torch.cuda.device(torch.device("cuda", device)):
         for r in requests:
-            key = (r.kind, r.side, r.n_angles)
+            key = request_key(r)
             if key not in seen:
                 seen.add(key)
-                operator(*request_geometry(r)).projector.plan(device)
+                request_operator(r, ops).projector.plan(device)
         torch.cuda.synchronize()
     return len(seen)
 
 
-def serve_one_at_a_time(requests, device=0, keep_results=False):
+def serve_one_at_a_time(requests, device=0, keep_results=False, operators=None):
     """Reference server: arrival order, one request at a time, one launch per
     operator call, SIRT as tomosipo's own loop (its own R and C; separate
-    elementwise operations)."""
+    elementwise operations). ``operators`` as for Server."""
     import torch
 
-    from .Operator import operator
-
-    ops, records, results = {}, {}, {}
+    ops = {} if operators is None else operators
+    records, results = {}, {}
     dev = torch.device("cuda", device)
     with torch.cuda.device(dev):
         t0 = time.perf_counter()
@@ -358,11 +374,7 @@ def serve_one_at_a_time(requests, device=0, keep_results=False):
             if r.arrival > now:
                 time.sleep(r.arrival - now)
             start = time.perf_counter() - t0
-            key = (r.kind, r.side, r.n_angles)
-            if key not in ops:
-                vg, pg = request_geometry(r)
-                ops[key] = operator(vg, pg)
-            job = _Job(r, ops[key], torch, dev)
+            job = _Job(r, request_operator(r, ops), torch, dev)
             A = job.A
             if r.op == "fp":
                 A(job.inp, out=job.out)
