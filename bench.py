@@ -302,9 +302,9 @@ def autograd_batch_benchmark(torch, ts, reps=10):
 
 def sirt_benchmark(torch, ts, iterations=50):
     """256^3 unit-cube volume, cone beam with 360 angles over 2 pi, 256 x 384 detector of size
-    (sqrt 2, 1.5 sqrt 2), cone_angle 0.5: SIRT iterations captured as a HIP graph with the
-    fused residual / update kernels, against the eager loop of separate operator calls and
-    elementwise kernels (tomosipo's own loop)."""
+    (sqrt 2, 1.5 sqrt 2), cone_angle 0.5: SIRT iterations with the fused residual / update
+    kernels or as tomosipo's own loop (separate operator calls and elementwise kernels), each
+    captured as a HIP graph or run eagerly."""
     from tomosipo_amd.sirt import SIRT
 
     vg = ts.volume(shape=256, size=1)
@@ -317,6 +317,8 @@ def sirt_benchmark(torch, ts, iterations=50):
     y = A(x)
     res = {"workload": "SIRT, 256^3 cone, 360 angles, 256x384 detector", "iterations": iterations}
     for name, kw in (("graph_fused", dict(fused=True, graph=True)),
+                     ("eager_fused", dict(fused=True, graph=False)),
+                     ("graph_unfused", dict(fused=False, graph=True)),
                      ("eager_unfused", dict(fused=False, graph=False))):
         s = SIRT(A, y, fused=kw["fused"])
         s.run(1, graph=kw["graph"])          # warm-up (and graph capture)
