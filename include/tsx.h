@@ -109,7 +109,8 @@ typedef struct {
     int32_t bp_kernel;         /* 0 = general, 1 = separable (upright detector) */
     int32_t fp_variant;        /* padded forward projector: 2 = per-lane slice ranges,
                                   3 = wave-synchronous slices, 4 = as 3 on row-paired
-                                  layouts (one 16-byte load per sample) */
+                                  layouts (one 16-byte load per sample), 5 = as 3 with
+                                  each slice's window staged through LDS */
     int32_t bp_brick_z;        /* z-extent of the separable backprojector's bricks (16 or 32) */
     int32_t tuned;             /* 1 once tsx_plan_autotune has chosen the variants */
 } tsx_plan_info;

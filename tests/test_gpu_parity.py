@@ -118,11 +118,12 @@ def test_backprojector_short_detector(kind):
     _check_fp_bp(vg, pg, x)
 
 
-@pytest.mark.parametrize("fp_variant,brick_z", [("2", "32"), ("3", "16"), ("4", "32")])
+@pytest.mark.parametrize("fp_variant,brick_z", [("2", "32"), ("3", "16"), ("4", "32"), ("5", "32")])
 def test_autotuner_alternatives_match_oracle(monkeypatch, fp_variant, brick_z):
     """The kernel variants the autotuner may choose besides the defaults: the
-    per-lane forward march, the row-paired layout and 16-deep backprojector
-    bricks (forced through the environment overrides), on a helical cone_vec
+    per-lane forward march, the plain and row-paired layouts, the LDS-staged
+    march and 16-deep backprojector bricks (forced through the environment
+    overrides), on a helical cone_vec
     geometry with a volume depth that is not a multiple of the brick depth."""
     monkeypatch.setenv("TSX_FP_VARIANT", fp_variant)
     monkeypatch.setenv("TSX_BP_BRICK_Z", brick_z)
@@ -133,6 +134,57 @@ def test_autotuner_alternatives_match_oracle(monkeypatch, fp_variant, brick_z):
     info = A.projector.plan(0).info()
     assert info["fp_variant"] == int(fp_variant) and info["bp_brick_z"] == int(brick_z)
     _check_fp_bp(vg, pg, x)
+
+
+def _fp_with_variant(monkeypatch, A, x, variant):
+    monkeypatch.setenv("TSX_FP_VARIANT", variant)
+    y = A(x)
+    torch.cuda.synchronize()
+    return y
+
+
+@pytest.mark.parametrize("case", ["cone", "parallel", "coarse", "supersampled", "fullsize"])
+def test_lds_staged_march_equals_plain_march_bitwise(monkeypatch, case):
+    """The LDS-staged forward march (variant 5) computes the plain march's
+    (variant 3) samples, weights and sums in the same order, so the two agree
+    bit for bit: on a helical cone_vec scan, a parallel scan, detector pixels
+    four voxels wide (the block's slice window exceeds the LDS window and the
+    kernel falls back to the plain march), detector supersampling 2 (four
+    marches per block) and the 512^3 / 512 x 768 throughput geometry on 36 of
+    its angles. Fused launches of variant 5 equal single ones."""
+    from tomosipo_amd.fused import fused_forward
+
+    if case == "cone":
+        vg = ts.volume(shape=(44, 40, 36), size=1)
+        pg = _cone_vec_helix(30, 44, 60, np.sqrt(2), 1.5 * np.sqrt(2), 0.25, 1e-3, seed=7)
+    elif case == "parallel":
+        vg = ts.volume(shape=(40, 36, 44), size=(1.0, 0.9, 1.1))
+        pg = ts.parallel(angles=45, shape=(40, 60), size=(1.2, 1.8))
+    elif case == "coarse":
+        vg = ts.volume(shape=64, size=1)
+        pg = ts.cone(angles=24, shape=(16, 24), size=(np.sqrt(2), 1.5 * np.sqrt(2)), cone_angle=0.5)
+    elif case == "supersampled":
+        vg = ts.volume(shape=40, size=1)
+        pg = ts.cone(angles=20, shape=(40, 60), size=(np.sqrt(2), 1.5 * np.sqrt(2)), cone_angle=0.5)
+    else:
+        vg = ts.volume(shape=512, size=1)
+        pg = ts.cone(angles=720, shape=(512, 768), size=(np.sqrt(2), 1.5 * np.sqrt(2)),
+                     cone_angle=0.5).to_vec()[::20]
+    A = ts.operator(vg, pg, detector_supersampling=2 if case == "supersampled" else 1)
+    g = torch.Generator(device=DEV)
+    g.manual_seed(5)
+    x = torch.rand(A.domain_shape, generator=g, device=DEV)
+    print(f"[lds] {case}: vol {vg.shape}, det {pg.det_shape}, {pg.num_angles} angles", flush=True)
+    y3 = _fp_with_variant(monkeypatch, A, x, "3")
+    y5 = _fp_with_variant(monkeypatch, A, x, "5")
+    assert torch.equal(y3, y5), (y3 - y5).abs().max().item()
+    if case in ("cone", "parallel"):
+        ys = fused_forward([A, A], [x, x.flip(0).contiguous()])
+        assert torch.equal(ys[0], y5)
+        assert torch.equal(ys[1], A(x.flip(0).contiguous()))
+    if case != "fullsize":
+        go = oracle.oracle_geometry(vg, pg, det_supersampling=2 if case == "supersampled" else 1)
+        assert rel_l2(y5.cpu().numpy(), oracle.fp(go, x.double().cpu().numpy())) <= TOL
 
 
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])

@@ -473,6 +473,215 @@ __device__ __forceinline__ float march_padded_sync(__amdgpu_buffer_rsrc_t rs, co
     return valid ? r.len * (s0 + s1) : 0.0f;
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged march (variant 5, plain layouts). The block's rays are nearly
+// parallel, so at slice n their samples lie in a small window of the slice
+// (about 92 x 13 cells for a 64 x 8 tile at 512^3 / 512 x 768). Per slice the
+// block loads that window with coalesced 16-byte loads into LDS (double
+// buffered: slice n + 1 is loaded while slice n is sampled, one barrier per
+// slice) and every ray reads its four corners from LDS, so the gather leaves
+// the vector-memory address path. Samples outside a ray's own slice range are
+// skipped (the plain march adds an exact 0 for them), so the result equals
+// the plain march's (variant 3) bit for bit: same positions, weights,
+// interpolation and summation order (even and odd slices of the block's
+// range in two partial sums).
+//
+// Window. Per ray the sample position q_c(n) = R_c + (n - nref) a_c is
+// affine in n; the minimum over the block's rays is concave and the maximum
+// convex, so the chords between the block range's end slices bound them from
+// below and above at every slice of the range. The chords, widened by one
+// cell and clamped to the layout, are kept in 16.16 fixed point, so the
+// window origin of each slice is a few scalar integer operations. A block
+// whose window would exceed FW_ROWS x FW_W cells at either end of its range
+// (the window's size is affine in n) falls back to the plain march.
+constexpr int FW_ROWS = 20, FW_W = 128;
+constexpr int FW_WP = FW_W + 4;             // LDS row pitch (floats): 16-byte rows, rows 4 banks apart
+constexpr int FW_BUF = FW_ROWS * FW_WP;     // floats per window buffer
+constexpr int FW_SLOTS = 2;                 // staged 16-byte chunks per thread per slice (at most)
+static_assert(FW_ROWS * (FW_W / 4) <= FW_SLOTS * FP_BX * FP_BY, "window chunks per thread");
+
+struct FpLdsShared {
+    float win[2][FW_BUF];
+    float red[FP_BY][8];  // per wave: min/max of the two in-slice positions at both range ends
+};
+
+__device__ __forceinline__ float wave_minf(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_maxf(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// 16.16 fixed-point affine bound x(n) = (a + b (n - n0)) / 2^16 of a chord
+// from x0 (slice n0) to x1 (slice n0 + span); `up` rounds it upwards.
+struct FxLine {
+    int a, b;
+};
+
+__device__ __forceinline__ FxLine fx_line(float x0, float x1, int span, bool up)
+{
+    const double s = span > 0 ? ((double)x1 - (double)x0) / span : 0.0;
+    FxLine l;
+    l.a = (int)(up ? ceil((double)x0 * 65536.0) : floor((double)x0 * 65536.0));
+    l.b = (int)(up ? ceil(s * 65536.0) : floor(s * 65536.0));
+    l.a = __builtin_amdgcn_readfirstlane(l.a);
+    l.b = __builtin_amdgcn_readfirstlane(l.b);
+    return l;
+}
+
+// floor (up = false) or ceil of the bound at slice offset dn = n - n0
+__device__ __forceinline__ int fx_at(const FxLine &l, int dn, bool up)
+{
+    const int v = l.a + l.b * dn;
+    return up ? (v + 65535) >> 16 : v >> 16;
+}
+
+__device__ __forceinline__ float march_lds(__amdgpu_buffer_rsrc_t rs, const RaySetup &r, bool valid, int nref,
+                                           int n1, int n2, uint32_t st_m, uint32_t st_1, FpLdsShared &sh)
+{
+    int wlo = wave_min(valid ? r.nlo : 0x7fffffff);
+    int whi = wave_max(valid ? r.nhi : -1);
+    block_range(wlo, whi);  // block-uniform from here on
+    if (wlo > whi) return 0.0f;
+    const int span = whi - wlo;
+    const int w = threadIdx.y, lane = threadIdx.x, t = w * FP_BX + lane;
+    // chords of the block's extreme sample positions (padded coordinates)
+    {
+        const float d0 = (float)(wlo - nref), d1 = (float)(whi - nref);
+        const float p10 = valid ? fmaf(d0, r.a1, r.R1) : 1e30f, p11 = valid ? fmaf(d1, r.a1, r.R1) : 1e30f;
+        const float p20 = valid ? fmaf(d0, r.a2, r.R2) : 1e30f, p21 = valid ? fmaf(d1, r.a2, r.R2) : 1e30f;
+        const float m10 = wave_minf(p10), m11 = wave_minf(p11), m20 = wave_minf(p20), m21 = wave_minf(p21);
+        const float x10 = wave_maxf(valid ? p10 : -1e30f), x11 = wave_maxf(valid ? p11 : -1e30f);
+        const float x20 = wave_maxf(valid ? p20 : -1e30f), x21 = wave_maxf(valid ? p21 : -1e30f);
+        if (lane == 0) {
+            sh.red[w][0] = m10;
+            sh.red[w][1] = m11;
+            sh.red[w][2] = x10;
+            sh.red[w][3] = x11;
+            sh.red[w][4] = m20;
+            sh.red[w][5] = m21;
+            sh.red[w][6] = x20;
+            sh.red[w][7] = x21;
+        }
+    }
+    __syncthreads();
+    float e[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) e[k] = sh.red[0][k];
+#pragma unroll
+    for (int q = 1; q < FP_BY; ++q) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) e[k] = (k & 2) ? fmaxf(e[k], sh.red[q][k]) : fminf(e[k], sh.red[q][k]);
+    }
+    // the window lines, widened by one cell
+    bool fits = true;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) fits = fits && fabsf(e[k]) < 30000.0f;
+    FxLine lo1 = {0, 0}, hi1 = {0, 0}, lo2 = {0, 0}, hi2 = {0, 0};
+    int rows_w = 0, chunks_w = 0;
+    if (fits) {
+        lo1 = fx_line(e[0] - 1.0f, e[1] - 1.0f, span, false);
+        hi1 = fx_line(e[2] + 1.0f, e[3] + 1.0f, span, true);
+        lo2 = fx_line(e[4] - 1.0f, e[5] - 1.0f, span, false);
+        hi2 = fx_line(e[6] + 1.0f, e[7] + 1.0f, span, true);
+        // window size: the distance between the unclamped lines is affine in
+        // n, so its largest value is at an end of the range (+1 for
+        // rounding); the origins clamped to the layout only shrink it
+#pragma unroll
+        for (int end = 0; end < 2; ++end) {
+            const int dn = end ? span : 0;
+            const int o1 = fx_at(lo1, dn, false), u1 = fx_at(hi1, dn, true) + 1;
+            const int o2 = fx_at(lo2, dn, false) & ~3, u2 = fx_at(hi2, dn, true) + 1;
+            rows_w = max(rows_w, u1 - o1 + 2);
+            chunks_w = max(chunks_w, (u2 - o2 + 4) / 4 + 1);
+        }
+        fits = rows_w <= FW_ROWS && chunks_w * 4 <= FW_W;
+    }
+    fits = __builtin_amdgcn_readfirstlane((int)fits) != 0;
+    if (!fits) return march_padded_sync<false>(rs, r, valid, nref, n1, n2, st_m, st_1);
+    rows_w = __builtin_amdgcn_readfirstlane(rows_w);
+    chunks_w = __builtin_amdgcn_readfirstlane(chunks_w);
+    // staging slots of this thread: chunk c of window row rr, at most FW_SLOTS
+    uint32_t soff[FW_SLOTS];  // byte offset in the layout relative to the window origin
+    int sdst[FW_SLOTS];       // float index in a window buffer; -1: no slot
+#pragma unroll
+    for (int k = 0; k < FW_SLOTS; ++k) {
+        const int slot = t + k * FP_BX * FP_BY;
+        const int rr = slot / chunks_w, c = slot - rr * chunks_w;
+        const bool use = rr < rows_w;
+        soff[k] = use ? ((uint32_t)rr * st_1 + (uint32_t)c * 4u) * 4u : 0u;
+        sdst[k] = use ? rr * FW_WP + c * 4 : -1;
+    }
+    const uint32_t stepb = st_m * 4u, rowb = st_1 * 4u;
+    float4 stage[FW_SLOTS];
+    auto origin = [&](int n, int &o1, int &o2) {
+        o1 = max(fx_at(lo1, n - wlo, false), 0);
+        o2 = max(fx_at(lo2, n - wlo, false), 0) & ~3;
+    };
+    auto load = [&](int n) {
+        int o1, o2;
+        origin(n, o1, o2);
+        const uint32_t base = (uint32_t)n * stepb + (uint32_t)o1 * rowb + (uint32_t)o2 * 4u;
+#pragma unroll
+        for (int k = 0; k < FW_SLOTS; ++k)
+            if (sdst[k] >= 0)
+                stage[k] = __builtin_bit_cast(float4,
+                                              __builtin_amdgcn_raw_buffer_load_b128(rs, base + soff[k], 0, 0));
+    };
+    auto store = [&](int buf) {
+        float4 *wb = reinterpret_cast<float4 *>(sh.win[buf]);
+#pragma unroll
+        for (int k = 0; k < FW_SLOTS; ++k)
+            if (sdst[k] >= 0) wb[sdst[k] >> 2] = stage[k];
+    };
+    // one slice: window of slice n in buffer buf, sample into s
+    auto sample = [&](int n, int buf, float dn, float &s) {
+        if (valid && n >= r.nlo && n <= r.nhi) {
+            int o1, o2;
+            origin(n, o1, o2);
+            const float q1 = fmaf(dn, r.a1, r.R1), q2 = fmaf(dn, r.a2, r.R2);
+            const int k1 = floor_to_int(q1), k2 = floor_to_int(q2);
+            // byte offset of cell (k1, k2) in buffer buf: ((k1 - o1) FW_WP + k2 - o2 + buf FW_BUF) * 4
+            const uint32_t base = (uint32_t)(o1 * FW_WP + o2 - buf * FW_BUF) * 4u;  // uniform
+            const uint32_t a = __umul24((uint32_t)k1, (uint32_t)(FW_WP * 4)) + ((uint32_t)k2 << 2) - base;
+            const float *p = reinterpret_cast<const float *>(reinterpret_cast<const char *>(&sh.win[0][0]) + a);
+            Corners c;
+            c.v00 = p[0];
+            c.v01 = p[1];
+            c.v10 = p[FW_WP];
+            c.v11 = p[FW_WP + 1];
+            c.w1 = __builtin_amdgcn_fractf(q1);
+            c.w2 = __builtin_amdgcn_fractf(q2);
+            s += interp(c);
+        }
+    };
+    float s0 = 0.0f, s1 = 0.0f;  // slices wlo + even and wlo + odd
+    load(wlo);
+    store(0);
+    float dn = (float)(wlo - nref);
+    for (int n = wlo; n <= whi; n += 2) {
+        // even slice n (buffer 0), then odd slice n + 1 (buffer 1)
+        if (n < whi) load(n + 1);
+        __syncthreads();  // window of slice n complete; readers of buffer 1 done
+        sample(n, 0, dn, s0);
+        if (n < whi) store(1);
+        if (n + 1 > whi) break;
+        if (n + 1 < whi) load(n + 2);
+        __syncthreads();  // window of slice n + 1 complete; readers of buffer 0 done
+        sample(n + 1, 1, dn + 1.0f, s1);
+        if (n + 1 < whi) store(0);
+        dn += 2.0f;
+    }
+    return valid ? r.len * (s0 + s1) : 0.0f;
+}
+
 // One instance of the padded forward projector as seen by a block.
 struct FpView {
     const AngleParams *angles;
@@ -513,7 +722,11 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
             RaySetup r;
             const double fj = SS == 1 ? (double)j : j + (q + 0.5) / SS - 0.5;
             const double fi = SS == 1 ? (double)i : i + (p + 0.5) / SS - 0.5;
-            if (PIPE >= 3) {
+            if (PIPE == 5) {
+                __shared__ __align__(16) FpLdsShared lsh;
+                const bool ok = inside && ray_setup(A, fj, fi, (double)PAD, r);
+                acc += march_lds(rs, r, ok, A.n[0] / 2, A.n[1], A.n[2], st_m, st_1, lsh);
+            } else if (PIPE >= 3) {
                 // every lane of the block joins the march (block-wide range,
                 // barriers inside; uniform calls)
                 const bool ok = inside && ray_setup(A, fj, fi, (double)PAD, r);

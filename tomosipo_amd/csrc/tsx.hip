@@ -456,7 +456,7 @@ int env_int(const char *name)
 int fp_variant_of(const tsx_plan *p)
 {
     const int e = env_int("TSX_FP_VARIANT");
-    int v = (e >= 2 && e <= 4) ? e : (p->fp_variant >= 0 ? p->fp_variant : 4);
+    int v = (e >= 2 && e <= 5) ? e : (p->fp_variant >= 0 ? p->fp_variant : 4);
     if (v == 4 && !p->fp_pair) v = 3;
     return v;
 }
@@ -763,6 +763,8 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     case K:                                                                                        \
         if (variant == 4)                                                                          \
             TSX_FP_LAUNCH(K, 4);                                                                   \
+        else if (variant == 5)                                                                     \
+            TSX_FP_LAUNCH(K, 5);                                                                   \
         else if (variant == 3)                                                                     \
             TSX_FP_LAUNCH(K, 3);                                                                   \
         else                                                                                       \
@@ -949,6 +951,8 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         const unsigned grid = xcd_grid(m.begin[m.n], &m.shift);
         if (variant == 4)
             hipLaunchKernelGGL(fp_padded_multi<4>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
+        else if (variant == 5)
+            hipLaunchKernelGGL(fp_padded_multi<5>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
         else if (variant == 3)
             hipLaunchKernelGGL(fp_padded_multi<3>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
         else
@@ -1246,7 +1250,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
                                  stream, t.e0, t.e1, 2, ms);
         };
         double best_ms = 1e300;
-        for (int cand : {4, 3, 2}) {
+        for (int cand : {4, 5, 3, 2}) {
             if (cand == 4 && !p->fp_pair) continue;
             float ms_hi = 0.0f, ms_lo = 0.0f;
             int rc = time_band(cand, band_hi, &ms_hi);
