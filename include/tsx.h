@@ -41,6 +41,15 @@
  * one kernel launch takes (65535 per grid dimension, 2^32 work-items per
  * grid) are split over several launches inside the call; results do not
  * depend on the split.
+ *
+ * Size limits: the fast kernels address their scratch layouts with 32-bit
+ * byte offsets. A volume whose padded forward layouts would reach 4 GiB
+ * (e.g. 1024^3) is forward-projected in z-slabs, each slab's ray sums added to
+ * the projections; projections whose transposed, length-weighted copy would
+ * reach 4 GiB are backprojected in ranges of angles, each range's sums added
+ * to the volume. Chunked results equal the unchunked ones up to float32
+ * rounding of the extra additions. TSX_CHUNK_BYTES, read by tsx_plan_create,
+ * lowers the 4 GiB bound (for tests of these paths at small sizes).
  */
 #ifndef TSX_H
 #define TSX_H
@@ -113,6 +122,10 @@ typedef struct {
                                   each slice's window staged through LDS */
     int32_t bp_brick_z;        /* z-extent of the separable backprojector's bricks (16 or 32) */
     int32_t tuned;             /* 1 once tsx_plan_autotune has chosen the variants */
+    int32_t fp_slabs;          /* padded forward projector: z-slabs it projects the volume in
+                                  (1 = whole volume; more when the layouts reach 4 GiB) */
+    int32_t bp_launch_angles;  /* separable backprojector: angles per launch (all of them
+                                  unless their transposed projections reach 4 GiB) */
 } tsx_plan_info;
 
 int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);

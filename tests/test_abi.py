@@ -66,6 +66,47 @@ def test_kernel_selection():
                  ts.parallel(angles=4, shape=(8, 1024), size=(0.008, 1.5))).info()["fp_kernel"] == "padded"
 
 
+def test_fast_kernels_at_1024_cubed():
+    """A 1024^3 volume's row-paired layouts (8.6 GB per stepping axis) and the
+    transposed projections of 720 angles of a 1024 x 1536 detector (4.6 GB)
+    exceed the 32-bit offsets of the fast kernels; the plan keeps both on the
+    fast kernels, the forward projector in z-slabs (the deepest that fit,
+    balanced: 3 of 342 slices) and the backprojector in ranges of angles
+    (678 angles of 6.3 MB each stay below 4 GiB)."""
+    vg = ts.volume(shape=1024, size=1)
+    pg = ts.cone(angles=720, shape=(1024, 1536), size=(np.sqrt(2), 1.5 * np.sqrt(2)), cone_angle=0.5)
+    info = _plan(vg, pg).info()
+    assert info["fp_kernel"] == "padded" and info["bp_kernel"] == "separable"
+    # (d + 4) * 1024 * 1028 * 8 bytes < 2^32  ->  d <= 506  ->  3 slabs of 342
+    assert info["fp_slabs"] == 3
+    assert (2 ** 32 - 1) // ((1024 + 6) * 1536 * 4) == 678
+    assert info["bp_launch_angles"] == 678
+    # the 512^3 workload fits one slab and one launch
+    small = _plan(ts.volume(shape=512, size=1),
+                  ts.cone(angles=720, shape=(512, 768), size=(np.sqrt(2), 1.5 * np.sqrt(2)), cone_angle=0.5))
+    assert small.info()["fp_slabs"] == 1 and small.info()["bp_launch_angles"] == 720
+
+
+def test_chunk_limit_override(monkeypatch):
+    """TSX_CHUNK_BYTES lowers the bound at plan creation: a 48^3 cone plan
+    then takes 3 slabs of 16 (row-paired x layout of a 16-slice slab:
+    20 * 48 * 52 * 8 = 399360 bytes < 400000) and launches of 25 angles
+    (54 * 72 * 4 = 15552 bytes per angle)."""
+    vg = ts.volume(shape=48, size=1)
+    pg = ts.cone(angles=40, shape=(48, 72), size=(np.sqrt(2), 1.5 * np.sqrt(2)), cone_angle=0.5)
+    monkeypatch.setenv("TSX_CHUNK_BYTES", "400000")
+    info = _plan(vg, pg).info()
+    assert (info["fp_kernel"], info["fp_slabs"]) == ("padded", 3)
+    assert (info["bp_kernel"], info["bp_launch_angles"]) == ("separable", 25)
+    # one angle's projections beyond the bound: the general backprojector
+    monkeypatch.setenv("TSX_CHUNK_BYTES", "15000")
+    info = _plan(vg, pg).info()
+    assert info["bp_kernel"] == "general" and info["bp_launch_angles"] == 0
+    monkeypatch.delenv("TSX_CHUNK_BYTES")
+    info = _plan(vg, pg).info()
+    assert (info["fp_slabs"], info["bp_launch_angles"]) == (1, 40)
+
+
 @pytest.mark.parametrize("bad", ["nan", "zero_u", "parallel_uv", "zero_ray"])
 def test_invalid_geometry_raises_value_error(bad):
     vg = ts.volume(shape=8, size=1)

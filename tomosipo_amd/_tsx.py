@@ -96,6 +96,8 @@ class TsxPlanInfo(ctypes.Structure):
         ("fp_variant", ctypes.c_int32),
         ("bp_brick_z", ctypes.c_int32),
         ("tuned", ctypes.c_int32),
+        ("fp_slabs", ctypes.c_int32),
+        ("bp_launch_angles", ctypes.c_int32),
     ]
 
 
@@ -221,6 +223,7 @@ class Plan:
             "bp_kernel": ("general", "separable")[inf.bp_kernel],
             "fp_variant": inf.fp_variant, "bp_brick_z": inf.bp_brick_z,
             "tuned": bool(inf.tuned),
+            "fp_slabs": inf.fp_slabs, "bp_launch_angles": inf.bp_launch_angles,
         }
 
     def fdk_workspace_size(self):

@@ -233,17 +233,18 @@ __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_an
 }
 
 // Length-weighted projections for the separable backprojector, transposed so
-// that the rows of one detector column are contiguous:
-//   out[b][a][j][RPAD + i] = l(a, i, j) * proj[b][i][a][j],
+// that the rows of one detector column are contiguous, for the kn angles
+// sorted[0 .. kn - 1] (each with its index in the projections in `pad`):
+//   out[b][k][j][RPAD + i] = l(a, i, j) * proj[b][i][a][j],  a = sorted[k].pad,
 // each column padded with RPAD zero rows on both sides (rows -RPAD .. -1 and
 // rows .. rows + RPAD - 1), so that loads of four consecutive rows starting
 // anywhere in [-RPAD, rows - 1] stay inside the column. Block (x, y): tile of
 // 32 rows x 32 columns (x = row tile + tiles_i * column tile) of the lines y,
-// y + gridDim.y, ... of the n_angles * batch (entry, angle) pairs, through LDS.
+// y + gridDim.y, ... of the kn * batch (entry, angle) pairs, through LDS.
 constexpr int RPAD = 3;
 
 __global__ void __launch_bounds__(256)
-scale_by_length_t(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols,
+scale_by_length_t(const AngleParams *__restrict__ sorted, int kn, int n_angles, int rows, int cols,
                   const float *__restrict__ proj, int64_t proj_stride, float *__restrict__ out,
                   int64_t out_stride, int64_t lines)
 {
@@ -253,9 +254,10 @@ scale_by_length_t(const AngleParams *__restrict__ angles, int n_angles, int rows
     const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
     const int rp = rows + 2 * RPAD;
     for (int64_t line = blockIdx.y; line < lines; line += gridDim.y) {
-        const int64_t b = line / n_angles;
-        const int a = (int)(line - b * n_angles);
-        const AngleParams &A = angles[a];
+        const int64_t b = line / kn;
+        const int k = (int)(line - b * kn);
+        const AngleParams &A = sorted[k];
+        const int a = A.pad;
         __syncthreads();  // the tile of the previous line has been written out
         for (int r = ty; r < 32; r += 8) {
             const int i = i0 + r, j = j0 + tx;
@@ -268,7 +270,7 @@ scale_by_length_t(const AngleParams *__restrict__ angles, int n_angles, int rows
         for (int c = ty; c < 32; c += 8) {
             const int j = j0 + c;
             if (j >= cols) continue;
-            float *col = out + b * out_stride + ((int64_t)a * cols + j) * rp;
+            float *col = out + b * out_stride + ((int64_t)k * cols + j) * rp;
             if (i0 + tx < rows) col[RPAD + i0 + tx] = tile[tx][c];
             if (tx < RPAD) {
                 if (i0 == 0) col[tx] = 0.0f;
@@ -705,7 +707,7 @@ __device__ __forceinline__ void bp_wave_work(const BpView &v, int logical, BpSha
         const bool mx = am == 0;
         const int m0 = mx ? bx * BR : by * BR;
         const int c0 = mx ? by * BR : bx * BR;
-        const int a = A.pad;  // index of the angle in the projections
+        const int a = k;  // the angle's transposed projections in ly
         for (int c = 0; c < cur_chunks; ++c) {
             // pass 1: thread (slice ty, slot tx) splats detector column j onto kz
             {

@@ -27,8 +27,10 @@ __device__ __forceinline__ void emit(T *out, int64_t o, int64_t b, T acc, const 
     } else if (e.mode == EP_RESIDUAL) {
         const int64_t q = b * e.stride + o;
         out[o] = (T)(e.w[q] * (e.y[q] - (float)acc));
-    } else {
+    } else if (e.mode == EP_SCALED_ADD) {
         out[o] += (T)(e.w[b * e.stride + o] * (float)acc);
+    } else {
+        out[o] -= (T)(e.w[b * e.stride + o] * (float)acc);
     }
 }
 

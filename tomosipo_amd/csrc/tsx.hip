@@ -116,9 +116,19 @@ struct tsx_plan {
     int32_t axis_count[3] = {0, 0, 0};
     // padded per-axis layouts for the forward projector (elements per layout)
     int64_t layout_elems[3] = {0, 0, 0};
-    bool fp_padded = false;   // all used layouts addressable with 32-bit byte offsets
+    bool fp_padded = false;   // padded forward projector (on the whole volume or in slabs)
     bool fp_pair = false;     // ... also with row-paired (8-byte) elements
+    // largest byte size of one padded layout or of one backprojection launch's
+    // transposed projections (chunk_limit() when the plan was created)
+    int64_t chunk_bytes = (int64_t)1 << 32;
+    // z-slabs of the forward projector when the whole volume's layouts reach
+    // chunk_bytes: plans of the same rays over slices [slab_z0[k], slab_z0[k+1])
+    std::vector<tsx_plan *> slabs;
+    std::vector<int32_t> slab_z0;
+    bool is_slab = false;
+    int32_t fp_slab_depth = 0;  // slices per slab (nz: no slabs)
     bool bp_separable = false;  // upright detector, x/y stepping, ss == 1
+    int32_t bp_chunk = 0;       // separable backprojector: sorted angles per launch
     std::string bp_reason;      // why bp_separable is off
     // kernel variants chosen by tsx_plan_autotune (-1: library default)
     int32_t fp_variant = -1, bp_bz = -1;
@@ -356,19 +366,44 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
     p->layout_elems[0] = (n3[2] + P2) * n3[0] * (n3[1] + P2);
     p->layout_elems[1] = (n3[2] + P2) * n3[1] * (n3[0] + P2);
     p->layout_elems[2] = n3[2] * (n3[1] + P2) * (n3[0] + P2);
-    p->fp_padded = true;
     // elements between rows of c1 in each layout; the march forms row offsets
     // with 24-bit multiplies, so a row must stay below 2^24 bytes
     const int64_t row1[3] = {n3[0] * (n3[1] + P2), n3[1] * (n3[0] + P2), n3[0] + P2};
+    p->fp_padded = true;
     p->fp_pair = true;
     for (int c = 0; c < 3; ++c) {
         if (p->axis_count[c] == 0) continue;
-        if (p->layout_elems[c] * 4 >= ((int64_t)1 << 32) || row1[c] * 4 >= ((int64_t)1 << 24))
-            p->fp_padded = false;
-        if (p->layout_elems[c] * 8 >= ((int64_t)1 << 32) || row1[c] * 8 >= ((int64_t)1 << 24))
-            p->fp_pair = false;
+        if (row1[c] * 4 >= ((int64_t)1 << 24)) p->fp_padded = false;
+        if (row1[c] * 8 >= ((int64_t)1 << 24)) p->fp_pair = false;
     }
     p->fp_pair = p->fp_pair && p->fp_padded;
+    // The march addresses a layout with 32-bit byte offsets. A volume whose
+    // layouts reach chunk_bytes is projected in z-slabs of at most `depth`
+    // slices: the layouts of a slab of depth d hold (d + 2 PAD) K_c elements
+    // (x, y stepping) or d K_z (z stepping). The depth is chosen for row-paired
+    // elements where those fit, so that every march variant can take the slabs.
+    p->fp_slab_depth = p->nz;
+    if (p->fp_padded && !p->is_slab) {
+        const int64_t K[3] = {n3[0] * (n3[1] + P2), n3[1] * (n3[0] + P2), (n3[1] + P2) * (n3[0] + P2)};
+        auto deepest = [&](int es) {
+            int64_t d = n3[2];
+            for (int c = 0; c < 3; ++c)
+                if (p->axis_count[c] > 0)
+                    d = std::min(d, (p->chunk_bytes - 1) / (4 * es * K[c]) - (c < 2 ? P2 : 0));
+            return d;
+        };
+        int64_t d = deepest(p->fp_pair ? 2 : 1);
+        if (d < 1 && p->fp_pair) {
+            p->fp_pair = false;
+            d = deepest(1);
+        }
+        if (d < 1) {
+            p->fp_padded = false;
+        } else if (d < n3[2]) {
+            const int64_t n_slabs = (n3[2] + d - 1) / d;
+            p->fp_slab_depth = (int32_t)((n3[2] + n_slabs - 1) / n_slabs);
+        }
+    }
     // separable backprojection: upright detector (v along z), stepping along x or y
     p->bp_separable = g->det_supersampling == 1;
     if (!p->bp_separable) p->bp_reason = "detector supersampling";
@@ -376,9 +411,15 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
         p->bp_separable = false;
         p->bp_reason = "more than " + std::to_string(BP_RT_MAX_COLS) + " detector columns";
     }
-    if ((int64_t)(g->det_row_count + 2 * RPAD) * g->n_angles * g->det_col_count * 4 >= ((int64_t)1 << 32)) {
-        p->bp_separable = false;
-        p->bp_reason = "projection data of 4 GiB or more";
+    // transposed projections of one angle (the separable backprojector's
+    // workspace); a launch takes as many angles as stay below chunk_bytes
+    {
+        const int64_t per_angle = (int64_t)(g->det_row_count + 2 * RPAD) * g->det_col_count * 4;
+        p->bp_chunk = (int32_t)std::min<int64_t>(g->n_angles, (p->chunk_bytes - 1) / per_angle);
+        if (p->bp_chunk < 1) {
+            p->bp_separable = false;
+            p->bp_reason = "the projections of one angle reach " + std::to_string(p->chunk_bytes) + " bytes";
+        }
     }
     for (int a = 0; a < g->n_angles && p->bp_separable; ++a) {
         const AngleParams &A = p->host[a];
@@ -464,6 +505,35 @@ int fp_variant_of(const tsx_plan *p)
 // Floats per element of the forward projector's padded layouts.
 int fp_esize(const tsx_plan *p) { return fp_variant_of(p) == 4 ? 2 : 1; }
 
+// Largest byte size of one padded forward layout and of the transposed
+// projections of one backprojection launch: the kernels address them with
+// 32-bit byte offsets. TSX_CHUNK_BYTES, read when a plan is created, lowers
+// it (tests reach the slab and angle-chunk paths at small sizes with it).
+int64_t chunk_limit()
+{
+    const int64_t full = (int64_t)1 << 32;
+    const char *e = std::getenv("TSX_CHUNK_BYTES");
+    if (e && e[0]) {
+        const long long v = std::atoll(e);
+        if (v > 0 && v < full) return v;
+    }
+    return full;
+}
+
+// The epilogue of the second and later chunks (slabs of the volume, ranges
+// of angles) of a launch whose first chunk takes `e`: the chunks' sums add
+// up in the output.
+Epilogue later_chunk(Epilogue e)
+{
+    if (e.mode == EP_SET) {
+        e.mode = EP_ADD;
+    } else if (e.mode == EP_RESIDUAL) {
+        e.mode = EP_SCALED_SUB;
+        e.y = nullptr;
+    }
+    return e;
+}
+
 // z-extent of the separable backprojector's bricks: 32 (default) or 16.
 // TSX_BP_BRICK_Z overrides the plan's choice.
 int bp_brick_z_of(const tsx_plan *p)
@@ -514,21 +584,36 @@ void keep_pool_memory(int device)
     done[device] = true;
 }
 
-// Scratch bytes one launch of the fast forward / back projector needs.
+// Floats per element of the padded layouts of forward variant `variant`.
+int esize_of(int variant) { return variant == 4 ? 2 : 1; }
+
+// Scratch floats of one batch entry of the fast forward projector: the
+// padded layouts of the volume (of its largest slab when it is projected in
+// slabs, the slabs taking turns in the same memory).
 int64_t fp_workspace_elems(const tsx_plan *p)
 {
+    const int es = fp_esize(p);
     int64_t per_entry = 0;
-    for (int c = 0; c < 3; ++c)
-        if (p->axis_count[c] > 0) per_entry += p->layout_elems[c] * fp_esize(p);
+    for (const tsx_plan *q : p->slabs.empty() ? std::vector<const tsx_plan *>{p}
+                                              : std::vector<const tsx_plan *>(p->slabs.begin(), p->slabs.end())) {
+        int64_t e = 0;
+        for (int c = 0; c < 3; ++c)
+            if (q->axis_count[c] > 0) e += q->layout_elems[c] * es;
+        per_entry = std::max(per_entry, e);
+    }
     return per_entry;
 }
 
+// Sorted angles per launch of the separable backprojector.
+int bp_chunk_of(const tsx_plan *p) { return std::max(1, std::min(p->bp_chunk, p->n_angles)); }
+
 // Length-weighted projections of one batch entry: transposed and padded
-// ([angle][col][RPAD + row], scale_by_length_t) for the separable kernel,
-// in the projections' own layout for the general one.
+// ([sorted angle][col][RPAD + row], scale_by_length_t) for the separable
+// kernel, for the angles of one launch; in the projections' own layout for
+// the general one.
 int64_t bp_workspace_elems(const tsx_plan *p)
 {
-    if (p->bp_separable) return (int64_t)p->n_angles * p->cols * (p->rows + 2 * RPAD);
+    if (p->bp_separable) return (int64_t)bp_chunk_of(p) * p->cols * (p->rows + 2 * RPAD);
     return (int64_t)p->rows * p->n_angles * p->cols;
 }
 
@@ -617,15 +702,16 @@ int build_range_table(tsx_plan *p, hipStream_t stream, uint32_t **out)
     return TSX_OK;
 }
 
-// The transposed, padded length-weighted projections of `batch` entries.
-void launch_scale_by_length_t(const tsx_plan *p, const float *proj, int64_t ps, float *ly, int64_t ly_stride,
-                              int64_t batch, hipStream_t stream)
+// The transposed, padded length-weighted projections of the sorted angles
+// k0 .. k0 + kn - 1 of `batch` entries.
+void launch_scale_by_length_t(const tsx_plan *p, int k0, int kn, const float *proj, int64_t ps, float *ly,
+                              int64_t ly_stride, int64_t batch, hipStream_t stream)
 {
-    const int64_t lines = (int64_t)p->n_angles * batch;
+    const int64_t lines = (int64_t)kn * batch;
     const unsigned tiles = (unsigned)(((p->rows + 31) / 32) * ((p->cols + 31) / 32));
     dim3 grid(tiles, (unsigned)std::min(lines, GRID_DIM_MAX));
-    hipLaunchKernelGGL(scale_by_length_t, grid, dim3(32, 8), 0, stream, p->dev, p->n_angles, p->rows, p->cols,
-                       proj, ps, ly, ly_stride, lines);
+    hipLaunchKernelGGL(scale_by_length_t, grid, dim3(32, 8), 0, stream, p->dev + p->n_angles + k0, kn,
+                       p->n_angles, p->rows, p->cols, proj, ps, ly, ly_stride, lines);
 }
 
 // Either the caller's workspace (checked for size) or stream-ordered memory.
@@ -647,10 +733,10 @@ int get_workspace(const tsx_plan *p, int64_t bytes, void *ws, int64_t ws_bytes, 
 }
 
 // Padded copies of one volume for every stepping axis the plan uses.
-void build_layouts(const tsx_plan *p, const float *src, float *dst, const int64_t offs[3],
+void build_layouts(const tsx_plan *p, int es, const float *src, float *dst, const int64_t offs[3],
                    hipStream_t stream)
 {
-    const int pair = fp_esize(p) == 2;
+    const int pair = es == 2;
     if (p->axis_count[0] > 0) {
         dim3 grid((unsigned)((p->nx + 31) / 32), (unsigned)((p->ny + 2 * PAD + 31) / 32),
                   (unsigned)(p->nz + 2 * PAD));
@@ -672,12 +758,12 @@ void build_layouts(const tsx_plan *p, const float *src, float *dst, const int64_
 }
 
 // Offsets (in floats) of the layouts of one volume, and their total.
-void layout_offsets(const tsx_plan *p, int64_t offs[3], int64_t *per_entry)
+void layout_offsets(const tsx_plan *p, int es, int64_t offs[3], int64_t *per_entry)
 {
     int64_t e = 0;
     for (int c = 0; c < 3; ++c) {
         offs[c] = e;
-        if (p->axis_count[c] > 0) e += p->layout_elems[c] * fp_esize(p);
+        if (p->axis_count[c] > 0) e += p->layout_elems[c] * es;
     }
     *per_entry = e;
 }
@@ -704,25 +790,19 @@ int launch_fp_general(const tsx_plan *p, const T *vol, int64_t vs, T *proj, int6
     return TSX_OK;
 }
 
-int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int64_t ps,
-              int64_t batch, Epilogue ep, hipStream_t stream, void *wsp = nullptr,
-              int64_t ws_bytes = 0)
+// The padded forward projector of variant `variant` on the whole volume of
+// plan `p` (a plan without slabs, or one slab), its layouts built in `base`.
+int launch_fp_padded(const tsx_plan *p, int variant, const float *vol, int64_t vs, float *proj, int64_t ps,
+                     int64_t batch, Epilogue ep, float *base, hipStream_t stream)
 {
-    if (batch <= 0) return TSX_OK;
     const int tiles_u = (p->cols + FP_BX - 1) / FP_BX;
     const int tiles_v = (p->rows + FP_BY - 1) / FP_BY;
-    if (!p->fp_padded || force_general() || p->ss > 4)
-        return launch_fp_general<float>(p, vol, vs, proj, ps, batch, ep, stream);
+    const int es = esize_of(variant);
     // padded copies of every batch entry for each stepping axis in use
     int64_t per_entry = 0;
     int64_t offs[3] = {0, 0, 0};
-    layout_offsets(p, offs, &per_entry);
-    Scratch owned;
-    void *wptr = nullptr;
-    int rc = get_workspace(p, per_entry * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
-    if (rc) return rc;
-    float *base = (float *)wptr;
-    for (int64_t b = 0; b < batch; ++b) build_layouts(p, vol + b * vs, base + b * per_entry, offs, stream);
+    layout_offsets(p, es, offs, &per_entry);
+    for (int64_t b = 0; b < batch; ++b) build_layouts(p, es, vol + b * vs, base + b * per_entry, offs, stream);
     TSX_HIP_CHECK(hipGetLastError());
     // launches of whole batch entries while they fit the grid limit, else of
     // angle ranges of one entry
@@ -731,7 +811,6 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     const int64_t per_b = tiles * p->n_angles;
     const int64_t bc = std::max<int64_t>(1, cap / per_b);      // entries per launch
     const int na_max = (int)std::min<int64_t>(p->n_angles, std::max<int64_t>(1, cap / tiles));
-    const int variant = fp_variant_of(p);
     for (int64_t b0 = 0; b0 < batch; b0 += bc) {
         const int64_t nbat = std::min(bc, batch - b0);
         for (int a0 = 0; a0 < p->n_angles; a0 += na_max) {
@@ -747,7 +826,7 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
             v.batch = (int)nbat;
             for (int c = 0; c < 3; ++c) {
                 v.L.ptr[c] = p->axis_count[c] > 0 ? base + b0 * per_entry + offs[c] : nullptr;
-                v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4 * fp_esize(p));
+                v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4 * es);
             }
             v.L.stride = per_entry;
             v.proj = proj + b0 * ps;
@@ -785,15 +864,42 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     return TSX_OK;
 }
 
+// Forward projection of `batch` volumes: the padded kernels on the whole
+// volume or slab by slab (each slab's sums added to the projections of the
+// previous ones), else the general kernel.
+int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int64_t ps,
+              int64_t batch, Epilogue ep, hipStream_t stream, void *wsp = nullptr,
+              int64_t ws_bytes = 0)
+{
+    if (batch <= 0) return TSX_OK;
+    if (!p->fp_padded || force_general() || p->ss > 4)
+        return launch_fp_general<float>(p, vol, vs, proj, ps, batch, ep, stream);
+    const int variant = fp_variant_of(p);
+    Scratch owned;
+    void *wptr = nullptr;
+    int rc = get_workspace(p, fp_workspace_elems(p) * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
+    if (rc) return rc;
+    if (p->slabs.empty())
+        return launch_fp_padded(p, variant, vol, vs, proj, ps, batch, ep, (float *)wptr, stream);
+    for (size_t k = 0; k < p->slabs.size(); ++k) {
+        rc = launch_fp_padded(p->slabs[k], variant, vol + (int64_t)p->slab_z0[k] * p->ny * p->nx, vs, proj, ps,
+                              batch, k == 0 ? ep : later_chunk(ep), (float *)wptr, stream);
+        if (rc) return rc;
+    }
+    return TSX_OK;
+}
+
+// The separable backprojector over the sorted angles k0 .. k0 + kn - 1, their
+// transposed projections in `ly` (batch entries ly_bytes apart).
 BpView bp_view(const tsx_plan *p, int BZ, float *vol, int64_t vs, const float *ly, uint32_t ly_bytes,
-               int batch, Epilogue ep)
+               int batch, Epilogue ep, int k0, int kn)
 {
     BpView v;
     v.angles = p->dev;
-    v.sorted = p->dev + p->n_angles;
-    v.rt = p->bp_rt;
+    v.sorted = p->dev + p->n_angles + k0;
+    v.rt = p->bp_rt + (int64_t)k0 * p->bp_rt_stride;
     v.rt_stride = p->bp_rt_stride;
-    v.n_angles = p->n_angles;
+    v.n_angles = kn;
     v.rows = p->rows;
     v.cols = p->cols;
     v.nx = p->nx;
@@ -849,32 +955,37 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
         }
         return launch_bp_general<float, false>(p, vol, vs, proj, ps, batch, ep, stream);
     }
-    const int64_t sino_t = bp_workspace_elems(p);  // transposed, padded
-    if (sino_t * 4 >= ((int64_t)1 << 32))
-        return fail(TSX_ERR_UNSUPPORTED, "projection data of 4 GiB or more per entry");
+    // transposed, padded projections of the angles of one launch (all of them
+    // unless they reach chunk_bytes: then ranges of sorted angles, each range's
+    // sums added to the volume of the previous ones)
+    const int kc = bp_chunk_of(p);
+    const int64_t sino_t = bp_workspace_elems(p);
     Scratch owned;
     void *wptr = nullptr;
     int rc = get_workspace(p, sino_t * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
     if (rc) return rc;
     float *ly = (float *)wptr;
-    launch_scale_by_length_t(p, proj, ps, ly, sino_t, batch, stream);
     const int BZ = bp_brick_z_of(p);
-    BpView v0 = bp_view(p, BZ, vol, vs, ly, (uint32_t)(sino_t * 4), 1, ep);
-    const int64_t bricks = (int64_t)v0.bricks_x * v0.bricks_y * v0.bricks_z;
+    const int64_t bricks = (int64_t)((p->nx + BR - 1) / BR) * ((p->ny + BR - 1) / BR) * ((p->nz + BZ - 1) / BZ);
     if (bricks > max_blocks(BR * BR)) return fail(TSX_ERR_UNSUPPORTED, "volume too large for one launch");
     const int64_t bc = max_blocks(BR * BR) / bricks;  // entries per launch
-    for (int64_t b0 = 0; b0 < batch; b0 += bc) {
-        const int64_t nbat = std::min(bc, batch - b0);
-        BpView v = bp_view(p, BZ, vol + b0 * vs, vs, ly + b0 * sino_t, (uint32_t)(sino_t * 4), (int)nbat,
-                           ep_from(ep, b0));
-        const int n_blocks = (int)(bricks * nbat);
-        int shift;
-        const unsigned grid = xcd_grid(n_blocks, &shift);
-        if (BZ == 32)
-            hipLaunchKernelGGL(bp_wave<32>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
-        else
-            hipLaunchKernelGGL(bp_wave<16>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
-        TSX_HIP_CHECK(hipGetLastError());
+    for (int k0 = 0; k0 < p->n_angles; k0 += kc) {
+        const int kn = std::min(kc, p->n_angles - k0);
+        const Epilogue e = k0 == 0 ? ep : later_chunk(ep);
+        launch_scale_by_length_t(p, k0, kn, proj, ps, ly, sino_t, batch, stream);
+        for (int64_t b0 = 0; b0 < batch; b0 += bc) {
+            const int64_t nbat = std::min(bc, batch - b0);
+            BpView v = bp_view(p, BZ, vol + b0 * vs, vs, ly + b0 * sino_t, (uint32_t)(sino_t * 4), (int)nbat,
+                               ep_from(e, b0), k0, kn);
+            const int n_blocks = (int)(bricks * nbat);
+            int shift;
+            const unsigned grid = xcd_grid(n_blocks, &shift);
+            if (BZ == 32)
+                hipLaunchKernelGGL(bp_wave<32>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
+            else
+                hipLaunchKernelGGL(bp_wave<16>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
+            TSX_HIP_CHECK(hipGetLastError());
+        }
     }
     return TSX_OK;
 }
@@ -967,7 +1078,7 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         const int64_t wb = align_up(workspace_bytes(p, TSX_OP_FP, 1));
         const int tiles_u = (p->cols + FP_BX - 1) / FP_BX, tiles_v = (p->rows + FP_BY - 1) / FP_BY;
         const int64_t blocks = (int64_t)tiles_u * tiles_v * p->n_angles;
-        if (!(fast && p->fp_padded && p->ss == 1) || blocks > cap) {
+        if (!(fast && p->fp_padded && p->ss == 1 && p->slabs.empty()) || blocks > cap) {
             // this instance takes its own launch(es)
             int rc = launch_fp(p, vols[k], 0, projs[k], 0, 1, eps[k], stream, wptr ? (char *)wptr + woff : nullptr, wb);
             if (rc) return rc;
@@ -979,11 +1090,12 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
             if (rc) return rc;
         }
         variant = fp_variant_of(p);
+        const int es = esize_of(variant);
         int64_t offs[3], per_entry;
-        layout_offsets(p, offs, &per_entry);
+        layout_offsets(p, es, offs, &per_entry);
         float *base = (float *)((char *)wptr + woff);
         woff += wb;
-        build_layouts(p, vols[k], base, offs, stream);
+        build_layouts(p, es, vols[k], base, offs, stream);
         FpView &v = m.v[m.n];
         v.angles = p->dev;
         v.n_angles = p->n_angles;
@@ -996,7 +1108,7 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         v.batch = 1;
         for (int c = 0; c < 3; ++c) {
             v.L.ptr[c] = p->axis_count[c] > 0 ? base + offs[c] : nullptr;
-            v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4 * fp_esize(p));
+            v.L.bytes[c] = (uint32_t)(p->layout_elems[c] * 4 * es);
         }
         v.L.stride = 0;
         v.proj = projs[k];
@@ -1047,7 +1159,7 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         const int64_t wb = align_up(workspace_bytes(p, TSX_OP_BP, 1));
         const int bz = bp_brick_z_of(p);
         const int64_t blocks = (int64_t)((p->nx + BR - 1) / BR) * ((p->ny + BR - 1) / BR) * ((p->nz + bz - 1) / bz);
-        if (!(fast && p->bp_separable) || blocks > cap) {
+        if (!(fast && p->bp_separable && bp_chunk_of(p) == p->n_angles) || blocks > cap) {
             int rc = launch_bp(p, vols[k], 0, projs[k], 0, 1, eps[k], stream, wptr ? (char *)wptr + woff : nullptr, wb);
             if (rc) return rc;
             woff += wb;
@@ -1061,8 +1173,8 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         float *ly = (float *)((char *)wptr + woff);
         woff += wb;
         const int64_t sino_t = bp_workspace_elems(p);
-        launch_scale_by_length_t(p, projs[k], 0, ly, sino_t, 1, stream);
-        m.v[m.n] = bp_view(p, BZ, vols[k], 0, ly, (uint32_t)(sino_t * 4), 1, eps[k]);
+        launch_scale_by_length_t(p, 0, p->n_angles, projs[k], 0, ly, sino_t, 1, stream);
+        m.v[m.n] = bp_view(p, BZ, vols[k], 0, ly, (uint32_t)(sino_t * 4), 1, eps[k], 0, p->n_angles);
         m.begin[m.n + 1] = m.begin[m.n] + (int)blocks;
         m.n += 1;
         if (m.n == MAX_FUSED) {
@@ -1183,6 +1295,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
     sub.fp_padded = p->fp_padded;
     sub.fp_pair = p->fp_pair;
     sub.bp_separable = p->bp_separable;
+    sub.bp_chunk = p->bp_chunk;
     TuneScratch t;
     const size_t vol_bytes = (size_t)p->nx * p->ny * p->nz * 4;
     const size_t proj_bytes = (size_t)p->rows * sub.n_angles * p->cols * 4;
@@ -1220,20 +1333,22 @@ int autotune(tsx_plan *p, hipStream_t stream)
         // the padded layouts, larger for the row-paired ones) plus a part per
         // row; both are fitted from the two bands and the candidates compared
         // on the predicted time of the whole plan.
+        // (a plan projected in z-slabs is timed on its first slab)
+        const tsx_plan *fpp = p->slabs.empty() ? p : p->slabs[0];
         const int band_hi = std::min(p->rows, 64), band_lo = std::min(p->rows, 32);
-        std::vector<AngleParams> shifted(p->host);
+        std::vector<AngleParams> shifted(fpp->host);
         tsx_plan band;
         band.device = p->device;
         band.kind = p->kind;
         band.cols = p->cols;
-        band.nx = p->nx;
-        band.ny = p->ny;
-        band.nz = p->nz;
+        band.nx = fpp->nx;
+        band.ny = fpp->ny;
+        band.nz = fpp->nz;
         band.ss = p->ss;
         band.n_angles = p->n_angles;
         for (int c = 0; c < 3; ++c) {
-            band.axis_count[c] = p->axis_count[c];
-            band.layout_elems[c] = p->layout_elems[c];
+            band.axis_count[c] = fpp->axis_count[c];
+            band.layout_elems[c] = fpp->layout_elems[c];
         }
         band.fp_padded = p->fp_padded;
         band.fp_pair = p->fp_pair;
@@ -1241,7 +1356,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
         auto time_band = [&](int cand, int band_rows, float *ms) -> int {
             const int r0 = (p->rows - band_rows) / 2;
             for (int a = 0; a < p->n_angles; ++a)
-                for (int k = 0; k < 3; ++k) shifted[a].P0[k] = p->host[a].P0[k] + r0 * p->host[a].V[k];
+                for (int k = 0; k < 3; ++k) shifted[a].P0[k] = fpp->host[a].P0[k] + r0 * fpp->host[a].V[k];
             TSX_HIP_CHECK(hipMemcpyAsync(t.band_angles, shifted.data(), sizeof(AngleParams) * shifted.size(),
                                          hipMemcpyHostToDevice, stream));
             band.rows = band_rows;
@@ -1288,6 +1403,55 @@ int autotune(tsx_plan *p, hipStream_t stream)
     p->bp_bz = best.bp_bz;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = best;
+    return TSX_OK;
+}
+
+// The z-slabs of a plan whose forward layouts reach chunk_bytes: plans of the
+// same rays over slices [z0, z0 + depth) of the volume (the window's z range
+// cut at the slab's faces), each with its angle table on `device` (none for
+// a host-only plan). The parent's choice of row pairing holds for the slabs.
+int build_slabs(const tsx_geometry *g, tsx_plan *p, int device)
+{
+    if (!p->fp_padded || p->fp_slab_depth >= p->nz) return TSX_OK;
+    const double hz = (g->window_max[2] - g->window_min[2]) / (double)p->nz;
+    for (int z0 = 0; z0 < p->nz; z0 += p->fp_slab_depth) {
+        const int d = std::min(p->fp_slab_depth, p->nz - z0);
+        tsx_geometry sg = *g;
+        sg.grid_slice_count = d;
+        sg.window_min[2] = g->window_min[2] + z0 * hz;
+        sg.window_max[2] = z0 + d == p->nz ? g->window_max[2] : g->window_min[2] + (z0 + d) * hz;
+        auto *s = new tsx_plan();
+        p->slabs.push_back(s);
+        p->slab_z0.push_back(z0);
+        s->kind = p->kind;
+        s->n_angles = p->n_angles;
+        s->rows = p->rows;
+        s->cols = p->cols;
+        s->nx = p->nx;
+        s->ny = p->ny;
+        s->nz = d;
+        s->ss = p->ss;
+        s->chunk_bytes = p->chunk_bytes;
+        s->is_slab = true;
+        int rc = build_angles(&sg, s);
+        if (rc) return rc;
+        s->fp_padded = p->fp_padded;
+        s->fp_pair = p->fp_pair;
+        s->fdk_host.clear();
+        if (device >= 0) {
+            const std::vector<AngleParams> table = device_angle_table(s->host);
+            const size_t bytes = sizeof(AngleParams) * table.size();
+            hipError_t e = hipMalloc((void **)&s->dev, bytes);
+            if (e != hipSuccess) {
+                s->dev = nullptr;
+                return fail(TSX_ERR_OUT_OF_MEMORY, std::string("slab angle table: ") + hipGetErrorString(e));
+            }
+            s->device = device;
+            e = hipMemcpy(s->dev, table.data(), bytes, hipMemcpyHostToDevice);
+            if (e != hipSuccess) return fail(TSX_ERR_HIP, std::string("slab angle table: ") + hipGetErrorString(e));
+        }
+    }
+    p->slab_z0.push_back(p->nz);
     return TSX_OK;
 }
 
@@ -1339,6 +1503,7 @@ int tsx_plan_create(const tsx_geometry *g, int device, tsx_plan **out)
     p->ny = g->grid_row_count;
     p->nz = g->grid_slice_count;
     p->ss = g->det_supersampling;
+    p->chunk_bytes = chunk_limit();
     int rc = build_angles(g, p);
     if (rc != TSX_OK) {
         delete p;
@@ -1392,6 +1557,18 @@ int tsx_plan_create(const tsx_geometry *g, int device, tsx_plan **out)
         }
         p->device = device;
     }
+    if (device >= 0) {
+        DeviceGuard guard(device);
+        rc = build_slabs(g, p, device);
+    } else {
+        rc = build_slabs(g, p, -1);
+    }
+    if (rc) {
+        const std::string msg = g_last_error;
+        tsx_plan_destroy(p);
+        g_last_error = msg;
+        return rc;
+    }
     *out = p;
     return TSX_OK;
 }
@@ -1399,6 +1576,7 @@ int tsx_plan_create(const tsx_geometry *g, int device, tsx_plan **out)
 int tsx_plan_destroy(tsx_plan *p)
 {
     if (!p) return TSX_OK;
+    for (tsx_plan *s : p->slabs) tsx_plan_destroy(s);
     if (p->dev) {
         DeviceGuard guard(p->device);
         (void)hipFree(p->dev);
@@ -1419,7 +1597,8 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->vol_shape[0] = p->nz;
     info->vol_shape[1] = p->ny;
     info->vol_shape[2] = p->nx;
-    info->plan_bytes = p->dev ? (int64_t)(sizeof(AngleParams) * 2 * p->host.size()) +
+    info->plan_bytes = p->dev ? (int64_t)(sizeof(AngleParams) * 2 * p->host.size()) *
+                                        (1 + (int64_t)p->slabs.size()) +
                                     (p->bp_rt ? p->bp_rt_stride * p->n_angles * 4 : 0)
                               : 0;
     for (int c = 0; c < 3; ++c) info->axis_count[c] = p->axis_count[c];
@@ -1428,6 +1607,8 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->fp_variant = fp_variant_of(p);
     info->bp_brick_z = bp_brick_z_of(p);
     info->tuned = p->fp_variant >= 0 || p->bp_bz > 0;
+    info->fp_slabs = info->fp_kernel ? std::max<int32_t>(1, (int32_t)p->slabs.size()) : 0;
+    info->bp_launch_angles = info->bp_kernel ? bp_chunk_of(p) : 0;
     return TSX_OK;
 }
 

@@ -44,12 +44,13 @@ struct FdkAngle {
 
 // How a kernel writes its result `acc` to element o of batch entry b.
 struct Epilogue {
-    int32_t mode;    // 0: out = acc; 1: out += acc; 2: out = w * (y - acc); 3: out += w * acc
+    int32_t mode;    // 0: out = acc; 1: out += acc; 2: out = w * (y - acc); 3: out += w * acc;
+                     // 4: out -= w * acc (later chunks of a chunked mode-2 launch)
     const float *y;  // mode 2: data term, same layout as the output
     const float *w;  // modes 2 and 3: elementwise weight, same layout as the output
     int64_t stride;  // elements between batch entries of y and w
 };
 
-enum { EP_SET = 0, EP_ADD = 1, EP_RESIDUAL = 2, EP_SCALED_ADD = 3 };
+enum { EP_SET = 0, EP_ADD = 1, EP_RESIDUAL = 2, EP_SCALED_ADD = 3, EP_SCALED_SUB = 4 };
 
 }  // namespace tsx
