@@ -332,6 +332,13 @@ def sirt_benchmark(torch, ts, iterations=50):
     return res
 
 
+def kernel_choices(A):
+    """The autotuner's choices for an operator's device plan."""
+    info = A.projector.plan(0).info()
+    return {k: info[k] for k in ("fp_kernel", "fp_variant", "fp_tile_rows", "fp_slabs", "bp_kernel",
+                                 "bp_brick_z", "bp_launch_angles", "tuned")}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -443,6 +450,7 @@ def main():
             "bp_ms_per_instance": round(bp_ms, 3),
             "gups_fp": round(N_VOX ** 3 * N_ANGLES / (fp_ms * 1e-3) / 1e9, 2),
             "gups_bp": round(N_VOX ** 3 * N_ANGLES / (bp_ms * 1e-3) / 1e9, 2),
+            "kernels": kernel_choices(ops[0]),
         },
         "roofline": roofline,
         "roofline_gather": {

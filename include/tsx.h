@@ -120,12 +120,13 @@ typedef struct {
                                   3 = wave-synchronous slices, 4 = as 3 on row-paired
                                   layouts (one 16-byte load per sample), 5 = as 3 with
                                   each slice's window staged through LDS */
-    int32_t bp_brick_z;        /* z-extent of the separable backprojector's bricks (16 or 32) */
+    int32_t bp_brick_z;        /* z-extent of the separable backprojector's bricks (16, 24 or 32) */
     int32_t tuned;             /* 1 once tsx_plan_autotune has chosen the variants */
     int32_t fp_slabs;          /* padded forward projector: z-slabs it projects the volume in
                                   (1 = whole volume; more when the layouts reach 4 GiB) */
     int32_t bp_launch_angles;  /* separable backprojector: angles per launch (all of them
                                   unless their transposed projections reach 4 GiB) */
+    int32_t fp_tile_rows;      /* padded forward projector: detector rows per block (4, 8, 16) */
 } tsx_plan_info;
 
 int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);
@@ -133,17 +134,19 @@ int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);
 /* No counterpart in the reference (ASTRA has one kernel per geometry type).
  * Times the kernel variants of the plan's forward projector and
  * backprojector on its own geometry and keeps the fastest for later launches
- * of this plan: forward variants on every angle over bands of 64 and 32
+ * of this plan: the forward march (row-paired, LDS-staged, plain or per-lane
+ * on tiles of 8 detector rows, then tiles of 4 and 16 rows for the chosen
+ * march where it takes them) on every angle over bands of 64 and 32
  * detector rows, compared on the predicted time of the whole plan (fixed
- * layout-building cost plus cost per row); backprojector brick depths on at
- * most 64 of its angles, in runs of consecutive angles from four quadrants of
- * the scan. Trials use zero-filled scratch buffers allocated and freed on
- * `stream`. Choices are cached per (type, detector, volume, angle count
- * rounded up to a power of two) shape, so a second plan of the same shape
- * takes them without timing. Synchronises `stream`. Must not run while
- * launches of the same plan are being issued from another thread. The
- * TSX_FP_VARIANT and TSX_BP_BRICK_Z environment variables, when set,
- * override any choice. */
+ * layout-building cost plus cost per row); the backprojector's brick depth
+ * (32, 24 or 16 slices) on at most 64 of its angles, in runs of consecutive
+ * angles from four quadrants of the scan. Trials use zero-filled scratch
+ * buffers allocated and freed on `stream`. Choices are cached per (type,
+ * detector, volume, angle count rounded up to a power of two) shape, so a
+ * second plan of the same shape takes them without timing. Synchronises
+ * `stream`. Must not run while launches of the same plan are being issued
+ * from another thread. The TSX_FP_VARIANT, TSX_FP_TILE_ROWS and
+ * TSX_BP_BRICK_Z environment variables, when set, override any choice. */
 int tsx_plan_autotune(tsx_plan *plan, void *stream);
 
 /* Replaces astra.experimental.accumulate_FDK(projector, vol, proj)

@@ -98,6 +98,7 @@ class TsxPlanInfo(ctypes.Structure):
         ("tuned", ctypes.c_int32),
         ("fp_slabs", ctypes.c_int32),
         ("bp_launch_angles", ctypes.c_int32),
+        ("fp_tile_rows", ctypes.c_int32),
     ]
 
 
@@ -224,6 +225,7 @@ class Plan:
             "fp_variant": inf.fp_variant, "bp_brick_z": inf.bp_brick_z,
             "tuned": bool(inf.tuned),
             "fp_slabs": inf.fp_slabs, "bp_launch_angles": inf.bp_launch_angles,
+            "fp_tile_rows": inf.fp_tile_rows,
         }
 
     def fdk_workspace_size(self):

@@ -243,6 +243,36 @@ __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_an
 // y + gridDim.y, ... of the kn * batch (entry, angle) pairs, through LDS.
 constexpr int RPAD = 3;
 
+// One (tile, angle) pair of scale_by_length_t: the tile (i0, j0) of sorted
+// angle k of one batch entry (proj and out moved to the entry).
+__device__ __forceinline__ void scale_t_tile(const AngleParams *__restrict__ sorted, int k, int n_angles,
+                                             int rows, int cols, const float *__restrict__ proj,
+                                             float *__restrict__ out, int i0, int j0, float (&tile)[32][33])
+{
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    const int rp = rows + 2 * RPAD;
+    const AngleParams &A = sorted[k];
+    const int a = A.pad;
+    __syncthreads();  // the tile of the block's previous pair has been written out
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, j = j0 + tx;
+        float v = 0.0f;
+        if (i < rows && j < cols) v = ray_length(A, i, j) * proj[((int64_t)i * n_angles + a) * cols + j];
+        tile[r][tx] = v;
+    }
+    __syncthreads();
+    for (int c = ty; c < 32; c += 8) {
+        const int j = j0 + c;
+        if (j >= cols) continue;
+        float *col = out + ((int64_t)k * cols + j) * rp;
+        if (i0 + tx < rows) col[RPAD + i0 + tx] = tile[tx][c];
+        if (tx < RPAD) {
+            if (i0 == 0) col[tx] = 0.0f;
+            if (i0 + 32 >= rows) col[RPAD + rows + tx] = 0.0f;
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256)
 scale_by_length_t(const AngleParams *__restrict__ sorted, int kn, int n_angles, int rows, int cols,
                   const float *__restrict__ proj, int64_t proj_stride, float *__restrict__ out,
@@ -251,33 +281,50 @@ scale_by_length_t(const AngleParams *__restrict__ sorted, int kn, int n_angles, 
     __shared__ float tile[32][33];
     const int tiles_i = (rows + 31) / 32;
     const int i0 = (int)(blockIdx.x % tiles_i) * 32, j0 = (int)(blockIdx.x / tiles_i) * 32;
-    const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
-    const int rp = rows + 2 * RPAD;
     for (int64_t line = blockIdx.y; line < lines; line += gridDim.y) {
         const int64_t b = line / kn;
         const int k = (int)(line - b * kn);
-        const AngleParams &A = sorted[k];
-        const int a = A.pad;
-        __syncthreads();  // the tile of the previous line has been written out
-        for (int r = ty; r < 32; r += 8) {
-            const int i = i0 + r, j = j0 + tx;
-            float v = 0.0f;
-            if (i < rows && j < cols)
-                v = ray_length(A, i, j) * proj[b * proj_stride + ((int64_t)i * n_angles + a) * cols + j];
-            tile[r][tx] = v;
-        }
-        __syncthreads();
-        for (int c = ty; c < 32; c += 8) {
-            const int j = j0 + c;
-            if (j >= cols) continue;
-            float *col = out + b * out_stride + ((int64_t)k * cols + j) * rp;
-            if (i0 + tx < rows) col[RPAD + i0 + tx] = tile[tx][c];
-            if (tx < RPAD) {
-                if (i0 == 0) col[tx] = 0.0f;
-                if (i0 + 32 >= rows) col[RPAD + rows + tx] = 0.0f;
-            }
-        }
+        scale_t_tile(sorted, k, n_angles, rows, cols, proj + b * proj_stride, out + b * out_stride, i0, j0, tile);
     }
+}
+
+// scale_by_length_t of several plans' projections in one launch: job k is
+// row y = k of the grid, whose blocks x < blocks[k] each take one (tile,
+// angle) pair (the rest of the row returns at once; no table lookup in
+// these short blocks).
+struct ScaleJob {
+    const AngleParams *sorted;
+    int kn, n_angles, rows, cols;
+    const float *proj;
+    float *out;
+    int blocks;
+};
+
+struct ScaleJobs {
+    ScaleJob job[MAX_FUSED];
+    int n;
+    uint32_t *zero;  // if not null: CLAIM_COUNTERS counters set to 0 (claims of the next fused launch)
+};
+
+__host__ __device__ inline int64_t scale_job_blocks(const ScaleJob &j)
+{
+    return (int64_t)((j.rows + 31) / 32) * ((j.cols + 31) / 32) * j.kn;
+}
+
+__global__ void __launch_bounds__(256) scale_by_length_t_multi(ScaleJobs S)
+{
+    __shared__ float tile[32][33];
+    if (S.zero && blockIdx.x == 0 && blockIdx.y == 0) {
+        const int t = threadIdx.y * 32 + threadIdx.x;
+        if (t < CLAIM_COUNTERS) S.zero[CLAIM_STRIDE * t] = 0u;
+    }
+    const ScaleJob &J = S.job[blockIdx.y];
+    const int r = blockIdx.x;
+    if (r >= J.blocks) return;
+    const int tiles_i = (J.rows + 31) / 32, tiles = tiles_i * ((J.cols + 31) / 32);
+    const int t = r % tiles;
+    scale_t_tile(J.sorted, r / tiles, J.n_angles, J.rows, J.cols, J.proj, J.out, (t % tiles_i) * 32,
+                 (t / tiles_i) * 32, tile);
 }
 
 // Detector column where the ray through in-slice point (n, c) (axes m, c2)
@@ -762,8 +809,13 @@ __device__ __forceinline__ void bp_wave_work(const BpView &v, int logical, BpSha
     }
 }
 
+// Waves per SIMD the register allocation is held to: 4 (16 per CU, four
+// blocks) for 32-deep bricks, 5 for 24-deep ones (their 28 KB of LDS fit five
+// blocks per CU), no bound for 16-deep ones.
+constexpr int bp_waves_per_eu(int bz) { return bz == 32 ? 4 : bz == 24 ? 5 : 1; }
+
 template <int BZ>
-__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ == 32 ? 4 : 1))) bp_wave(BpView v, int n_blocks, int shift)
+__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu(BZ)))) bp_wave(BpView v, int n_blocks, int shift)
 {
     __shared__ __align__(16) BpShared<BZ> sh;
     const int logical = xcd_logical(shift);
@@ -772,23 +824,27 @@ __global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ 
 }
 
 
+// Fused persistent launch (persistent_blocks, as fp_padded_multi).
 struct BpMulti {
     BpView v[MAX_FUSED];
     int begin[MAX_FUSED + 1];
     int n;
-    int shift;  // chunk size of the XCD order (xcd_logical)
+    int shift;        // chunk size of the XCD order (xcd_logical_of)
+    unsigned vgrid;   // virtual blocks
+    uint32_t *claim;  // CLAIM_COUNTERS counters, zero at the start of the launch
 };
 
 template <int BZ>
-__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(BZ == 32 ? 4 : 1))) bp_wave_multi(BpMulti m)
+__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu(BZ)))) bp_wave_multi(BpMulti m)
 {
     __shared__ __align__(16) BpShared<BZ> sh;
     const int total = m.begin[m.n];
-    const int logical = xcd_logical(m.shift);
-    if (logical >= total) return;
-    int k = 0;
-    while (k + 1 < m.n && logical >= m.begin[k + 1]) ++k;
-    bp_wave_work<BZ>(m.v[k], logical - m.begin[k], sh);
+    persistent_blocks(m.vgrid, m.claim, [&](unsigned vb) {
+        const int logical = xcd_logical_of(vb, m.shift);
+        if (logical >= total) return;
+        const int k = owner_of(m.begin, logical);
+        bp_wave_work<BZ>(m.v[k], logical - m.begin[k], sh);
+    });
 }
 
 }  // namespace tsx

@@ -175,15 +175,14 @@ fp_general(const AngleParams *__restrict__ angles, int n_angles, int a0, int row
 // With `pair`, element (c1, c2) of a layout holds the two floats of rows c1
 // and c1 + 1 at c2 (8 bytes; the row past the last is 0), so that the four
 // corners of a bilinear sample are 16 contiguous bytes.
-__global__ void pad_copy_yz(const float *__restrict__ src, float *__restrict__ dst, int nx, int ny,
-                            int nz, int axis, int pair)
+//
+// y- and z-stepping layouts: thread `xp` of the row (yp, zp) of the layout.
+__device__ __forceinline__ void pad_copy_yz_at(const float *__restrict__ src, float *__restrict__ dst, int nx,
+                                               int ny, int nz, int axis, int pair, int xp, int yp, int zp)
 {
     // axis 1: dst [nz+2P][ny][nx+2P] (c1 = z);  axis 2: dst [nz][ny+2P][nx+2P] (c1 = y)
     const int X = nx + 2 * PAD;
     const int Y = axis == 1 ? ny : ny + 2 * PAD;
-    const int xp = blockIdx.x * blockDim.x + threadIdx.x;
-    const int yp = blockIdx.y;
-    const int zp = blockIdx.z;
     if (xp >= X) return;
     const int x = xp - PAD;
     const int y = axis == 1 ? yp : yp - PAD;
@@ -200,34 +199,113 @@ __global__ void pad_copy_yz(const float *__restrict__ src, float *__restrict__ d
         dst[o] = at(z, y);
 }
 
-// dst [nz+2P][nx][ny+2P] (c1 = z) from src [nz][ny][nx], 32x32 tiles through
-// LDS (two z planes for the paired layout).
-__global__ void pad_transpose_x(const float *__restrict__ src, float *__restrict__ dst, int nx,
-                                int ny, int nz, int pair)
+// x-stepping layout dst [nz+2P][nx][ny+2P] (c1 = z) from src [nz][ny][nx]:
+// the 32 x 32 tile (x0, yp0) of layout plane zp through LDS (two z planes for
+// the paired layout), by a block of 32 x 8 threads (tx, ty).
+__device__ __forceinline__ void pad_transpose_x_at(const float *__restrict__ src, float *__restrict__ dst,
+                                                   int nx, int ny, int nz, int pair, int x0, int yp0, int zp,
+                                                   int tx, int ty, float (&tile)[2][32][33])
 {
-    __shared__ float tile[2][32][33];
     const int Yp = ny + 2 * PAD;
-    const int zp = blockIdx.z;
-    const int x0 = blockIdx.x * 32, yp0 = blockIdx.y * 32;
     for (int d = 0; d <= pair; ++d) {
         const int z = zp - PAD + d;
-        for (int r = threadIdx.y; r < 32; r += blockDim.y) {
-            const int yp = yp0 + r, x = x0 + threadIdx.x, y = yp - PAD;
+        for (int r = ty; r < 32; r += 8) {
+            const int yp = yp0 + r, x = x0 + tx, y = yp - PAD;
             float v = 0.0f;
             if (z >= 0 && z < nz && y >= 0 && y < ny && x < nx) v = src[((int64_t)z * ny + y) * nx + x];
-            tile[d][r][threadIdx.x] = v;
+            tile[d][r][tx] = v;
         }
     }
     __syncthreads();
-    for (int r = threadIdx.y; r < 32; r += blockDim.y) {
-        const int x = x0 + r, yp = yp0 + threadIdx.x;
+    for (int r = ty; r < 32; r += 8) {
+        const int x = x0 + r, yp = yp0 + tx;
         if (x < nx && yp < Yp) {
             const int64_t o = ((int64_t)zp * nx + x) * Yp + yp;
             if (pair)
-                reinterpret_cast<float2 *>(dst)[o] = make_float2(tile[0][threadIdx.x][r], tile[1][threadIdx.x][r]);
+                reinterpret_cast<float2 *>(dst)[o] = make_float2(tile[0][tx][r], tile[1][tx][r]);
             else
-                dst[o] = tile[0][threadIdx.x][r];
+                dst[o] = tile[0][tx][r];
         }
+    }
+}
+
+// Index k of the job that owns id i (begin[k] <= i < begin[k + 1]): a count
+// over every entry, so that the loads of begin[] (kernel arguments) issue
+// together rather than one after another as in a search that stops early
+// (with 32 layout jobs that search made the launch 1.6x slower). Entries
+// past the last job hold INT_MAX.
+template <int N>
+__device__ __forceinline__ int owner_of(const int (&begin)[N], int i)
+{
+    int k = 0;
+#pragma unroll
+    for (int e = 1; e < N - 1; ++e) k += i >= begin[e] ? 1 : 0;
+    return k;
+}
+
+// The padded layouts of several volumes in one launch: job k (a volume and
+// one stepping axis) is row y = k of the grid, whose blocks x < blocks[k]
+// build it (32 x 8 threads; the rest of the row returns at once). x layouts
+// take (ceil(nx / 32) x ceil((ny + 2P) / 32)) tiles per plane, y and z layouts
+// ceil((nx + 2P) / 256) blocks of 256 per row. (With the job found from a
+// table of first blocks instead, every block's table loads doubled the time
+// of these short blocks.)
+constexpr int MAX_LAYOUT_JOBS = 48;
+// claim counters of the persistent launches (persistent_blocks): one per
+// XCD, CLAIM_STRIDE uint32 (256 bytes) apart
+constexpr int CLAIM_STRIDE = 64;
+constexpr int CLAIM_COUNTERS = 8;
+constexpr int CLAIM_BYTES = CLAIM_COUNTERS * CLAIM_STRIDE * 4;
+
+struct LayoutJob {
+    const float *src;
+    float *dst;
+    int nx, ny, nz;
+    int axis;  // stepping axis of the layout: 0 (transposed), 1, 2
+    int pair;
+    int blocks;
+};
+
+struct LayoutJobs {
+    LayoutJob job[MAX_LAYOUT_JOBS];
+    int n;
+    uint32_t *zero;  // if not null: CLAIM_COUNTERS counters set to 0 (claims of the next fused launch)
+};
+
+// Blocks of job `j` in build_layouts_multi.
+__host__ __device__ inline int64_t layout_job_blocks(const LayoutJob &j)
+{
+    if (j.axis == 0)
+        return (int64_t)((j.nx + 31) / 32) * ((j.ny + 2 * PAD + 31) / 32) * (j.nz + 2 * PAD);
+    const int64_t rows = j.axis == 1 ? (int64_t)j.ny * (j.nz + 2 * PAD) : (int64_t)(j.ny + 2 * PAD) * j.nz;
+    return (int64_t)((j.nx + 2 * PAD + 255) / 256) * rows;
+}
+
+// x layouts (TRANSPOSE, through an LDS tile) or y and z layouts (plain
+// copies, no LDS: with the tile allocated for every block the copies ran
+// 2.3x slower) of a job list holding only that kind.
+template <bool TRANSPOSE>
+__global__ void __launch_bounds__(256) build_layouts_multi(LayoutJobs L)
+{
+    if (L.zero && blockIdx.x == 0 && blockIdx.y == 0) {
+        const int t = threadIdx.y * 32 + threadIdx.x;
+        if (t < CLAIM_COUNTERS) L.zero[CLAIM_STRIDE * t] = 0u;
+    }
+    const LayoutJob &J = L.job[blockIdx.y];
+    const int r = blockIdx.x;
+    if (r >= J.blocks) return;
+    if (TRANSPOSE) {
+        __shared__ float tile[2][32][33];
+        const int tx_tiles = (J.nx + 31) / 32, ty_tiles = (J.ny + 2 * PAD + 31) / 32;
+        const int bx = r % tx_tiles, by = (r / tx_tiles) % ty_tiles, zp = r / (tx_tiles * ty_tiles);
+        pad_transpose_x_at(J.src, J.dst, J.nx, J.ny, J.nz, J.pair, bx * 32, by * 32, zp, threadIdx.x,
+                           threadIdx.y, tile);
+    } else {
+        const int xb = (J.nx + 2 * PAD + 255) / 256;
+        const int Y = J.axis == 1 ? J.ny : J.ny + 2 * PAD;
+        const int row = r / xb;
+        pad_copy_yz_at(J.src, J.dst, J.nx, J.ny, J.nz, J.axis, J.pair,
+                       (r % xb) * 256 + threadIdx.y * 32 + threadIdx.x, row % Y, row / Y);
     }
 }
 
@@ -396,10 +474,11 @@ __device__ __forceinline__ void request_rowpair(__amdgpu_buffer_rsrc_t rs, const
 // All waves then march the same slices, in step: the rays of neighbouring
 // detector rows sample neighbouring volume rows, which they then find in L1
 // together (with independent waves the 8-row blocks are 12 % slower at 512^3).
+template <int BY>
 __device__ __forceinline__ void block_range(int &lo, int &hi)
 {
     static_assert(FP_BX == 64, "one wave per block row");
-    __shared__ int blo[FP_BY], bhi[FP_BY];
+    __shared__ int blo[BY], bhi[BY];
     __syncthreads();  // the previous call's reads are done
     if (threadIdx.x == 0) {
         blo[threadIdx.y] = lo;
@@ -407,7 +486,7 @@ __device__ __forceinline__ void block_range(int &lo, int &hi)
     }
     __syncthreads();
 #pragma unroll
-    for (int e = 0; e < FP_BY; ++e) {
+    for (int e = 0; e < BY; ++e) {
         lo = min(lo, blo[e]);
         hi = max(hi, bhi[e]);
     }
@@ -415,14 +494,14 @@ __device__ __forceinline__ void block_range(int &lo, int &hi)
     hi = __builtin_amdgcn_readfirstlane(hi);
 }
 
-template <bool ROWPAIR>
+template <bool ROWPAIR, int BY>
 __device__ __forceinline__ float march_padded_sync(__amdgpu_buffer_rsrc_t rs, const RaySetup &r,
                                                    bool valid, int nref, int n1, int n2,
                                                    uint32_t st_m, uint32_t st_1)
 {
     int wlo = wave_min(valid ? r.nlo : 0x7fffffff);
     int whi = wave_max(valid ? r.nhi : -1);
-    block_range(wlo, whi);  // block-uniform from here on
+    block_range<BY>(wlo, whi);  // block-uniform from here on
     if (wlo > whi) return 0.0f;
     SyncRay q;
     q.lo1 = (float)PAD - 1.5f;
@@ -550,7 +629,7 @@ __device__ __forceinline__ float march_lds(__amdgpu_buffer_rsrc_t rs, const RayS
 {
     int wlo = wave_min(valid ? r.nlo : 0x7fffffff);
     int whi = wave_max(valid ? r.nhi : -1);
-    block_range(wlo, whi);  // block-uniform from here on
+    block_range<FP_BY>(wlo, whi);  // block-uniform from here on
     if (wlo > whi) return 0.0f;
     const int span = whi - wlo;
     const int w = threadIdx.y, lane = threadIdx.x, t = w * FP_BX + lane;
@@ -607,7 +686,7 @@ __device__ __forceinline__ float march_lds(__amdgpu_buffer_rsrc_t rs, const RayS
         fits = rows_w <= FW_ROWS && chunks_w * 4 <= FW_W;
     }
     fits = __builtin_amdgcn_readfirstlane((int)fits) != 0;
-    if (!fits) return march_padded_sync<false>(rs, r, valid, nref, n1, n2, st_m, st_1);
+    if (!fits) return march_padded_sync<false, FP_BY>(rs, r, valid, nref, n1, n2, st_m, st_1);
     rows_w = __builtin_amdgcn_readfirstlane(rows_w);
     chunks_w = __builtin_amdgcn_readfirstlane(chunks_w);
     // staging slots of this thread: chunk c of window row rr, at most FW_SLOTS
@@ -698,7 +777,7 @@ struct FpView {
 // Work item `logical` of one instance: (batch entry, detector tile, angle),
 // angle fastest so that blocks resident together share a detector tile and
 // therefore a slab of the volume.
-template <int SS, int PIPE>
+template <int SS, int PIPE, int BY>
 __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
 {
     const int a = v.a0 + logical % v.na;
@@ -707,7 +786,7 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
     const int tile = rest % tiles;
     const int b = rest / tiles;
     const int j = (tile % v.tiles_u) * FP_BX + threadIdx.x;
-    const int i = (tile / v.tiles_u) * FP_BY + threadIdx.y;
+    const int i = (tile / v.tiles_u) * BY + threadIdx.y;
     if (b >= v.batch) return;  // uniform over the block
     const bool inside = j < v.cols && i < v.rows;
     if (PIPE < 3 && !inside) return;
@@ -725,6 +804,7 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
             const double fj = SS == 1 ? (double)j : j + (q + 0.5) / SS - 0.5;
             const double fi = SS == 1 ? (double)i : i + (p + 0.5) / SS - 0.5;
             if (PIPE == 5) {
+                static_assert(PIPE != 5 || BY == FP_BY, "the LDS-staged march takes 8-row tiles");
                 __shared__ __align__(16) FpLdsShared lsh;
                 const bool ok = inside && ray_setup(A, fj, fi, (double)PAD, r);
                 acc += march_lds(rs, r, ok, A.n[0] / 2, A.n[1], A.n[2], st_m, st_1, lsh);
@@ -732,7 +812,7 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
                 // every lane of the block joins the march (block-wide range,
                 // barriers inside; uniform calls)
                 const bool ok = inside && ray_setup(A, fj, fi, (double)PAD, r);
-                acc += march_padded_sync<PIPE == 4>(rs, r, ok, A.n[0] / 2, A.n[1], A.n[2], st_m, st_1);
+                acc += march_padded_sync<PIPE == 4, BY>(rs, r, ok, A.n[0] / 2, A.n[1], A.n[2], st_m, st_1);
             } else if (ray_setup(A, fj, fi, (double)PAD, r)) {
                 acc += march_padded_pipe4(rs, r, st_m, st_1);
             }
@@ -752,24 +832,67 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
 // row), and every XCD holds a share of every part of the launch. With one
 // contiguous range per XCD instead, the XCDs that hold the detector's edge
 // rows (or the volume's edge slabs) run out of work early.
-__device__ __forceinline__ int xcd_logical(int shift)
+__device__ __forceinline__ int xcd_logical_of(unsigned vb, int shift)
 {
-    const int x = (int)(blockIdx.x % 8u), j = (int)(blockIdx.x / 8u);
+    const int x = (int)(vb % 8u), j = (int)(vb / 8u);
     return ((((j >> shift) << 3) + x) << shift) + (j & ((1 << shift) - 1));
 }
 
-template <int SS, int PIPE>
-__global__ void __launch_bounds__(FP_BX *FP_BY) fp_padded(FpView v, int n_blocks, int shift)
+__device__ __forceinline__ int xcd_logical(int shift) { return xcd_logical_of(blockIdx.x, shift); }
+
+// Block of FP_BX x BY threads: a tile of BY detector rows (one wave each).
+template <int SS, int PIPE, int BY>
+__global__ void __launch_bounds__(FP_BX *BY) fp_padded(FpView v, int n_blocks, int shift)
 {
     const int logical = xcd_logical(shift);
     if (logical >= n_blocks) return;
-    fp_padded_work<SS, PIPE>(v, logical);
+    fp_padded_work<SS, PIPE, BY>(v, logical);
 }
 
-// Fused launch over up to MAX_FUSED instances (any shapes): instance k owns the
-// logical ids [begin[k], begin[k + 1]).
+// Fused persistent launches. The grid holds a fixed number of blocks per CU
+// (a multiple of 8 in all), and they work through a virtual grid of vgrid
+// blocks (xcd_grid; virtual block vb = 8 j + x belongs to XCD x, and the
+// hardware places block b on XCD b % 8). Block b runs j = b / 8 of XCD b % 8
+// first, then claims the next unclaimed j of its XCD from that XCD's counter
+// (claim[CLAIM_STRIDE x], zeroed by the launch's pre-pass kernel). The
+// blocks of an XCD thus take its virtual blocks in order, as the hardware
+// dispatcher hands out the blocks of a launch of one block per virtual
+// block, and the blocks at work at a time stay on neighbouring logical ids.
+// Every virtual block is run once. Measured at 512^3 (forward, 8-row tiles,
+// one instance): a fixed stride instead of claims, 3x slower (blocks of
+// cheap and dear work items drift apart); claims of 16 virtual blocks at a
+// time, 2.8x; claims spread over 8 counters per XCD, 2.3x; one counter per
+// XCD: 62, 44.3 and 72 ms with 3, 2 and 1 blocks of 8 waves per CU against
+// 43.3 ms for the launch of one block per virtual block (the grid's blocks
+// per CU are set by persistent_grid). A claim is issued when the block
+// starts a work item and read when it ends it.
+template <typename F>
+__device__ __forceinline__ void persistent_blocks(unsigned vgrid, uint32_t *__restrict__ claim, F &&work)
+{
+    __shared__ unsigned nxt[2];
+    const unsigned x = blockIdx.x % 8u, per_xcd = gridDim.x / 8u, jn = vgrid / 8u;
+    const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
+    unsigned j = blockIdx.x / 8u;
+    for (int it = 0; j < jn; ++it) {
+        unsigned c = 0;
+        if (leader) c = atomicAdd(&claim[CLAIM_STRIDE * x], 1u);
+        work(j * 8u + x);
+        if (leader) nxt[it & 1] = c + per_xcd;
+        __syncthreads();
+        // uniform by construction; said so, so that what derives from it
+        // (instance, angle, buffer descriptors) stays in scalar registers
+        j = __builtin_amdgcn_readfirstlane(nxt[it & 1]);
+    }
+}
+
+// Fused launch over up to MAX_FUSED instances (any shapes): instance k owns
+// the logical ids [begin[k], begin[k + 1]).
 constexpr int MAX_FUSED = 16;
 
+// The fused forward projector takes one block per logical id in the XCD
+// order (not persistent_blocks: at 512^3 x 16 instances its persistent form
+// took 49.4 ms per instance at its best, 2 blocks of 8 waves per CU, against
+// 43.6 ms here).
 struct FpMulti {
     FpView v[MAX_FUSED];
     int begin[MAX_FUSED + 1];
@@ -777,15 +900,13 @@ struct FpMulti {
     int shift;  // chunk size of the XCD order (xcd_logical)
 };
 
-template <int PIPE>
-__global__ void __launch_bounds__(FP_BX *FP_BY) fp_padded_multi(FpMulti m)
+template <int PIPE, int BY>
+__global__ void __launch_bounds__(FP_BX *BY) fp_padded_multi(FpMulti m)
 {
-    const int total = m.begin[m.n];
     const int logical = xcd_logical(m.shift);
-    if (logical >= total) return;
-    int k = 0;
-    while (k + 1 < m.n && logical >= m.begin[k + 1]) ++k;
-    fp_padded_work<1, PIPE>(m.v[k], logical - m.begin[k]);
+    if (logical >= m.begin[m.n]) return;
+    const int k = owner_of(m.begin, logical);
+    fp_padded_work<1, PIPE, BY>(m.v[k], logical - m.begin[k]);
 }
 
 }  // namespace tsx

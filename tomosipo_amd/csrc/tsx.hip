@@ -131,7 +131,7 @@ struct tsx_plan {
     int32_t bp_chunk = 0;       // separable backprojector: sorted angles per launch
     std::string bp_reason;      // why bp_separable is off
     // kernel variants chosen by tsx_plan_autotune (-1: library default)
-    int32_t fp_variant = -1, bp_bz = -1;
+    int32_t fp_variant = -1, bp_bz = -1, fp_by = -1;
     // FDK (cone plans): per-angle quantities and the volume window
     std::vector<FdkAngle> fdk_host;
     FdkAngle *fdk_dev = nullptr;
@@ -505,6 +505,18 @@ int fp_variant_of(const tsx_plan *p)
 // Floats per element of the forward projector's padded layouts.
 int fp_esize(const tsx_plan *p) { return fp_variant_of(p) == 4 ? 2 : 1; }
 
+// Detector rows per block of the padded forward projector: 8 (default), or
+// 4 or 16 for marches 3 and 4 without supersampling. TSX_FP_TILE_ROWS
+// overrides the plan's choice.
+int fp_tile_rows_of(const tsx_plan *p)
+{
+    const int v = fp_variant_of(p);
+    if (p->ss != 1 || (v != 3 && v != 4)) return 8;
+    const int e = env_int("TSX_FP_TILE_ROWS");
+    if (e == 4 || e == 8 || e == 16) return e;
+    return p->fp_by > 0 ? p->fp_by : 8;
+}
+
 // Largest byte size of one padded forward layout and of the transposed
 // projections of one backprojection launch: the kernels address them with
 // 32-bit byte offsets. TSX_CHUNK_BYTES, read when a plan is created, lowers
@@ -534,12 +546,12 @@ Epilogue later_chunk(Epilogue e)
     return e;
 }
 
-// z-extent of the separable backprojector's bricks: 32 (default) or 16.
+// z-extent of the separable backprojector's bricks: 32 (default), 24 or 16.
 // TSX_BP_BRICK_Z overrides the plan's choice.
 int bp_brick_z_of(const tsx_plan *p)
 {
     const int e = env_int("TSX_BP_BRICK_Z");
-    if (e == 16 || e == 32) return e;
+    if (e == 16 || e == 24 || e == 32) return e;
     return p->bp_bz > 0 ? p->bp_bz : 32;
 }
 
@@ -732,31 +744,6 @@ int get_workspace(const tsx_plan *p, int64_t bytes, void *ws, int64_t ws_bytes, 
     return rc;
 }
 
-// Padded copies of one volume for every stepping axis the plan uses.
-void build_layouts(const tsx_plan *p, int es, const float *src, float *dst, const int64_t offs[3],
-                   hipStream_t stream)
-{
-    const int pair = es == 2;
-    if (p->axis_count[0] > 0) {
-        dim3 grid((unsigned)((p->nx + 31) / 32), (unsigned)((p->ny + 2 * PAD + 31) / 32),
-                  (unsigned)(p->nz + 2 * PAD));
-        hipLaunchKernelGGL(pad_transpose_x, grid, dim3(32, 8), 0, stream, src, dst + offs[0], p->nx,
-                           p->ny, p->nz, pair);
-    }
-    if (p->axis_count[1] > 0) {
-        dim3 grid((unsigned)((p->nx + 2 * PAD + 255) / 256), (unsigned)p->ny,
-                  (unsigned)(p->nz + 2 * PAD));
-        hipLaunchKernelGGL(pad_copy_yz, grid, dim3(256), 0, stream, src, dst + offs[1], p->nx, p->ny,
-                           p->nz, 1, pair);
-    }
-    if (p->axis_count[2] > 0) {
-        dim3 grid((unsigned)((p->nx + 2 * PAD + 255) / 256), (unsigned)(p->ny + 2 * PAD),
-                  (unsigned)p->nz);
-        hipLaunchKernelGGL(pad_copy_yz, grid, dim3(256), 0, stream, src, dst + offs[2], p->nx, p->ny,
-                           p->nz, 2, pair);
-    }
-}
-
 // Offsets (in floats) of the layouts of one volume, and their total.
 void layout_offsets(const tsx_plan *p, int es, int64_t offs[3], int64_t *per_entry)
 {
@@ -767,6 +754,76 @@ void layout_offsets(const tsx_plan *p, int es, int64_t offs[3], int64_t *per_ent
     }
     *per_entry = e;
 }
+
+// Entries of a job table past job n - 1 (and its end begin[n]): INT_MAX, as
+// owner_of() expects.
+template <int N>
+void pad_begin(int (&begin)[N], int n)
+{
+    for (int e = n + 1; e < N; ++e) begin[e] = 0x7fffffff;
+}
+
+// Padded layouts of several volumes, built by build_layouts_multi in as few
+// launches as its job lists and the dispatch limit allow: one list of x
+// layouts (transposes), one of y and z layouts (copies). add() collects,
+// flush() launches what was collected; the first launch of a flush zeroes
+// `zero` (the claim counters of a persistent launch that follows), if set.
+struct LayoutBatch {
+    LayoutJobs L[2];  // [0]: copies, [1]: transposes
+    int maxb[2] = {0, 0};
+    hipStream_t stream;
+    uint32_t *zero;
+    explicit LayoutBatch(hipStream_t s, uint32_t *z = nullptr) : stream(s), zero(z)
+    {
+        for (LayoutJobs &l : L) {
+            l.n = 0;
+            l.zero = nullptr;
+        }
+    }
+    void add(const tsx_plan *p, int es, const float *src, float *dst, const int64_t offs[3])
+    {
+        for (int c = 0; c < 3; ++c) {
+            if (p->axis_count[c] == 0) continue;
+            LayoutJob j;
+            j.src = src;
+            j.dst = dst + offs[c];
+            j.nx = p->nx;
+            j.ny = p->ny;
+            j.nz = p->nz;
+            j.axis = c;
+            j.pair = es == 2;
+            j.blocks = (int)layout_job_blocks(j);
+            const int t = c == 0;
+            LayoutJobs &l = L[t];
+            // a grid of (largest job) x (jobs) blocks of 256 stays below the
+            // dispatch limit
+            if (l.n == MAX_LAYOUT_JOBS ||
+                (l.n > 0 && (int64_t)std::max(maxb[t], j.blocks) * (l.n + 1) > max_blocks(256)))
+                launch(t);
+            l.job[l.n] = j;
+            maxb[t] = std::max(maxb[t], j.blocks);
+            l.n += 1;
+        }
+    }
+    void launch(int t)
+    {
+        LayoutJobs &l = L[t];
+        if (l.n == 0) return;
+        l.zero = zero;
+        const dim3 grid((unsigned)maxb[t], (unsigned)l.n);
+        if (t)
+            hipLaunchKernelGGL(build_layouts_multi<true>, grid, dim3(32, 8), 0, stream, l);
+        else
+            hipLaunchKernelGGL(build_layouts_multi<false>, grid, dim3(32, 8), 0, stream, l);
+        l.n = 0;
+        maxb[t] = 0;
+    }
+    void flush()
+    {
+        launch(1);
+        launch(0);
+    }
+};
 
 // fp_general over every angle and batch entry, in launches of at most 65535
 // angles and 65535 entries.
@@ -790,24 +847,66 @@ int launch_fp_general(const tsx_plan *p, const T *vol, int64_t vs, T *proj, int6
     return TSX_OK;
 }
 
-// The padded forward projector of variant `variant` on the whole volume of
-// plan `p` (a plan without slabs, or one slab), its layouts built in `base`.
-int launch_fp_padded(const tsx_plan *p, int variant, const float *vol, int64_t vs, float *proj, int64_t ps,
-                     int64_t batch, Epilogue ep, float *base, hipStream_t stream)
+// The padded forward kernels by supersampling, march variant and tile rows
+// (tiles of 4 and 16 rows for variants 3 and 4 without supersampling; 8 rows
+// otherwise).
+using FpKernel = void (*)(FpView, int, int);
+using FpMultiKernel = void (*)(FpMulti);
+
+template <int SS>
+FpKernel fp_kernel_ss(int variant)
+{
+    if (variant == 4) return fp_padded<SS, 4, 8>;
+    if (variant == 5) return fp_padded<SS, 5, 8>;
+    if (variant == 3) return fp_padded<SS, 3, 8>;
+    return fp_padded<SS, 2, 8>;
+}
+
+FpKernel fp_kernel(int ss, int variant, int by)
+{
+    if (ss == 1 && by == 4) return variant == 4 ? fp_padded<1, 4, 4> : fp_padded<1, 3, 4>;
+    if (ss == 1 && by == 16) return variant == 4 ? fp_padded<1, 4, 16> : fp_padded<1, 3, 16>;
+    switch (ss) {
+    case 1: return fp_kernel_ss<1>(variant);
+    case 2: return fp_kernel_ss<2>(variant);
+    case 3: return fp_kernel_ss<3>(variant);
+    default: return fp_kernel_ss<4>(variant);
+    }
+}
+
+FpMultiKernel fp_multi_kernel(int variant, int by)
+{
+    if (by == 4) return variant == 4 ? fp_padded_multi<4, 4> : fp_padded_multi<3, 4>;
+    if (by == 16) return variant == 4 ? fp_padded_multi<4, 16> : fp_padded_multi<3, 16>;
+    if (variant == 4) return fp_padded_multi<4, 8>;
+    if (variant == 5) return fp_padded_multi<5, 8>;
+    if (variant == 3) return fp_padded_multi<3, 8>;
+    return fp_padded_multi<2, 8>;
+}
+
+// The padded forward projector of variant `variant` with tiles of `by`
+// detector rows on the whole volume of plan `p` (a plan without slabs, or
+// one slab), its layouts built in `base`.
+int launch_fp_padded(const tsx_plan *p, int variant, int by, const float *vol, int64_t vs, float *proj,
+                     int64_t ps, int64_t batch, Epilogue ep, float *base, hipStream_t stream)
 {
     const int tiles_u = (p->cols + FP_BX - 1) / FP_BX;
-    const int tiles_v = (p->rows + FP_BY - 1) / FP_BY;
+    const int tiles_v = (p->rows + by - 1) / by;
     const int es = esize_of(variant);
     // padded copies of every batch entry for each stepping axis in use
     int64_t per_entry = 0;
     int64_t offs[3] = {0, 0, 0};
     layout_offsets(p, es, offs, &per_entry);
-    for (int64_t b = 0; b < batch; ++b) build_layouts(p, es, vol + b * vs, base + b * per_entry, offs, stream);
+    {
+        LayoutBatch lb(stream);
+        for (int64_t b = 0; b < batch; ++b) lb.add(p, es, vol + b * vs, base + b * per_entry, offs);
+        lb.flush();
+    }
     TSX_HIP_CHECK(hipGetLastError());
     // launches of whole batch entries while they fit the grid limit, else of
     // angle ranges of one entry
     const int64_t tiles = (int64_t)tiles_u * tiles_v;
-    const int64_t cap = max_blocks(FP_BX * FP_BY);
+    const int64_t cap = max_blocks(FP_BX * by);
     const int64_t per_b = tiles * p->n_angles;
     const int64_t bc = std::max<int64_t>(1, cap / per_b);      // entries per launch
     const int na_max = (int)std::min<int64_t>(p->n_angles, std::max<int64_t>(1, cap / tiles));
@@ -835,29 +934,8 @@ int launch_fp_padded(const tsx_plan *p, int variant, const float *vol, int64_t v
             const int n_blocks = (int)(tiles * v.na * nbat);
             int shift;
             const unsigned grid = xcd_grid(n_blocks, &shift);
-            switch (p->ss) {
-#define TSX_FP_LAUNCH(K, V)                                                                        \
-    hipLaunchKernelGGL((fp_padded<K, V>), dim3(grid), dim3(FP_BX, FP_BY), 0, stream, v, n_blocks, shift)
-#define TSX_FP_CASE(K)                                                                             \
-    case K:                                                                                        \
-        if (variant == 4)                                                                          \
-            TSX_FP_LAUNCH(K, 4);                                                                   \
-        else if (variant == 5)                                                                     \
-            TSX_FP_LAUNCH(K, 5);                                                                   \
-        else if (variant == 3)                                                                     \
-            TSX_FP_LAUNCH(K, 3);                                                                   \
-        else                                                                                       \
-            TSX_FP_LAUNCH(K, 2);                                                                   \
-        break;
-                TSX_FP_CASE(1)
-                TSX_FP_CASE(2)
-                TSX_FP_CASE(3)
-                TSX_FP_CASE(4)
-#undef TSX_FP_CASE
-#undef TSX_FP_LAUNCH
-            default:
-                break;
-            }
+            hipLaunchKernelGGL(fp_kernel(p->ss, variant, by), dim3(grid), dim3(FP_BX, by), 0, stream, v, n_blocks,
+                               shift);
             TSX_HIP_CHECK(hipGetLastError());
         }
     }
@@ -874,16 +952,16 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
     if (batch <= 0) return TSX_OK;
     if (!p->fp_padded || force_general() || p->ss > 4)
         return launch_fp_general<float>(p, vol, vs, proj, ps, batch, ep, stream);
-    const int variant = fp_variant_of(p);
+    const int variant = fp_variant_of(p), by = fp_tile_rows_of(p);
     Scratch owned;
     void *wptr = nullptr;
     int rc = get_workspace(p, fp_workspace_elems(p) * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
     if (rc) return rc;
     if (p->slabs.empty())
-        return launch_fp_padded(p, variant, vol, vs, proj, ps, batch, ep, (float *)wptr, stream);
+        return launch_fp_padded(p, variant, by, vol, vs, proj, ps, batch, ep, (float *)wptr, stream);
     for (size_t k = 0; k < p->slabs.size(); ++k) {
-        rc = launch_fp_padded(p->slabs[k], variant, vol + (int64_t)p->slab_z0[k] * p->ny * p->nx, vs, proj, ps,
-                              batch, k == 0 ? ep : later_chunk(ep), (float *)wptr, stream);
+        rc = launch_fp_padded(p->slabs[k], variant, by, vol + (int64_t)p->slab_z0[k] * p->ny * p->nx, vs, proj,
+                              ps, batch, k == 0 ? ep : later_chunk(ep), (float *)wptr, stream);
         if (rc) return rc;
     }
     return TSX_OK;
@@ -982,6 +1060,8 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
             const unsigned grid = xcd_grid(n_blocks, &shift);
             if (BZ == 32)
                 hipLaunchKernelGGL(bp_wave<32>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
+            else if (BZ == 24)
+                hipLaunchKernelGGL(bp_wave<24>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
             else
                 hipLaunchKernelGGL(bp_wave<16>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
             TSX_HIP_CHECK(hipGetLastError());
@@ -1028,46 +1108,81 @@ int check_multi(const tsx_plan *const *plans, int n)
     return TSX_OK;
 }
 
+// Scratch of a fused launch: the persistent kernels' claim counters, then
+// each instance's workspace.
 int64_t multi_workspace_bytes(const tsx_plan *const *plans, int n, int op)
 {
-    int64_t total = 0;
+    int64_t total = align_up(CLAIM_BYTES);
     for (int k = 0; k < n; ++k) total += align_up(workspace_bytes(plans[k], op, 1));
     return total;
 }
 
+// Blocks of `kernel` (`threads` per block) that fit one CU at once and the
+// device's CU count (cached per kernel and device).
+void resident_blocks(const void *kernel, int threads, int device, int *per_cu, int *cus)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void *, int>, std::pair<int, int>> cache;
+    std::lock_guard<std::mutex> lk(mu);
+    const auto key = std::make_pair(kernel, device);
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        int pc = 0, n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kernel, threads, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+            (void)hipGetLastError();
+            pc = 1;
+            n = 256;
+        }
+        it = cache.emplace(key, std::make_pair(std::max(1, pc), std::max(1, n))).first;
+    }
+    *per_cu = it->second.first;
+    *cus = it->second.second;
+}
+
+// Grid of a persistent launch over vgrid virtual blocks (a multiple of 8):
+// the blocks of `kernel` that fit a CU at once (occupancy query), at most
+// 16 waves per CU, on every CU; no more than vgrid. (The backprojector's
+// 4-wave blocks: 4 per CU, 42 ms at 512^3 against 73 ms with 2. The forward
+// projector's 8-wave blocks ran best at 2 per CU, 44.3 ms against 62 ms
+// with the 3 that fit, where the query reports 4.)
+unsigned persistent_grid(const void *kernel, int threads, int device, unsigned vgrid)
+{
+    const int waves = (threads + 63) / 64;
+    int per_cu = 0, cus = 0;
+    resident_blocks(kernel, threads, device, &per_cu, &cus);
+    per_cu = std::max(1, std::min(per_cu, 16 / waves));
+    const unsigned grid = (unsigned)std::max(8, per_cu * cus / 8 * 8);
+    return std::min(vgrid, grid);
+}
+
 // Fused launches: consecutive instances on the fast kernels with the same
-// kernel variant share one launch (up to MAX_FUSED instances, as many blocks
-// as the grid limit takes); an instance on the general kernels, with
-// detector supersampling, or with more blocks than one launch takes, gets its
-// own launch(es). eps[k] is the epilogue of instance k.
+// kernel variant share one persistent launch (up to MAX_FUSED instances, as
+// many logical blocks as an int counts), their padded layouts (forward) or
+// transposed projections (backward) built before it in one launch; an
+// instance on the general kernels, with detector supersampling, in slabs or
+// angle ranges gets its own launch(es). eps[k] is the epilogue of instance k.
 int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *const *projs,
              const Epilogue *eps, int n, void *wsp, int64_t ws_bytes, hipStream_t stream)
 {
     const int64_t need = multi_workspace_bytes(plans, n, TSX_OP_FP);
     Scratch owned;
     void *wptr = nullptr;
-    if (need > 0) {
-        int rc = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
-        if (rc) return rc;
-    }
+    int rc0 = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
+    if (rc0) return rc0;
     const bool fast = !force_general();
-    const int64_t cap = max_blocks(FP_BX * FP_BY);
     FpMulti m;
     m.n = 0;
     m.begin[0] = 0;
-    int64_t woff = 0;
-    int variant = -1;
+    LayoutBatch layouts(stream);
+    int64_t woff = align_up(CLAIM_BYTES);
+    int variant = -1, by = 8;
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
+        layouts.flush();
         const unsigned grid = xcd_grid(m.begin[m.n], &m.shift);
-        if (variant == 4)
-            hipLaunchKernelGGL(fp_padded_multi<4>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
-        else if (variant == 5)
-            hipLaunchKernelGGL(fp_padded_multi<5>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
-        else if (variant == 3)
-            hipLaunchKernelGGL(fp_padded_multi<3>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
-        else
-            hipLaunchKernelGGL(fp_padded_multi<2>, dim3(grid), dim3(FP_BX, FP_BY), 0, stream, m);
+        pad_begin(m.begin, m.n);
+        hipLaunchKernelGGL(fp_multi_kernel(variant, by), dim3(grid), dim3(FP_BX, by), 0, stream, m);
         TSX_HIP_CHECK(hipGetLastError());
         m.n = 0;
         return TSX_OK;
@@ -1076,8 +1191,10 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         const tsx_plan *p = plans[k];
         if (!vols[k] || !projs[k]) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
         const int64_t wb = align_up(workspace_bytes(p, TSX_OP_FP, 1));
-        const int tiles_u = (p->cols + FP_BX - 1) / FP_BX, tiles_v = (p->rows + FP_BY - 1) / FP_BY;
+        const int pby = fp_tile_rows_of(p);
+        const int tiles_u = (p->cols + FP_BX - 1) / FP_BX, tiles_v = (p->rows + pby - 1) / pby;
         const int64_t blocks = (int64_t)tiles_u * tiles_v * p->n_angles;
+        const int64_t cap = max_blocks(FP_BX * pby);
         if (!(fast && p->fp_padded && p->ss == 1 && p->slabs.empty()) || blocks > cap) {
             // this instance takes its own launch(es)
             int rc = launch_fp(p, vols[k], 0, projs[k], 0, 1, eps[k], stream, wptr ? (char *)wptr + woff : nullptr, wb);
@@ -1085,17 +1202,18 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
             woff += wb;
             continue;
         }
-        if (m.n > 0 && (fp_variant_of(p) != variant || (int64_t)m.begin[m.n] + blocks > cap)) {
+        if (m.n > 0 && (fp_variant_of(p) != variant || pby != by || (int64_t)m.begin[m.n] + blocks > cap)) {
             int rc = flush();
             if (rc) return rc;
         }
         variant = fp_variant_of(p);
+        by = pby;
         const int es = esize_of(variant);
         int64_t offs[3], per_entry;
         layout_offsets(p, es, offs, &per_entry);
         float *base = (float *)((char *)wptr + woff);
         woff += wb;
-        build_layouts(p, es, vols[k], base, offs, stream);
+        layouts.add(p, es, vols[k], base, offs);
         FpView &v = m.v[m.n];
         v.angles = p->dev;
         v.n_angles = p->n_angles;
@@ -1131,22 +1249,34 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
     const int64_t need = multi_workspace_bytes(plans, n, TSX_OP_BP);
     Scratch owned;
     void *wptr = nullptr;
-    if (need > 0) {
-        int rc = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
-        if (rc) return rc;
-    }
+    int rc0 = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
+    if (rc0) return rc0;
     const bool fast = !force_general();
-    const int64_t cap = max_blocks(BR * BR);
     int BZ = 32;
     BpMulti m;
     m.n = 0;
     m.begin[0] = 0;
-    int64_t woff = 0;
+    m.claim = (uint32_t *)wptr;
+    ScaleJobs sj;
+    sj.zero = m.claim;  // the transposition launch before a fused launch zeroes the claims
+    int sj_max = 0;     // blocks of the largest job in sj
+    int64_t woff = align_up(CLAIM_BYTES);
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
-        const unsigned grid = xcd_grid(m.begin[m.n], &m.shift);
+        sj.n = m.n;
+        pad_begin(m.begin, m.n);
+        hipLaunchKernelGGL(scale_by_length_t_multi, dim3((unsigned)sj_max, (unsigned)sj.n), dim3(32, 8), 0, stream,
+                           sj);
+        sj_max = 0;
+        m.vgrid = xcd_grid(m.begin[m.n], &m.shift);
+        const void *kern = BZ == 32   ? (const void *)bp_wave_multi<32>
+                           : BZ == 24 ? (const void *)bp_wave_multi<24>
+                                      : (const void *)bp_wave_multi<16>;
+        const unsigned grid = persistent_grid(kern, BR * BR, plans[0]->device, m.vgrid);
         if (BZ == 32)
             hipLaunchKernelGGL(bp_wave_multi<32>, dim3(grid), dim3(BR, BR), 0, stream, m);
+        else if (BZ == 24)
+            hipLaunchKernelGGL(bp_wave_multi<24>, dim3(grid), dim3(BR, BR), 0, stream, m);
         else
             hipLaunchKernelGGL(bp_wave_multi<16>, dim3(grid), dim3(BR, BR), 0, stream, m);
         TSX_HIP_CHECK(hipGetLastError());
@@ -1159,13 +1289,23 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         const int64_t wb = align_up(workspace_bytes(p, TSX_OP_BP, 1));
         const int bz = bp_brick_z_of(p);
         const int64_t blocks = (int64_t)((p->nx + BR - 1) / BR) * ((p->ny + BR - 1) / BR) * ((p->nz + bz - 1) / bz);
-        if (!(fast && p->bp_separable && bp_chunk_of(p) == p->n_angles) || blocks > cap) {
+        ScaleJob job;
+        job.sorted = p->dev + p->n_angles;
+        job.kn = p->n_angles;
+        job.n_angles = p->n_angles;
+        job.rows = p->rows;
+        job.cols = p->cols;
+        job.proj = projs[k];
+        const int64_t tblocks = scale_job_blocks(job);
+        if (!(fast && p->bp_separable && bp_chunk_of(p) == p->n_angles) || blocks > MAX_LOGICAL ||
+            tblocks > max_blocks(256)) {
             int rc = launch_bp(p, vols[k], 0, projs[k], 0, 1, eps[k], stream, wptr ? (char *)wptr + woff : nullptr, wb);
             if (rc) return rc;
             woff += wb;
             continue;
         }
-        if (m.n > 0 && (bz != BZ || (int64_t)m.begin[m.n] + blocks > cap)) {
+        if (m.n > 0 && (bz != BZ || (int64_t)m.begin[m.n] + blocks > MAX_LOGICAL ||
+                        std::max<int64_t>(sj_max, tblocks) * (m.n + 1) > max_blocks(256))) {
             int rc = flush();
             if (rc) return rc;
         }
@@ -1173,7 +1313,10 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         float *ly = (float *)((char *)wptr + woff);
         woff += wb;
         const int64_t sino_t = bp_workspace_elems(p);
-        launch_scale_by_length_t(p, 0, p->n_angles, projs[k], 0, ly, sino_t, 1, stream);
+        job.out = ly;
+        job.blocks = (int)tblocks;
+        sj.job[m.n] = job;
+        sj_max = std::max(sj_max, (int)tblocks);
         m.v[m.n] = bp_view(p, BZ, vols[k], 0, ly, (uint32_t)(sino_t * 4), 1, eps[k], 0, p->n_angles);
         m.begin[m.n + 1] = m.begin[m.n] + (int)blocks;
         m.n += 1;
@@ -1190,7 +1333,7 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
 // autotuner
 // ---------------------------------------------------------------------------
 struct TuneChoice {
-    int fp_variant, bp_bz;
+    int fp_variant, bp_bz, fp_by;
 };
 std::mutex g_tune_mu;
 std::map<std::string, TuneChoice> g_tune_cache;
@@ -1253,6 +1396,14 @@ struct TuneScratch {
     }
 };
 
+// Detector rows of the autotuner's tile-row trials: all of them, or as many
+// as 2 GiB of projections hold.
+int tile_band_rows(const tsx_plan *p)
+{
+    const int64_t per_row = (int64_t)p->n_angles * p->cols * 4;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(p->rows, ((int64_t)1 << 31) / per_row));
+}
+
 int autotune(tsx_plan *p, hipStream_t stream)
 {
     const std::string key = tune_key(p);
@@ -1262,6 +1413,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
         if (it != g_tune_cache.end()) {
             p->fp_variant = it->second.fp_variant;
             p->bp_bz = it->second.bp_bz;
+            p->fp_by = it->second.fp_by;
             return TSX_OK;
         }
     }
@@ -1299,7 +1451,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
     TuneScratch t;
     const size_t vol_bytes = (size_t)p->nx * p->ny * p->nz * 4;
     const size_t proj_bytes = (size_t)p->rows * sub.n_angles * p->cols * 4;
-    const size_t band_bytes = (size_t)std::min(p->rows, 64) * p->n_angles * p->cols * 4;
+    const size_t band_bytes = (size_t)std::max(tile_band_rows(p), std::min(p->rows, 64)) * p->n_angles * p->cols * 4;
     const std::vector<AngleParams> sub_table = device_angle_table(sub.host);
     if (hipMalloc(&t.angles, sizeof(AngleParams) * sub_table.size()) != hipSuccess ||
         hipMalloc(&t.band_angles, sizeof(AngleParams) * p->host.size()) != hipSuccess ||
@@ -1322,7 +1474,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
     TSX_HIP_CHECK(hipEventCreate(&t.e1));
     const Epilogue ep = plain(0);
     float *vol = (float *)t.vol, *proj = (float *)t.proj;
-    TuneChoice best{-1, -1};
+    TuneChoice best{-1, -1, -1};
     const bool fp_choices = p->fp_padded && p->ss >= 1 && p->ss <= 4 && !force_general();
     if (fp_choices) {
         // The forward kernel walks all angles of one detector tile before the
@@ -1353,7 +1505,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
         band.fp_padded = p->fp_padded;
         band.fp_pair = p->fp_pair;
         band.dev = t.band_angles;
-        auto time_band = [&](int cand, int band_rows, float *ms) -> int {
+        auto time_band = [&](int cand, int by, int band_rows, float *ms) -> int {
             const int r0 = (p->rows - band_rows) / 2;
             for (int a = 0; a < p->n_angles; ++a)
                 for (int k = 0; k < 3; ++k) shifted[a].P0[k] = fpp->host[a].P0[k] + r0 * fpp->host[a].V[k];
@@ -1361,32 +1513,60 @@ int autotune(tsx_plan *p, hipStream_t stream)
                                          hipMemcpyHostToDevice, stream));
             band.rows = band_rows;
             band.fp_variant = cand;
+            band.fp_by = by;
             return time_launches([&] { return launch_fp(&band, vol, 0, (float *)t.band_proj, 0, 1, ep, stream); },
                                  stream, t.e0, t.e1, 2, ms);
         };
-        double best_ms = 1e300;
-        for (int cand : {4, 5, 3, 2}) {
-            if (cand == 4 && !p->fp_pair) continue;
+        auto predict = [&](int cand, int by, double *out) -> int {
             float ms_hi = 0.0f, ms_lo = 0.0f;
-            int rc = time_band(cand, band_hi, &ms_hi);
+            int rc = time_band(cand, by, band_hi, &ms_hi);
             if (rc) return rc;
-            double predicted = ms_hi;
+            *out = ms_hi;
             if (band_hi < p->rows) {
-                rc = time_band(cand, band_lo, &ms_lo);
+                rc = time_band(cand, by, band_lo, &ms_lo);
                 if (rc) return rc;
                 const double per_row = std::max(0.0, (double)(ms_hi - ms_lo) / (band_hi - band_lo));
                 const double fixed = std::max(0.0, ms_hi - per_row * band_hi);
-                predicted = fixed + per_row * p->rows;
+                *out = fixed + per_row * p->rows;
             }
+            return TSX_OK;
+        };
+        // the march on tiles of 8 rows, then the tile rows of the chosen march
+        double best_ms = 1e300;
+        for (int cand : {4, 5, 3, 2}) {
+            if (cand == 4 && !p->fp_pair) continue;
+            double predicted = 0.0;
+            int rc = predict(cand, 8, &predicted);
+            if (rc) return rc;
             if (predicted < best_ms) {
                 best_ms = predicted;
                 best.fp_variant = cand;
+                best.fp_by = 8;
+            }
+        }
+        if (p->ss == 1 && (best.fp_variant == 3 || best.fp_variant == 4)) {
+            // Tile rows change how many blocks share a detector tile's volume
+            // data and how the short rays near the detector's edges group;
+            // they are timed on the whole plan (every angle and row, at most
+            // 2 GiB of projections: a band through the middle otherwise).
+            // Bands of 128 and 256 middle rows preferred 4-row tiles, 4 %
+            // slower than 8-row ones on the whole 512^3 plan.
+            const int band_tile = tile_band_rows(p);
+            float tile_best = 1e30f;
+            for (int by : {8, 4, 16}) {
+                float ms = 0.0f;
+                int rc = time_band(best.fp_variant, by, band_tile, &ms);
+                if (rc) return rc;
+                if (ms < tile_best) {
+                    tile_best = ms;
+                    best.fp_by = by;
+                }
             }
         }
     }
     if (p->bp_separable && !force_general()) {
         float best_ms = 1e30f;
-        for (int bz : {32, 16}) {
+        for (int bz : {32, 24, 16}) {
             sub.bp_bz = bz;
             float ms = 0.0f;
             int rc = time_launches([&] { return launch_bp(&sub, vol, 0, proj, 0, 1, ep, stream); },
@@ -1401,6 +1581,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
     TSX_HIP_CHECK(hipStreamSynchronize(stream));
     p->fp_variant = best.fp_variant;
     p->bp_bz = best.bp_bz;
+    p->fp_by = best.fp_by;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = best;
     return TSX_OK;
@@ -1608,6 +1789,7 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->bp_brick_z = bp_brick_z_of(p);
     info->tuned = p->fp_variant >= 0 || p->bp_bz > 0;
     info->fp_slabs = info->fp_kernel ? std::max<int32_t>(1, (int32_t)p->slabs.size()) : 0;
+    info->fp_tile_rows = info->fp_kernel ? fp_tile_rows_of(p) : 0;
     info->bp_launch_angles = info->bp_kernel ? bp_chunk_of(p) : 0;
     return TSX_OK;
 }
