@@ -303,7 +303,6 @@ struct ScaleJob {
 struct ScaleJobs {
     ScaleJob job[MAX_FUSED];
     int n;
-    uint32_t *zero;  // if not null: CLAIM_COUNTERS counters set to 0 (claims of the next fused launch)
 };
 
 __host__ __device__ inline int64_t scale_job_blocks(const ScaleJob &j)
@@ -314,10 +313,6 @@ __host__ __device__ inline int64_t scale_job_blocks(const ScaleJob &j)
 __global__ void __launch_bounds__(256) scale_by_length_t_multi(ScaleJobs S)
 {
     __shared__ float tile[32][33];
-    if (S.zero && blockIdx.x == 0 && blockIdx.y == 0) {
-        const int t = threadIdx.y * 32 + threadIdx.x;
-        if (t < CLAIM_COUNTERS) S.zero[CLAIM_STRIDE * t] = 0u;
-    }
     const ScaleJob &J = S.job[blockIdx.y];
     const int r = blockIdx.x;
     if (r >= J.blocks) return;
@@ -824,27 +819,24 @@ __global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(bp_
 }
 
 
-// Fused persistent launch (persistent_blocks, as fp_padded_multi).
+// Fused launch over up to MAX_FUSED instances, one block per logical id in
+// the XCD order (as fp_padded_multi; a persistent form cost the mixed
+// serving stream 3 % and gained nothing at 512^3).
 struct BpMulti {
     BpView v[MAX_FUSED];
     int begin[MAX_FUSED + 1];
     int n;
-    int shift;        // chunk size of the XCD order (xcd_logical_of)
-    unsigned vgrid;   // virtual blocks
-    uint32_t *claim;  // CLAIM_COUNTERS counters, zero at the start of the launch
+    int shift;  // chunk size of the XCD order (xcd_logical)
 };
 
 template <int BZ>
 __global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu(BZ)))) bp_wave_multi(BpMulti m)
 {
     __shared__ __align__(16) BpShared<BZ> sh;
-    const int total = m.begin[m.n];
-    persistent_blocks(m.vgrid, m.claim, [&](unsigned vb) {
-        const int logical = xcd_logical_of(vb, m.shift);
-        if (logical >= total) return;
-        const int k = owner_of(m.begin, logical);
-        bp_wave_work<BZ>(m.v[k], logical - m.begin[k], sh);
-    });
+    const int logical = xcd_logical(m.shift);
+    if (logical >= m.begin[m.n]) return;
+    const int k = owner_of(m.begin, logical);
+    bp_wave_work<BZ>(m.v[k], logical - m.begin[k], sh);
 }
 
 }  // namespace tsx

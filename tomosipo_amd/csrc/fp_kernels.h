@@ -251,11 +251,6 @@ __device__ __forceinline__ int owner_of(const int (&begin)[N], int i)
 // table of first blocks instead, every block's table loads doubled the time
 // of these short blocks.)
 constexpr int MAX_LAYOUT_JOBS = 48;
-// claim counters of the persistent launches (persistent_blocks): one per
-// XCD, CLAIM_STRIDE uint32 (256 bytes) apart
-constexpr int CLAIM_STRIDE = 64;
-constexpr int CLAIM_COUNTERS = 8;
-constexpr int CLAIM_BYTES = CLAIM_COUNTERS * CLAIM_STRIDE * 4;
 
 struct LayoutJob {
     const float *src;
@@ -269,7 +264,6 @@ struct LayoutJob {
 struct LayoutJobs {
     LayoutJob job[MAX_LAYOUT_JOBS];
     int n;
-    uint32_t *zero;  // if not null: CLAIM_COUNTERS counters set to 0 (claims of the next fused launch)
 };
 
 // Blocks of job `j` in build_layouts_multi.
@@ -287,10 +281,6 @@ __host__ __device__ inline int64_t layout_job_blocks(const LayoutJob &j)
 template <bool TRANSPOSE>
 __global__ void __launch_bounds__(256) build_layouts_multi(LayoutJobs L)
 {
-    if (L.zero && blockIdx.x == 0 && blockIdx.y == 0) {
-        const int t = threadIdx.y * 32 + threadIdx.x;
-        if (t < CLAIM_COUNTERS) L.zero[CLAIM_STRIDE * t] = 0u;
-    }
     const LayoutJob &J = L.job[blockIdx.y];
     const int r = blockIdx.x;
     if (r >= J.blocks) return;
@@ -849,50 +839,14 @@ __global__ void __launch_bounds__(FP_BX *BY) fp_padded(FpView v, int n_blocks, i
     fp_padded_work<SS, PIPE, BY>(v, logical);
 }
 
-// Fused persistent launches. The grid holds a fixed number of blocks per CU
-// (a multiple of 8 in all), and they work through a virtual grid of vgrid
-// blocks (xcd_grid; virtual block vb = 8 j + x belongs to XCD x, and the
-// hardware places block b on XCD b % 8). Block b runs j = b / 8 of XCD b % 8
-// first, then claims the next unclaimed j of its XCD from that XCD's counter
-// (claim[CLAIM_STRIDE x], zeroed by the launch's pre-pass kernel). The
-// blocks of an XCD thus take its virtual blocks in order, as the hardware
-// dispatcher hands out the blocks of a launch of one block per virtual
-// block, and the blocks at work at a time stay on neighbouring logical ids.
-// Every virtual block is run once. Measured at 512^3 (forward, 8-row tiles,
-// one instance): a fixed stride instead of claims, 3x slower (blocks of
-// cheap and dear work items drift apart); claims of 16 virtual blocks at a
-// time, 2.8x; claims spread over 8 counters per XCD, 2.3x; one counter per
-// XCD: 62, 44.3 and 72 ms with 3, 2 and 1 blocks of 8 waves per CU against
-// 43.3 ms for the launch of one block per virtual block (the grid's blocks
-// per CU are set by persistent_grid). A claim is issued when the block
-// starts a work item and read when it ends it.
-template <typename F>
-__device__ __forceinline__ void persistent_blocks(unsigned vgrid, uint32_t *__restrict__ claim, F &&work)
-{
-    __shared__ unsigned nxt[2];
-    const unsigned x = blockIdx.x % 8u, per_xcd = gridDim.x / 8u, jn = vgrid / 8u;
-    const bool leader = threadIdx.x == 0 && threadIdx.y == 0;
-    unsigned j = blockIdx.x / 8u;
-    for (int it = 0; j < jn; ++it) {
-        unsigned c = 0;
-        if (leader) c = atomicAdd(&claim[CLAIM_STRIDE * x], 1u);
-        work(j * 8u + x);
-        if (leader) nxt[it & 1] = c + per_xcd;
-        __syncthreads();
-        // uniform by construction; said so, so that what derives from it
-        // (instance, angle, buffer descriptors) stays in scalar registers
-        j = __builtin_amdgcn_readfirstlane(nxt[it & 1]);
-    }
-}
-
 // Fused launch over up to MAX_FUSED instances (any shapes): instance k owns
 // the logical ids [begin[k], begin[k + 1]).
 constexpr int MAX_FUSED = 16;
 
-// The fused forward projector takes one block per logical id in the XCD
-// order (not persistent_blocks: at 512^3 x 16 instances its persistent form
-// took 49.4 ms per instance at its best, 2 blocks of 8 waves per CU, against
-// 43.6 ms here).
+// One block per logical id in the XCD order, as the single launches. (A
+// persistent form, a resident grid claiming the next logical id of its XCD
+// from a per-XCD counter, measured slower: 49.4 against 43.6 ms per 512^3
+// instance at its best.)
 struct FpMulti {
     FpView v[MAX_FUSED];
     int begin[MAX_FUSED + 1];

@@ -766,19 +766,14 @@ void pad_begin(int (&begin)[N], int n)
 // Padded layouts of several volumes, built by build_layouts_multi in as few
 // launches as its job lists and the dispatch limit allow: one list of x
 // layouts (transposes), one of y and z layouts (copies). add() collects,
-// flush() launches what was collected; the first launch of a flush zeroes
-// `zero` (the claim counters of a persistent launch that follows), if set.
+// flush() launches what was collected.
 struct LayoutBatch {
     LayoutJobs L[2];  // [0]: copies, [1]: transposes
     int maxb[2] = {0, 0};
     hipStream_t stream;
-    uint32_t *zero;
-    explicit LayoutBatch(hipStream_t s, uint32_t *z = nullptr) : stream(s), zero(z)
+    explicit LayoutBatch(hipStream_t s) : stream(s)
     {
-        for (LayoutJobs &l : L) {
-            l.n = 0;
-            l.zero = nullptr;
-        }
+        for (LayoutJobs &l : L) l.n = 0;
     }
     void add(const tsx_plan *p, int es, const float *src, float *dst, const int64_t offs[3])
     {
@@ -809,7 +804,6 @@ struct LayoutBatch {
     {
         LayoutJobs &l = L[t];
         if (l.n == 0) return;
-        l.zero = zero;
         const dim3 grid((unsigned)maxb[t], (unsigned)l.n);
         if (t)
             hipLaunchKernelGGL(build_layouts_multi<true>, grid, dim3(32, 8), 0, stream, l);
@@ -1108,74 +1102,38 @@ int check_multi(const tsx_plan *const *plans, int n)
     return TSX_OK;
 }
 
-// Scratch of a fused launch: the persistent kernels' claim counters, then
-// each instance's workspace.
+// Scratch of a fused launch: each instance's workspace.
 int64_t multi_workspace_bytes(const tsx_plan *const *plans, int n, int op)
 {
-    int64_t total = align_up(CLAIM_BYTES);
+    int64_t total = 0;
     for (int k = 0; k < n; ++k) total += align_up(workspace_bytes(plans[k], op, 1));
     return total;
 }
 
-// Blocks of `kernel` (`threads` per block) that fit one CU at once and the
-// device's CU count (cached per kernel and device).
-void resident_blocks(const void *kernel, int threads, int device, int *per_cu, int *cus)
-{
-    static std::mutex mu;
-    static std::map<std::pair<const void *, int>, std::pair<int, int>> cache;
-    std::lock_guard<std::mutex> lk(mu);
-    const auto key = std::make_pair(kernel, device);
-    auto it = cache.find(key);
-    if (it == cache.end()) {
-        int pc = 0, n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kernel, threads, 0) != hipSuccess ||
-            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
-            (void)hipGetLastError();
-            pc = 1;
-            n = 256;
-        }
-        it = cache.emplace(key, std::make_pair(std::max(1, pc), std::max(1, n))).first;
-    }
-    *per_cu = it->second.first;
-    *cus = it->second.second;
-}
-
-// Grid of a persistent launch over vgrid virtual blocks (a multiple of 8):
-// the blocks of `kernel` that fit a CU at once (occupancy query), at most
-// 16 waves per CU, on every CU; no more than vgrid. (The backprojector's
-// 4-wave blocks: 4 per CU, 42 ms at 512^3 against 73 ms with 2. The forward
-// projector's 8-wave blocks ran best at 2 per CU, 44.3 ms against 62 ms
-// with the 3 that fit, where the query reports 4.)
-unsigned persistent_grid(const void *kernel, int threads, int device, unsigned vgrid)
-{
-    const int waves = (threads + 63) / 64;
-    int per_cu = 0, cus = 0;
-    resident_blocks(kernel, threads, device, &per_cu, &cus);
-    per_cu = std::max(1, std::min(per_cu, 16 / waves));
-    const unsigned grid = (unsigned)std::max(8, per_cu * cus / 8 * 8);
-    return std::min(vgrid, grid);
-}
-
 // Fused launches: consecutive instances on the fast kernels with the same
-// kernel variant share one persistent launch (up to MAX_FUSED instances, as
-// many logical blocks as an int counts), their padded layouts (forward) or
-// transposed projections (backward) built before it in one launch; an
-// instance on the general kernels, with detector supersampling, in slabs or
-// angle ranges gets its own launch(es). eps[k] is the epilogue of instance k.
+// kernel variant share one launch (up to MAX_FUSED instances, as many blocks
+// as the dispatch limit takes), their padded layouts (forward: one launch of
+// transposes, one of copies) or transposed projections (backward: one
+// launch) built before it; an instance on the general kernels, with
+// detector supersampling, in slabs or angle ranges, or with more blocks than
+// one launch takes, gets its own launch(es). eps[k] is the epilogue of
+// instance k.
 int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *const *projs,
              const Epilogue *eps, int n, void *wsp, int64_t ws_bytes, hipStream_t stream)
 {
     const int64_t need = multi_workspace_bytes(plans, n, TSX_OP_FP);
     Scratch owned;
     void *wptr = nullptr;
-    int rc0 = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
-    if (rc0) return rc0;
+    if (need > 0) {
+        int rc = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
+        if (rc) return rc;
+    }
     const bool fast = !force_general();
     FpMulti m;
     m.n = 0;
     m.begin[0] = 0;
     LayoutBatch layouts(stream);
-    int64_t woff = align_up(CLAIM_BYTES);
+    int64_t woff = 0;
     int variant = -1, by = 8;
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
@@ -1249,18 +1207,18 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
     const int64_t need = multi_workspace_bytes(plans, n, TSX_OP_BP);
     Scratch owned;
     void *wptr = nullptr;
-    int rc0 = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
-    if (rc0) return rc0;
+    if (need > 0) {
+        int rc = get_workspace(plans[0], need, wsp, ws_bytes, stream, owned, &wptr);
+        if (rc) return rc;
+    }
     const bool fast = !force_general();
     int BZ = 32;
     BpMulti m;
     m.n = 0;
     m.begin[0] = 0;
-    m.claim = (uint32_t *)wptr;
     ScaleJobs sj;
-    sj.zero = m.claim;  // the transposition launch before a fused launch zeroes the claims
-    int sj_max = 0;     // blocks of the largest job in sj
-    int64_t woff = align_up(CLAIM_BYTES);
+    int sj_max = 0;  // blocks of the largest job in sj
+    int64_t woff = 0;
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
         sj.n = m.n;
@@ -1268,11 +1226,7 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         hipLaunchKernelGGL(scale_by_length_t_multi, dim3((unsigned)sj_max, (unsigned)sj.n), dim3(32, 8), 0, stream,
                            sj);
         sj_max = 0;
-        m.vgrid = xcd_grid(m.begin[m.n], &m.shift);
-        const void *kern = BZ == 32   ? (const void *)bp_wave_multi<32>
-                           : BZ == 24 ? (const void *)bp_wave_multi<24>
-                                      : (const void *)bp_wave_multi<16>;
-        const unsigned grid = persistent_grid(kern, BR * BR, plans[0]->device, m.vgrid);
+        const unsigned grid = xcd_grid(m.begin[m.n], &m.shift);
         if (BZ == 32)
             hipLaunchKernelGGL(bp_wave_multi<32>, dim3(grid), dim3(BR, BR), 0, stream, m);
         else if (BZ == 24)
@@ -1297,14 +1251,15 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         job.cols = p->cols;
         job.proj = projs[k];
         const int64_t tblocks = scale_job_blocks(job);
-        if (!(fast && p->bp_separable && bp_chunk_of(p) == p->n_angles) || blocks > MAX_LOGICAL ||
+        const int64_t cap = max_blocks(BR * BR);
+        if (!(fast && p->bp_separable && bp_chunk_of(p) == p->n_angles) || blocks > cap ||
             tblocks > max_blocks(256)) {
             int rc = launch_bp(p, vols[k], 0, projs[k], 0, 1, eps[k], stream, wptr ? (char *)wptr + woff : nullptr, wb);
             if (rc) return rc;
             woff += wb;
             continue;
         }
-        if (m.n > 0 && (bz != BZ || (int64_t)m.begin[m.n] + blocks > MAX_LOGICAL ||
+        if (m.n > 0 && (bz != BZ || (int64_t)m.begin[m.n] + blocks > cap ||
                         std::max<int64_t>(sj_max, tblocks) * (m.n + 1) > max_blocks(256))) {
             int rc = flush();
             if (rc) return rc;
