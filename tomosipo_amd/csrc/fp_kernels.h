@@ -266,13 +266,22 @@ struct LayoutJobs {
     int n;
 };
 
-// Blocks of job `j` in build_layouts_multi.
+// Elements of the y or z layout of job `j`.
+__host__ __device__ inline int64_t layout_job_elems(const LayoutJob &j)
+{
+    const int64_t X = j.nx + 2 * PAD;
+    return j.axis == 1 ? X * j.ny * (j.nz + 2 * PAD) : X * (j.ny + 2 * PAD) * j.nz;
+}
+
+// Blocks of job `j` in build_layouts_multi: x layouts a block per 32 x 32
+// tile of a plane, y and z layouts a block per 256 consecutive elements of
+// the layout (rows of nx + 4 elements rarely fill blocks of their own: at
+// 256^3 every row took two blocks, the second with 4 threads busy).
 __host__ __device__ inline int64_t layout_job_blocks(const LayoutJob &j)
 {
     if (j.axis == 0)
         return (int64_t)((j.nx + 31) / 32) * ((j.ny + 2 * PAD + 31) / 32) * (j.nz + 2 * PAD);
-    const int64_t rows = j.axis == 1 ? (int64_t)j.ny * (j.nz + 2 * PAD) : (int64_t)(j.ny + 2 * PAD) * j.nz;
-    return (int64_t)((j.nx + 2 * PAD + 255) / 256) * rows;
+    return (layout_job_elems(j) + 255) / 256;
 }
 
 // x layouts (TRANSPOSE, through an LDS tile) or y and z layouts (plain
@@ -291,11 +300,12 @@ __global__ void __launch_bounds__(256) build_layouts_multi(LayoutJobs L)
         pad_transpose_x_at(J.src, J.dst, J.nx, J.ny, J.nz, J.pair, bx * 32, by * 32, zp, threadIdx.x,
                            threadIdx.y, tile);
     } else {
-        const int xb = (J.nx + 2 * PAD + 255) / 256;
-        const int Y = J.axis == 1 ? J.ny : J.ny + 2 * PAD;
-        const int row = r / xb;
-        pad_copy_yz_at(J.src, J.dst, J.nx, J.ny, J.nz, J.axis, J.pair,
-                       (r % xb) * 256 + threadIdx.y * 32 + threadIdx.x, row % Y, row / Y);
+        const uint32_t X = J.nx + 2 * PAD, Y = J.axis == 1 ? J.ny : J.ny + 2 * PAD;
+        const uint32_t e = (uint32_t)r * 256u + threadIdx.y * 32u + threadIdx.x;
+        if ((int64_t)e >= layout_job_elems(J)) return;
+        const uint32_t row = e / X;
+        pad_copy_yz_at(J.src, J.dst, J.nx, J.ny, J.nz, J.axis, J.pair, (int)(e - row * X), (int)(row % Y),
+                       (int)(row / Y));
     }
 }
 

@@ -4,9 +4,10 @@ Requests (forward projection, backprojection or an n-iteration SIRT
 reconstruction, each on its own volume and geometry) arrive over time. The
 server admits a request as soon as it has arrived and its device memory fits
 the HBM budget, keeps its volume and projections resident, and advances every
-admitted request by one projection per tick: all requests whose next step is
-a forward projection go into one fused launch, all whose next step is a
-backprojection into another (``tomosipo_amd.fused``). A SIRT step rides in
+admitted request once per tick: all requests whose next step is a forward
+projection go into one fused launch, then all whose next step is a
+backprojection (a SIRT request's, right after its forward step in the same
+tick) into another (``tomosipo_amd.fused``). A SIRT step rides in
 those launches with its elementwise work fused into the kernels' epilogues
 (forward: ``r = R * (y - A x)``; backprojection: ``x += C * A.T(r)``), and the
 SIRT weights R = 1 / A(1), C = 1 / A.T(1) are computed once per geometry and
@@ -303,13 +304,15 @@ class Server:
                 # arrivals join the next launch instead of queueing behind many
                 while len(ticks) >= 2:
                     ticks.popleft().synchronize()
-                # one tick: one fused forward launch and one fused backprojection launch
+                # one tick: one fused forward launch, then one fused
+                # backprojection launch that also takes the SIRT requests
+                # whose forward step just ran (a SIRT iteration per tick)
                 failed = []
                 fps = [j for j in active if j.steps[0] == "fp"]
-                bps = [j for j in active if j.steps[0] == "bp"]
                 if fps:
                     for j in self._launch(fps, True, records, failed):
                         j.steps.popleft()
+                bps = [j for j in active if j.steps and j.steps[0] == "bp"]
                 if bps:
                     for j in self._launch(bps, False, records, failed):
                         j.steps.popleft()
