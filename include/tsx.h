@@ -135,8 +135,8 @@ int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);
  * Times the kernel variants of the plan's forward projector and
  * backprojector on its own geometry and keeps the fastest for later launches
  * of this plan: the forward march (row-paired, LDS-staged, plain or per-lane
- * on tiles of 8 detector rows, then tiles of 4 and 16 rows for the chosen
- * march where it takes them) on every angle over bands of 64 and 32
+ * on tiles of 8 detector rows, then tiles of 4 rows for the chosen march
+ * where it takes them, kept if more than 3 % faster) on every angle over bands of 64 and 32
  * detector rows, compared on the predicted time of the whole plan (fixed
  * layout-building cost plus cost per row); the backprojector's brick depth
  * (32, 24 or 16 slices) on at most 64 of its angles, in runs of consecutive

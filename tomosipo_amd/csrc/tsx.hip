@@ -842,7 +842,8 @@ int launch_fp_general(const tsx_plan *p, const T *vol, int64_t vs, T *proj, int6
 }
 
 // The padded forward kernels by supersampling, march variant and tile rows
-// (tiles of 4 and 16 rows for variants 3 and 4 without supersampling; 8 rows
+// (tiles of 4 and 16 rows for variants 3 and 4 without supersampling, 16
+// only through TSX_FP_TILE_ROWS; 8 rows
 // otherwise).
 using FpKernel = void (*)(FpView, int, int);
 using FpMultiKernel = void (*)(FpMulti);
@@ -1505,13 +1506,19 @@ int autotune(tsx_plan *p, hipStream_t stream)
             // they are timed on the whole plan (every angle and row, at most
             // 2 GiB of projections: a band through the middle otherwise).
             // Bands of 128 and 256 middle rows preferred 4-row tiles, 4 %
-            // slower than 8-row ones on the whole 512^3 plan.
+            // slower than 8-row ones on the whole 512^3 plan. 4-row tiles
+            // replace the 8-row default only when more than 3 % faster.
+            // (16-row tiles were a candidate: within 2 % of 8-row ones on one
+            // 512^3 plan, ahead or behind from run to run, and 6 % slower in
+            // the fused launch of 16 instances; their kernels stay, for
+            // TSX_FP_TILE_ROWS.)
             const int band_tile = tile_band_rows(p);
             float tile_best = 1e30f;
-            for (int by : {8, 4, 16}) {
+            for (int by : {8, 4}) {
                 float ms = 0.0f;
                 int rc = time_band(best.fp_variant, by, band_tile, &ms);
                 if (rc) return rc;
+                if (by != 8) ms *= 1.03f;
                 if (ms < tile_best) {
                     tile_best = ms;
                     best.fp_by = by;
