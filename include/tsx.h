@@ -115,7 +115,8 @@ typedef struct {
     int64_t plan_bytes;        /* device bytes held by the plan */
     int32_t axis_count[3];     /* number of angles stepping along x, y, z */
     int32_t fp_kernel;         /* 0 = general, 1 = padded per-axis layouts */
-    int32_t bp_kernel;         /* 0 = general, 1 = separable (upright detector) */
+    int32_t bp_kernel;         /* 0 = general, 1 = separable (detector v parallel to a volume
+                                  axis: z, or x or y through bp_axis_swap) */
     int32_t fp_variant;        /* padded forward projector: 2 = per-lane slice ranges,
                                   3 = wave-synchronous slices, 4 = as 3 on row-paired
                                   layouts (one 16-byte load per sample), 5 = as 3 with
@@ -127,6 +128,8 @@ typedef struct {
     int32_t bp_launch_angles;  /* separable backprojector: angles per launch (all of them
                                   unless their transposed projections reach 4 GiB) */
     int32_t fp_tile_rows;      /* padded forward projector: detector rows per block (4, 8, 16) */
+    int32_t bp_axis_swap;      /* separable backprojector run on the plan with volume axes swapped:
+                                  0 = none, 1 = x and z (detector v parallel to x), 2 = y and z */
 } tsx_plan_info;
 
 int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);

@@ -175,7 +175,9 @@ int main()
         w[6] = 0.02;
         w[10] = 0.02;
     }
-    host_plan(geometry(0, zv, 10, 14, 12), -1, 0);
+    // rays along z, detector v along y: the separable backprojector of the
+    // plan with y and z swapped
+    host_plan(geometry(0, zv, 10, 14, 12), -1, 1);
 
     std::printf(g_bad ? "host checks FAILED (%d)\n" : "host checks passed\n", g_bad);
     return g_bad ? 1 : 0;

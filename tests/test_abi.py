@@ -87,6 +87,24 @@ def test_fast_kernels_at_1024_cubed():
     assert small.info()["fp_slabs"] == 1 and small.info()["bp_launch_angles"] == 720
 
 
+def test_swapped_axes_separable_backprojector():
+    """A scan whose detector v axis is parallel to x (the usual scan turned
+    by 90 degrees about y) or to y (turned about x) keeps the separable
+    backprojector, run on the plan with that axis and z swapped; a tilted
+    detector keeps the general one."""
+    vg = ts.volume(shape=(40, 48, 56), size=(40 / 48, 1, 56 / 48))
+    pg = ts.cone(angles=30, shape=(48, 72), size=(np.sqrt(2), 1.5 * np.sqrt(2)), cone_angle=0.5).to_vec()
+    assert _plan(vg, pg).info()["bp_axis_swap"] == 0
+    vx = ts.rotate(pos=0, axis=(0, 1, 0), angles=np.pi / 2) * pg
+    info = _plan(vg, vx).info()
+    assert (info["bp_kernel"], info["bp_axis_swap"]) == ("separable", 1)
+    vy = ts.rotate(pos=0, axis=(0, 0, 1), angles=np.pi / 2) * pg
+    info = _plan(vg, vy).info()
+    assert (info["bp_kernel"], info["bp_axis_swap"]) == ("separable", 2)
+    tilted = ts.rotate(pos=0, axis=(0, 1, 0), angles=1.0) * pg
+    assert _plan(vg, tilted).info()["bp_kernel"] == "general"
+
+
 def test_chunk_limit_override(monkeypatch):
     """TSX_CHUNK_BYTES lowers the bound at plan creation: a 48^3 cone plan
     then takes 3 slabs of 16 (row-paired x layout of a 16-slice slab:

@@ -322,6 +322,45 @@ __global__ void __launch_bounds__(256) scale_by_length_t_multi(ScaleJobs S)
                  (t / tiles_i) * 32, tile);
 }
 
+// The backprojection of a plan with volume axis swap - 1 (x or y) and z
+// exchanged, tmp ([x][y][z] for swap 1, [y][z][x] for swap 2; entries ts
+// floats apart), written to the volume out ([z][y][x], entries vs apart)
+// through the epilogue. swap 1: a block per 32 x 32 tile of the (z, x) plane
+// of one y, through LDS (blockIdx.x = tile + tiles * y); swap 2: a block per
+// 256 x-elements of one (z, y) row, whose x-rows are contiguous in both.
+// blockIdx.z: batch entry.
+__global__ void __launch_bounds__(256) unswap_volume(const float *__restrict__ tmp, int64_t ts,
+                                                      float *__restrict__ out, int64_t vs, int nx, int ny,
+                                                      int nz, int swap, Epilogue ep)
+{
+    const int64_t b = blockIdx.z;
+    const float *src = tmp + b * ts;
+    float *dst = out + b * vs;
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    if (swap == 1) {
+        __shared__ float tile[32][33];
+        const int tiles_x = (nx + 31) / 32, tiles = tiles_x * ((nz + 31) / 32);
+        const int y = blockIdx.x / tiles, t = blockIdx.x % tiles;
+        const int x0 = (t % tiles_x) * 32, z0 = (t / tiles_x) * 32;
+        for (int r = ty; r < 32; r += 8) {  // tmp rows x0 + r, z contiguous
+            const int x = x0 + r, z = z0 + tx;
+            tile[r][tx] = (x < nx && z < nz) ? src[((int64_t)x * ny + y) * nz + z] : 0.0f;
+        }
+        __syncthreads();
+        for (int r = ty; r < 32; r += 8) {  // out rows z0 + r, x contiguous
+            const int z = z0 + r, x = x0 + tx;
+            if (x < nx && z < nz) emit(dst, ((int64_t)z * ny + y) * nx + x, b, tile[tx][r], ep);
+        }
+    } else {
+        const int xb = (nx + 255) / 256;
+        const int64_t row = blockIdx.x / xb;  // (z, y) of out
+        const int x = (int)(blockIdx.x % xb) * 256 + ty * 32 + tx;
+        if (x >= nx) return;
+        const int z = (int)(row / ny), y = (int)(row % ny);
+        emit(dst, row * nx + x, b, src[((int64_t)y * nz + z) * nx + x], ep);
+    }
+}
+
 // Detector column where the ray through in-slice point (n, c) (axes m, c2)
 // meets the detector. Independent of z for an upright detector.
 template <typename AP>

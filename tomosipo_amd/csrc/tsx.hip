@@ -129,6 +129,10 @@ struct tsx_plan {
     int32_t fp_slab_depth = 0;  // slices per slab (nz: no slabs)
     bool bp_separable = false;  // upright detector, x/y stepping, ss == 1
     int32_t bp_chunk = 0;       // separable backprojector: sorted angles per launch
+    // detector v parallel to x (bp_swap 1) or y (2) at every angle: the
+    // separable backprojector of the plan with that axis and z swapped
+    tsx_plan *bp_perm = nullptr;
+    int32_t bp_swap = 0;
     std::string bp_reason;      // why bp_separable is off
     // kernel variants chosen by tsx_plan_autotune (-1: library default)
     int32_t fp_variant = -1, bp_bz = -1, fp_by = -1;
@@ -637,10 +641,15 @@ bool bp_general_prescaled(const tsx_plan *p, int64_t /*batch*/)
     return !p->bp_separable && p->ss == 1;
 }
 
+// Scratch sizes rounded up to 256 bytes.
+int64_t align_ws(int64_t v) { return (v + 255) / 256 * 256; }
+
 int64_t workspace_bytes(const tsx_plan *p, int op, int64_t batch)
 {
     if (force_general()) return 0;
     if (op == TSX_OP_FP) return p->fp_padded ? fp_workspace_elems(p) * batch * 4 : 0;
+    if (p->bp_perm)  // the swapped plan's result, then that plan's workspace
+        return align_ws((int64_t)p->nx * p->ny * p->nz * batch * 4) + workspace_bytes(p->bp_perm, op, batch);
     return p->bp_separable || bp_general_prescaled(p, batch) ? bp_workspace_elems(p) * batch * 4 : 0;
 }
 
@@ -1010,11 +1019,45 @@ int launch_bp_general(const tsx_plan *p, T *vol, int64_t vs, const T *proj, int6
     return TSX_OK;
 }
 
+Epilogue plain(int additive)
+{
+    Epilogue e;
+    e.mode = additive ? EP_ADD : EP_SET;
+    e.y = nullptr;
+    e.w = nullptr;
+    e.stride = 0;
+    return e;
+}
+
 int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int64_t ps,
               int64_t batch, Epilogue ep, hipStream_t stream, void *wsp = nullptr,
               int64_t ws_bytes = 0)
 {
     if (batch <= 0) return TSX_OK;
+    if (p->bp_perm && !force_general()) {
+        // the separable backprojector of the axis-swapped plan, then its
+        // result moved into this plan's layout through the epilogue
+        const int64_t ve = (int64_t)p->nx * p->ny * p->nz;
+        Scratch owned;
+        void *wptr = nullptr;
+        int rc = get_workspace(p, workspace_bytes(p, TSX_OP_BP, batch), wsp, ws_bytes, stream, owned, &wptr);
+        if (rc) return rc;
+        float *tmp = (float *)wptr;
+        char *rest = (char *)wptr + align_ws(ve * batch * 4);
+        rc = launch_bp(p->bp_perm, tmp, ve, proj, ps, batch, plain(0), stream, rest,
+                       workspace_bytes(p->bp_perm, TSX_OP_BP, batch));
+        if (rc) return rc;
+        const unsigned blocks = p->bp_swap == 1
+            ? (unsigned)(((p->nx + 31) / 32) * ((p->nz + 31) / 32) * (int64_t)p->ny)
+            : (unsigned)(((p->nx + 255) / 256) * (int64_t)p->ny * p->nz);
+        for (int64_t b0 = 0; b0 < batch; b0 += GRID_DIM_MAX) {
+            const int64_t nb = std::min(GRID_DIM_MAX, batch - b0);
+            hipLaunchKernelGGL(unswap_volume, dim3(blocks, 1, (unsigned)nb), dim3(32, 8), 0, stream, tmp + b0 * ve,
+                               ve, vol + b0 * vs, vs, p->nx, p->ny, p->nz, p->bp_swap, ep_from(ep, b0));
+        }
+        TSX_HIP_CHECK(hipGetLastError());
+        return TSX_OK;
+    }
     const int64_t sino = (int64_t)p->rows * p->n_angles * p->cols;
     if (!p->bp_separable || force_general()) {
         if (!force_general() && bp_general_prescaled(p, batch)) {
@@ -1065,15 +1108,6 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
     return TSX_OK;
 }
 
-Epilogue plain(int additive)
-{
-    Epilogue e;
-    e.mode = additive ? EP_ADD : EP_SET;
-    e.y = nullptr;
-    e.w = nullptr;
-    e.stride = 0;
-    return e;
-}
 
 template <typename T>
 int launch_general(const tsx_plan *p, bool forward, T *vol, int64_t vs, T *proj, int64_t ps,
@@ -1598,6 +1632,42 @@ int build_slabs(const tsx_geometry *g, tsx_plan *p, int device)
     return TSX_OK;
 }
 
+// 1 if the detector's v axis is parallel to x at every angle, 2 if to y,
+// else 0.
+int v_parallel_swap(const tsx_geometry *g)
+{
+    for (int axis = 0; axis < 2; ++axis) {
+        bool all = true;
+        for (int a = 0; a < g->n_angles && all; ++a) {
+            const double *v = g->vectors + 12 * (int64_t)a + 9;
+            const double vn = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+            for (int c = 0; c < 3; ++c)
+                if (c != axis && !(std::fabs(v[c]) <= 1e-12 * vn)) all = false;
+        }
+        if (all) return axis + 1;
+    }
+    return 0;
+}
+
+// The geometry with volume axis swap - 1 (x or y) and z exchanged: grid
+// counts, window and every vector's components; `vec` holds the vectors.
+tsx_geometry swapped_geometry(const tsx_geometry *g, int swap, std::vector<double> &vec)
+{
+    const int a = swap - 1;
+    tsx_geometry sg = *g;
+    vec.assign(g->vectors, g->vectors + 12 * (int64_t)g->n_angles);
+    for (int64_t k = 0; k < 4 * (int64_t)g->n_angles; ++k) std::swap(vec[3 * k + a], vec[3 * k + 2]);
+    sg.vectors = vec.data();
+    int32_t n[3] = {g->grid_col_count, g->grid_row_count, g->grid_slice_count};
+    std::swap(n[a], n[2]);
+    sg.grid_col_count = n[0];
+    sg.grid_row_count = n[1];
+    sg.grid_slice_count = n[2];
+    std::swap(sg.window_min[a], sg.window_min[2]);
+    std::swap(sg.window_max[a], sg.window_max[2]);
+    return sg;
+}
+
 // The upright FDK backprojector reads row-paired filtered projections (8 bytes
 // per pixel, addressed with 32-bit offsets); the general one plain ones.
 bool fdk_paired(const tsx_plan *p)
@@ -1712,6 +1782,21 @@ int tsx_plan_create(const tsx_geometry *g, int device, tsx_plan **out)
         g_last_error = msg;
         return rc;
     }
+    // a detector whose v axis is parallel to x or y: the separable
+    // backprojector of the same scan with that axis and z swapped
+    const int swap = p->bp_separable || p->ss != 1 ? 0 : v_parallel_swap(g);
+    if (swap) {
+        std::vector<double> vec;
+        tsx_geometry sg = swapped_geometry(g, swap, vec);
+        tsx_plan *q = nullptr;
+        if (tsx_plan_create(&sg, device, &q) == TSX_OK && q->bp_separable) {
+            p->bp_perm = q;
+            p->bp_swap = swap;
+        } else {
+            tsx_plan_destroy(q);
+        }
+        g_last_error.clear();
+    }
     *out = p;
     return TSX_OK;
 }
@@ -1720,6 +1805,7 @@ int tsx_plan_destroy(tsx_plan *p)
 {
     if (!p) return TSX_OK;
     for (tsx_plan *s : p->slabs) tsx_plan_destroy(s);
+    tsx_plan_destroy(p->bp_perm);
     if (p->dev) {
         DeviceGuard guard(p->device);
         (void)hipFree(p->dev);
@@ -1740,19 +1826,22 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->vol_shape[0] = p->nz;
     info->vol_shape[1] = p->ny;
     info->vol_shape[2] = p->nx;
-    info->plan_bytes = p->dev ? (int64_t)(sizeof(AngleParams) * 2 * p->host.size()) *
+    tsx_plan_info sub;
+    const int64_t perm_bytes = p->bp_perm && tsx_plan_info_get(p->bp_perm, &sub) == TSX_OK ? sub.plan_bytes : 0;
+    info->plan_bytes = perm_bytes + (p->dev ? (int64_t)(sizeof(AngleParams) * 2 * p->host.size()) *
                                         (1 + (int64_t)p->slabs.size()) +
                                     (p->bp_rt ? p->bp_rt_stride * p->n_angles * 4 : 0)
-                              : 0;
+                              : 0);
     for (int c = 0; c < 3; ++c) info->axis_count[c] = p->axis_count[c];
     info->fp_kernel = p->fp_padded && !force_general() ? 1 : 0;
-    info->bp_kernel = p->bp_separable && !force_general() ? 1 : 0;
+    info->bp_kernel = (p->bp_separable || p->bp_perm) && !force_general() ? 1 : 0;
     info->fp_variant = fp_variant_of(p);
-    info->bp_brick_z = bp_brick_z_of(p);
+    info->bp_brick_z = bp_brick_z_of(p->bp_perm ? p->bp_perm : p);
     info->tuned = p->fp_variant >= 0 || p->bp_bz > 0;
     info->fp_slabs = info->fp_kernel ? std::max<int32_t>(1, (int32_t)p->slabs.size()) : 0;
     info->fp_tile_rows = info->fp_kernel ? fp_tile_rows_of(p) : 0;
-    info->bp_launch_angles = info->bp_kernel ? bp_chunk_of(p) : 0;
+    info->bp_launch_angles = info->bp_kernel ? bp_chunk_of(p->bp_perm ? p->bp_perm : p) : 0;
+    info->bp_axis_swap = info->bp_kernel ? p->bp_swap : 0;
     return TSX_OK;
 }
 
@@ -1820,7 +1909,9 @@ int tsx_plan_autotune(tsx_plan *p, void *stream)
     TraceRange trace("tsx_plan_autotune");
     DeviceGuard dg(p->device);
     if (!dg.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
-    return autotune(p, (hipStream_t)stream);
+    rc = autotune(p, (hipStream_t)stream);
+    if (rc == TSX_OK && p->bp_perm) rc = autotune(p->bp_perm, (hipStream_t)stream);
+    return rc;
 }
 
 int tsx_plan_axes(const tsx_plan *p, int32_t *axes)
