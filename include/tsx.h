@@ -130,6 +130,9 @@ typedef struct {
     int32_t fp_tile_rows;      /* padded forward projector: detector rows per block (4, 8, 16) */
     int32_t bp_axis_swap;      /* separable backprojector run on the plan with volume axes swapped:
                                   0 = none, 1 = x and z (detector v parallel to x), 2 = y and z */
+    int32_t fp_axis_swap;      /* padded forward projector run on that plan (the volume moved
+                                  into its layout first): 0 = none, 1 or 2 as bp_axis_swap; the
+                                  fp_* fields above then describe that plan's kernels */
 } tsx_plan_info;
 
 int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);

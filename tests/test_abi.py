@@ -90,19 +90,24 @@ def test_fast_kernels_at_1024_cubed():
 def test_swapped_axes_separable_backprojector():
     """A scan whose detector v axis is parallel to x (the usual scan turned
     by 90 degrees about y) or to y (turned about x) keeps the separable
-    backprojector, run on the plan with that axis and z swapped; a tilted
-    detector keeps the general one."""
+    backprojector and the padded forward projector, both run on the plan
+    with that axis and z swapped; a tilted detector keeps the general
+    backprojector and its own forward layouts."""
     vg = ts.volume(shape=(40, 48, 56), size=(40 / 48, 1, 56 / 48))
     pg = ts.cone(angles=30, shape=(48, 72), size=(np.sqrt(2), 1.5 * np.sqrt(2)), cone_angle=0.5).to_vec()
-    assert _plan(vg, pg).info()["bp_axis_swap"] == 0
+    info = _plan(vg, pg).info()
+    assert info["bp_axis_swap"] == info["fp_axis_swap"] == 0
     vx = ts.rotate(pos=0, axis=(0, 1, 0), angles=np.pi / 2) * pg
     info = _plan(vg, vx).info()
-    assert (info["bp_kernel"], info["bp_axis_swap"]) == ("separable", 1)
+    assert (info["bp_kernel"], info["bp_axis_swap"], info["fp_kernel"], info["fp_axis_swap"]) == \
+        ("separable", 1, "padded", 1)
     vy = ts.rotate(pos=0, axis=(0, 0, 1), angles=np.pi / 2) * pg
     info = _plan(vg, vy).info()
-    assert (info["bp_kernel"], info["bp_axis_swap"]) == ("separable", 2)
+    assert (info["bp_kernel"], info["bp_axis_swap"], info["fp_kernel"], info["fp_axis_swap"]) == \
+        ("separable", 2, "padded", 2)
     tilted = ts.rotate(pos=0, axis=(0, 1, 0), angles=1.0) * pg
-    assert _plan(vg, tilted).info()["bp_kernel"] == "general"
+    info = _plan(vg, tilted).info()
+    assert (info["bp_kernel"], info["fp_kernel"], info["fp_axis_swap"]) == ("general", "padded", 0)
 
 
 def test_chunk_limit_override(monkeypatch):

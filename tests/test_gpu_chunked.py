@@ -46,16 +46,20 @@ def _cases():
     # rotated about y by 1.2 rad: some angles step along z (slabs cut their
     # slice range) and the backprojector is the general one
     tilted = ts.rotate(pos=0, axis=(0, 1, 0), angles=1.2) * cone.to_vec()
-    return {"cone": (vg, cone), "parallel": (vg, par), "tilted": (vg, tilted)}
+    # turned 90 degrees about y: both projectors run on the plan with x and z
+    # swapped, whose slabs and angle ranges are cut by the same bound
+    turned = ts.rotate(pos=0, axis=(0, 1, 0), angles=np.pi / 2) * cone.to_vec()
+    return {"cone": (vg, cone), "parallel": (vg, par), "tilted": (vg, tilted), "turned": (vg, turned)}
 
 
-@pytest.mark.parametrize("case", ["cone", "parallel", "tilted"])
+@pytest.mark.parametrize("case", ["cone", "parallel", "tilted", "turned"])
 def test_slabs_and_angle_ranges_match_oracle(monkeypatch, case):
     vg, pg = _cases()[case]
     A, info = _chunked_operator(monkeypatch, vg, pg, 400000)
     print(f"[chunked] {case}: {info['fp_slabs']} slabs, bp {info['bp_kernel']} "
-          f"{info['bp_launch_angles']} angles per launch", flush=True)
+          f"{info['bp_launch_angles']} angles per launch, axis swap {info['fp_axis_swap']}", flush=True)
     assert info["fp_kernel"] == "padded" and info["fp_slabs"] == 3
+    assert info["fp_axis_swap"] == info["bp_axis_swap"] == (1 if case == "turned" else 0)
     if case != "tilted":
         assert info["bp_kernel"] == "separable" and info["bp_launch_angles"] == 25
     x = ts.phantom.box_noise((48, 48, 48), 12, 30, 0.01, seed=11)

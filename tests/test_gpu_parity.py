@@ -512,33 +512,68 @@ def test_autotune_keeps_parity_and_caches_choices():
 
 
 @pytest.mark.parametrize("swap", [1, 2])
-def test_swapped_axes_backprojector(swap):
+def test_swapped_axes_projectors(swap):
     """Detector v parallel to x (swap 1: the scan turned 90 degrees about y)
-    or to y (swap 2: about x): the separable backprojector of the plan with
-    that axis and z swapped, its result moved back through the epilogue.
-    Parity with the oracle and the matched adjoint (1e-5), agreement with the
-    general backprojector of the same plan (1e-6), ADD and scaled-add
-    epilogues and a batch of two."""
+    or to y (swap 2: about x): both projectors run on the plan with that axis
+    and z swapped -- the forward projector on the volume moved into that
+    plan's layout, the separable backprojector's result moved back through
+    the epilogue. Parity with the oracle and the matched adjoint (1e-5),
+    agreement with the general kernels of the same plan (forward 1e-5: other
+    sample order; backward 1e-6), ADD, residual and scaled-add epilogues, a
+    batch of two, and a fused launch that holds the instance."""
     import os
+    from tomosipo_amd.fused import fused_forward, fused_backward
+
     vg = ts.volume(shape=(40, 48, 56), size=(40 / 48, 1, 56 / 48))
     base = ts.cone(angles=30, shape=(48, 72), size=(np.sqrt(2), 1.5 * np.sqrt(2)), cone_angle=0.5).to_vec()
     axis = (0, 1, 0) if swap == 1 else (0, 0, 1)
     pg = ts.rotate(pos=0, axis=axis, angles=np.pi / 2) * base
     A = ts.operator(vg, pg)
     plan = A.projector.plan(0)
-    assert plan.info()["bp_axis_swap"] == swap
+    info = plan.info()
+    assert info["bp_axis_swap"] == swap and info["fp_axis_swap"] == swap and info["fp_kernel"] == "padded"
     x = ts.phantom.box_noise(vg.shape, 10, 30, 0.01, seed=21)
     _check_fp_bp(vg, pg, x)
     g = torch.Generator(device=DEV)
     g.manual_seed(22)
     y = torch.rand(A.range_shape, generator=g, device=DEV)
+    xd = torch.from_numpy(x).to(DEV)
+    fx = A(xd)
     b = A.T(y)
     os.environ["TSX_FORCE_GENERAL"] = "1"
     try:
+        assert plan.info()["fp_axis_swap"] == 0
+        fx_gen = A(xd)
         b_gen = A.T(y)
     finally:
         del os.environ["TSX_FORCE_GENERAL"]
+    assert (torch.linalg.norm(fx - fx_gen) / torch.linalg.norm(fx_gen)).item() <= 1e-5
     assert (torch.linalg.norm(b - b_gen) / torch.linalg.norm(b_gen)).item() <= 1e-6
+    s = torch.cuda.current_stream().cuda_stream
+    accp = torch.rand(A.range_shape, generator=g, device=DEV)
+    op_ = accp.clone()
+    plan.forward(xd.data_ptr(), op_.data_ptr(), True, s)
+    assert (torch.linalg.norm(op_ - accp - fx) / torch.linalg.norm(fx)).item() <= 1e-6
+    w = torch.rand(A.range_shape, generator=g, device=DEV)
+    r = torch.empty(A.range_shape, device=DEV)
+    plan.forward_residual(xd.data_ptr(), y.data_ptr(), w.data_ptr(), r.data_ptr(), s)
+    assert (torch.linalg.norm(r - w * (y - fx)) / torch.linalg.norm(w * (y - fx))).item() <= 1e-6
+    xs = torch.rand((2,) + tuple(A.domain_shape), generator=g, device=DEV)
+    fs = torch.empty((2,) + tuple(A.range_shape), device=DEV)
+    plan.forward(xs.data_ptr(), fs.data_ptr(), False, s, batch=2, vol_stride=xs[0].numel(),
+                 proj_stride=fs[0].numel())
+    for k in range(2):
+        assert torch.equal(fs[k], A(xs[k]))
+    other = ts.operator(ts.volume(shape=24, size=1),
+                        ts.cone(angles=20, shape=(24, 36), size=(1.4, 2.1), cone_angle=0.5))
+    ops = [other, A, other]
+    ins = [torch.rand(op.domain_shape, generator=g, device=DEV) for op in ops]
+    outs = fused_forward(ops, ins)
+    for op, xi, yi in zip(ops, ins, outs):
+        assert torch.equal(yi, op(xi))
+    backs = fused_backward(ops, outs)
+    for op, yi, bi in zip(ops, outs, backs):
+        assert torch.equal(bi, op.T(yi))
     s = torch.cuda.current_stream().cuda_stream
     acc = torch.rand(A.domain_shape, generator=g, device=DEV)
     o = acc.clone()

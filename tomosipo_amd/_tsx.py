@@ -100,6 +100,7 @@ class TsxPlanInfo(ctypes.Structure):
         ("bp_launch_angles", ctypes.c_int32),
         ("fp_tile_rows", ctypes.c_int32),
         ("bp_axis_swap", ctypes.c_int32),
+        ("fp_axis_swap", ctypes.c_int32),
     ]
 
 
@@ -226,7 +227,7 @@ class Plan:
             "fp_variant": inf.fp_variant, "bp_brick_z": inf.bp_brick_z,
             "tuned": bool(inf.tuned),
             "fp_slabs": inf.fp_slabs, "bp_launch_angles": inf.bp_launch_angles,
-            "fp_tile_rows": inf.fp_tile_rows, "bp_axis_swap": inf.bp_axis_swap,
+            "fp_tile_rows": inf.fp_tile_rows, "bp_axis_swap": inf.bp_axis_swap, "fp_axis_swap": inf.fp_axis_swap,
         }
 
     def fdk_workspace_size(self):

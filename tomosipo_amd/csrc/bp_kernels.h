@@ -322,25 +322,28 @@ __global__ void __launch_bounds__(256) scale_by_length_t_multi(ScaleJobs S)
                  (t / tiles_i) * 32, tile);
 }
 
-// The backprojection of a plan with volume axis swap - 1 (x or y) and z
-// exchanged, tmp ([x][y][z] for swap 1, [y][z][x] for swap 2; entries ts
-// floats apart), written to the volume out ([z][y][x], entries vs apart)
-// through the epilogue. swap 1: a block per 32 x 32 tile of the (z, x) plane
-// of one y, through LDS (blockIdx.x = tile + tiles * y); swap 2: a block per
-// 256 x-elements of one (z, y) row, whose x-rows are contiguous in both.
-// blockIdx.z: batch entry.
+// A volume in the layout with axis swap - 1 (x or y) and z exchanged, tmp
+// ([x][y][z] for swap 1, [y][z][x] for swap 2; entries ts floats apart),
+// written to out ([z][y][x] of extents nx, ny, nz; entries vs apart) through
+// the epilogue. The exchange is its own inverse: with the extents of the
+// swapped layout it also moves a volume into that layout. swap 1: a block per
+// 32 x 32 tile of the (z, x) plane of one y, through LDS (block id = tile +
+// tiles * y); swap 2: a block per 256 x-elements of one (z, y) row, whose
+// x-rows are contiguous in both. Block ids start at `first` (launches over
+// ranges of them stay within the dispatch limit); blockIdx.z: batch entry.
 __global__ void __launch_bounds__(256) unswap_volume(const float *__restrict__ tmp, int64_t ts,
                                                       float *__restrict__ out, int64_t vs, int nx, int ny,
-                                                      int nz, int swap, Epilogue ep)
+                                                      int nz, int swap, int64_t first, Epilogue ep)
 {
     const int64_t b = blockIdx.z;
     const float *src = tmp + b * ts;
     float *dst = out + b * vs;
     const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    const int64_t id = first + blockIdx.x;
     if (swap == 1) {
         __shared__ float tile[32][33];
         const int tiles_x = (nx + 31) / 32, tiles = tiles_x * ((nz + 31) / 32);
-        const int y = blockIdx.x / tiles, t = blockIdx.x % tiles;
+        const int y = (int)(id / tiles), t = (int)(id % tiles);
         const int x0 = (t % tiles_x) * 32, z0 = (t / tiles_x) * 32;
         for (int r = ty; r < 32; r += 8) {  // tmp rows x0 + r, z contiguous
             const int x = x0 + r, z = z0 + tx;
@@ -353,8 +356,8 @@ __global__ void __launch_bounds__(256) unswap_volume(const float *__restrict__ t
         }
     } else {
         const int xb = (nx + 255) / 256;
-        const int64_t row = blockIdx.x / xb;  // (z, y) of out
-        const int x = (int)(blockIdx.x % xb) * 256 + ty * 32 + tx;
+        const int64_t row = id / xb;  // (z, y) of out
+        const int x = (int)(id % xb) * 256 + ty * 32 + tx;
         if (x >= nx) return;
         const int z = (int)(row / ny), y = (int)(row % ny);
         emit(dst, row * nx + x, b, src[((int64_t)y * nz + z) * nx + x], ep);

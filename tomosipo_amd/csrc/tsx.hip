@@ -129,10 +129,11 @@ struct tsx_plan {
     int32_t fp_slab_depth = 0;  // slices per slab (nz: no slabs)
     bool bp_separable = false;  // upright detector, x/y stepping, ss == 1
     int32_t bp_chunk = 0;       // separable backprojector: sorted angles per launch
-    // detector v parallel to x (bp_swap 1) or y (2) at every angle: the
-    // separable backprojector of the plan with that axis and z swapped
-    tsx_plan *bp_perm = nullptr;
-    int32_t bp_swap = 0;
+    // detector v parallel to x (swap 1) or y (2) at every angle: the plan of
+    // the same scan with that axis and z swapped, whose detector is upright
+    // (separable backprojector; forward layouts with u along their rows)
+    tsx_plan *perm = nullptr;
+    int32_t swap = 0;
     std::string bp_reason;      // why bp_separable is off
     // kernel variants chosen by tsx_plan_autotune (-1: library default)
     int32_t fp_variant = -1, bp_bz = -1, fp_by = -1;
@@ -644,12 +645,21 @@ bool bp_general_prescaled(const tsx_plan *p, int64_t /*batch*/)
 // Scratch sizes rounded up to 256 bytes.
 int64_t align_ws(int64_t v) { return (v + 255) / 256 * 256; }
 
+// The forward projector of a plan with an axis-swapped plan runs on that
+// plan (the volume moved into its layout first) when that plan has the padded
+// kernels: its detector u axis lies along the layouts' rows, where a wave's
+// rays read consecutive elements.
+bool fp_swapped(const tsx_plan *p) { return p->perm && p->perm->fp_padded && !force_general(); }
+
 int64_t workspace_bytes(const tsx_plan *p, int op, int64_t batch)
 {
     if (force_general()) return 0;
+    // the axis-swapped plan's volume, then that plan's workspace
+    if (op == TSX_OP_FP && fp_swapped(p))
+        return align_ws((int64_t)p->nx * p->ny * p->nz * batch * 4) + workspace_bytes(p->perm, op, batch);
     if (op == TSX_OP_FP) return p->fp_padded ? fp_workspace_elems(p) * batch * 4 : 0;
-    if (p->bp_perm)  // the swapped plan's result, then that plan's workspace
-        return align_ws((int64_t)p->nx * p->ny * p->nz * batch * 4) + workspace_bytes(p->bp_perm, op, batch);
+    if (p->perm)
+        return align_ws((int64_t)p->nx * p->ny * p->nz * batch * 4) + workspace_bytes(p->perm, op, batch);
     return p->bp_separable || bp_general_prescaled(p, batch) ? bp_workspace_elems(p) * batch * 4 : 0;
 }
 
@@ -946,6 +956,36 @@ int launch_fp_padded(const tsx_plan *p, int variant, int by, const float *vol, i
     return TSX_OK;
 }
 
+Epilogue plain(int additive)
+{
+    Epilogue e;
+    e.mode = additive ? EP_ADD : EP_SET;
+    e.y = nullptr;
+    e.w = nullptr;
+    e.stride = 0;
+    return e;
+}
+
+// out (extents nx, ny, nz; entries os floats apart) = src in the layout with
+// axis swap - 1 and z exchanged (entries ss apart), through the epilogue; also
+// the move of a volume into that layout (extents of the swapped layout).
+int launch_unswap(const float *src, int64_t ss, float *out, int64_t os, int nx, int ny, int nz, int swap,
+                  int64_t batch, Epilogue ep, hipStream_t stream)
+{
+    const int64_t blocks = swap == 1 ? (int64_t)((nx + 31) / 32) * ((nz + 31) / 32) * ny
+                                     : (int64_t)((nx + 255) / 256) * ny * nz;
+    const int64_t cap = max_blocks(256);
+    for (int64_t b0 = 0; b0 < batch; b0 += GRID_DIM_MAX) {
+        const int64_t nb = std::min(GRID_DIM_MAX, batch - b0);
+        for (int64_t f = 0; f < blocks; f += cap)
+            hipLaunchKernelGGL(unswap_volume, dim3((unsigned)std::min(cap, blocks - f), 1, (unsigned)nb),
+                               dim3(32, 8), 0, stream, src + b0 * ss, ss, out + b0 * os, os, nx, ny, nz, swap, f,
+                               ep_from(ep, b0));
+    }
+    TSX_HIP_CHECK(hipGetLastError());
+    return TSX_OK;
+}
+
 // Forward projection of `batch` volumes: the padded kernels on the whole
 // volume or slab by slab (each slab's sums added to the projections of the
 // previous ones), else the general kernel.
@@ -954,6 +994,19 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
               int64_t ws_bytes = 0)
 {
     if (batch <= 0) return TSX_OK;
+    if (fp_swapped(p)) {
+        const int64_t ve = (int64_t)p->nx * p->ny * p->nz;
+        Scratch owned;
+        void *wptr = nullptr;
+        int rc = get_workspace(p, workspace_bytes(p, TSX_OP_FP, batch), wsp, ws_bytes, stream, owned, &wptr);
+        if (rc) return rc;
+        float *tmp = (float *)wptr;
+        const tsx_plan *q = p->perm;
+        rc = launch_unswap(vol, vs, tmp, ve, q->nx, q->ny, q->nz, p->swap, batch, plain(0), stream);
+        if (rc) return rc;
+        return launch_fp(q, tmp, ve, proj, ps, batch, ep, stream, (char *)wptr + align_ws(ve * batch * 4),
+                         workspace_bytes(q, TSX_OP_FP, batch));
+    }
     if (!p->fp_padded || force_general() || p->ss > 4)
         return launch_fp_general<float>(p, vol, vs, proj, ps, batch, ep, stream);
     const int variant = fp_variant_of(p), by = fp_tile_rows_of(p);
@@ -1019,22 +1072,13 @@ int launch_bp_general(const tsx_plan *p, T *vol, int64_t vs, const T *proj, int6
     return TSX_OK;
 }
 
-Epilogue plain(int additive)
-{
-    Epilogue e;
-    e.mode = additive ? EP_ADD : EP_SET;
-    e.y = nullptr;
-    e.w = nullptr;
-    e.stride = 0;
-    return e;
-}
 
 int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int64_t ps,
               int64_t batch, Epilogue ep, hipStream_t stream, void *wsp = nullptr,
               int64_t ws_bytes = 0)
 {
     if (batch <= 0) return TSX_OK;
-    if (p->bp_perm && !force_general()) {
+    if (p->perm && !force_general()) {
         // the separable backprojector of the axis-swapped plan, then its
         // result moved into this plan's layout through the epilogue
         const int64_t ve = (int64_t)p->nx * p->ny * p->nz;
@@ -1044,19 +1088,10 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
         if (rc) return rc;
         float *tmp = (float *)wptr;
         char *rest = (char *)wptr + align_ws(ve * batch * 4);
-        rc = launch_bp(p->bp_perm, tmp, ve, proj, ps, batch, plain(0), stream, rest,
-                       workspace_bytes(p->bp_perm, TSX_OP_BP, batch));
+        rc = launch_bp(p->perm, tmp, ve, proj, ps, batch, plain(0), stream, rest,
+                       workspace_bytes(p->perm, TSX_OP_BP, batch));
         if (rc) return rc;
-        const unsigned blocks = p->bp_swap == 1
-            ? (unsigned)(((p->nx + 31) / 32) * ((p->nz + 31) / 32) * (int64_t)p->ny)
-            : (unsigned)(((p->nx + 255) / 256) * (int64_t)p->ny * p->nz);
-        for (int64_t b0 = 0; b0 < batch; b0 += GRID_DIM_MAX) {
-            const int64_t nb = std::min(GRID_DIM_MAX, batch - b0);
-            hipLaunchKernelGGL(unswap_volume, dim3(blocks, 1, (unsigned)nb), dim3(32, 8), 0, stream, tmp + b0 * ve,
-                               ve, vol + b0 * vs, vs, p->nx, p->ny, p->nz, p->bp_swap, ep_from(ep, b0));
-        }
-        TSX_HIP_CHECK(hipGetLastError());
-        return TSX_OK;
+        return launch_unswap(tmp, ve, vol, vs, p->nx, p->ny, p->nz, p->swap, batch, ep, stream);
     }
     const int64_t sino = (int64_t)p->rows * p->n_angles * p->cols;
     if (!p->bp_separable || force_general()) {
@@ -1188,7 +1223,7 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         const int tiles_u = (p->cols + FP_BX - 1) / FP_BX, tiles_v = (p->rows + pby - 1) / pby;
         const int64_t blocks = (int64_t)tiles_u * tiles_v * p->n_angles;
         const int64_t cap = max_blocks(FP_BX * pby);
-        if (!(fast && p->fp_padded && p->ss == 1 && p->slabs.empty()) || blocks > cap) {
+        if (!(fast && p->fp_padded && p->ss == 1 && p->slabs.empty() && !fp_swapped(p)) || blocks > cap) {
             // this instance takes its own launch(es)
             int rc = launch_fp(p, vols[k], 0, projs[k], 0, 1, eps[k], stream, wptr ? (char *)wptr + woff : nullptr, wb);
             if (rc) return rc;
@@ -1465,7 +1500,9 @@ int autotune(tsx_plan *p, hipStream_t stream)
     const Epilogue ep = plain(0);
     float *vol = (float *)t.vol, *proj = (float *)t.proj;
     TuneChoice best{-1, -1, -1};
-    const bool fp_choices = p->fp_padded && p->ss >= 1 && p->ss <= 4 && !force_general();
+    // (a plan whose forward projection runs on its axis-swapped plan leaves the
+    // choice to that plan's own tuning)
+    const bool fp_choices = p->fp_padded && p->ss >= 1 && p->ss <= 4 && !force_general() && !fp_swapped(p);
     if (fp_choices) {
         // The forward kernel walks all angles of one detector tile before the
         // next tile, so its cache reuse depends on the angle count: the
@@ -1790,8 +1827,8 @@ int tsx_plan_create(const tsx_geometry *g, int device, tsx_plan **out)
         tsx_geometry sg = swapped_geometry(g, swap, vec);
         tsx_plan *q = nullptr;
         if (tsx_plan_create(&sg, device, &q) == TSX_OK && q->bp_separable) {
-            p->bp_perm = q;
-            p->bp_swap = swap;
+            p->perm = q;
+            p->swap = swap;
         } else {
             tsx_plan_destroy(q);
         }
@@ -1805,7 +1842,7 @@ int tsx_plan_destroy(tsx_plan *p)
 {
     if (!p) return TSX_OK;
     for (tsx_plan *s : p->slabs) tsx_plan_destroy(s);
-    tsx_plan_destroy(p->bp_perm);
+    tsx_plan_destroy(p->perm);
     if (p->dev) {
         DeviceGuard guard(p->device);
         (void)hipFree(p->dev);
@@ -1827,21 +1864,23 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->vol_shape[1] = p->ny;
     info->vol_shape[2] = p->nx;
     tsx_plan_info sub;
-    const int64_t perm_bytes = p->bp_perm && tsx_plan_info_get(p->bp_perm, &sub) == TSX_OK ? sub.plan_bytes : 0;
+    const int64_t perm_bytes = p->perm && tsx_plan_info_get(p->perm, &sub) == TSX_OK ? sub.plan_bytes : 0;
     info->plan_bytes = perm_bytes + (p->dev ? (int64_t)(sizeof(AngleParams) * 2 * p->host.size()) *
                                         (1 + (int64_t)p->slabs.size()) +
                                     (p->bp_rt ? p->bp_rt_stride * p->n_angles * 4 : 0)
                               : 0);
     for (int c = 0; c < 3; ++c) info->axis_count[c] = p->axis_count[c];
-    info->fp_kernel = p->fp_padded && !force_general() ? 1 : 0;
-    info->bp_kernel = (p->bp_separable || p->bp_perm) && !force_general() ? 1 : 0;
-    info->fp_variant = fp_variant_of(p);
-    info->bp_brick_z = bp_brick_z_of(p->bp_perm ? p->bp_perm : p);
-    info->tuned = p->fp_variant >= 0 || p->bp_bz > 0;
-    info->fp_slabs = info->fp_kernel ? std::max<int32_t>(1, (int32_t)p->slabs.size()) : 0;
-    info->fp_tile_rows = info->fp_kernel ? fp_tile_rows_of(p) : 0;
-    info->bp_launch_angles = info->bp_kernel ? bp_chunk_of(p->bp_perm ? p->bp_perm : p) : 0;
-    info->bp_axis_swap = info->bp_kernel ? p->bp_swap : 0;
+    const tsx_plan *f = fp_swapped(p) ? p->perm : p;  // the plan whose forward kernels run
+    info->fp_kernel = f->fp_padded && !force_general() ? 1 : 0;
+    info->bp_kernel = (p->bp_separable || p->perm) && !force_general() ? 1 : 0;
+    info->fp_variant = fp_variant_of(f);
+    info->bp_brick_z = bp_brick_z_of(p->perm ? p->perm : p);
+    info->tuned = p->fp_variant >= 0 || p->bp_bz > 0 || (p->perm && (p->perm->fp_variant >= 0 || p->perm->bp_bz > 0));
+    info->fp_slabs = info->fp_kernel ? std::max<int32_t>(1, (int32_t)f->slabs.size()) : 0;
+    info->fp_tile_rows = info->fp_kernel ? fp_tile_rows_of(f) : 0;
+    info->fp_axis_swap = fp_swapped(p) ? p->swap : 0;
+    info->bp_launch_angles = info->bp_kernel ? bp_chunk_of(p->perm ? p->perm : p) : 0;
+    info->bp_axis_swap = info->bp_kernel ? p->swap : 0;
     return TSX_OK;
 }
 
@@ -1910,7 +1949,7 @@ int tsx_plan_autotune(tsx_plan *p, void *stream)
     DeviceGuard dg(p->device);
     if (!dg.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
     rc = autotune(p, (hipStream_t)stream);
-    if (rc == TSX_OK && p->bp_perm) rc = autotune(p->bp_perm, (hipStream_t)stream);
+    if (rc == TSX_OK && p->perm) rc = autotune(p->perm, (hipStream_t)stream);
     return rc;
 }
 
