@@ -116,7 +116,8 @@ typedef struct {
     int32_t axis_count[3];     /* number of angles stepping along x, y, z */
     int32_t fp_kernel;         /* 0 = general, 1 = padded per-axis layouts */
     int32_t bp_kernel;         /* 0 = general, 1 = separable (detector v parallel to a volume
-                                  axis: z, or x or y through bp_axis_swap) */
+                                  axis: z, or x or y through bp_axis_swap; any detector
+                                  supersampling through bp_refined_detector) */
     int32_t fp_variant;        /* padded forward projector: 2 = per-lane slice ranges,
                                   3 = wave-synchronous slices, 4 = as 3 on row-paired
                                   layouts (one 16-byte load per sample), 5 = as 3 with
@@ -133,6 +134,10 @@ typedef struct {
     int32_t fp_axis_swap;      /* padded forward projector run on that plan (the volume moved
                                   into its layout first): 0 = none, 1 or 2 as bp_axis_swap; the
                                   fp_* fields above then describe that plan's kernels */
+    int32_t bp_refined_detector; /* detector supersampling ss > 1 with the separable
+                                  backprojector: ss, the backprojection run on the detector
+                                  refined ss times (its sub-rays as pixels); 0 otherwise. The
+                                  bp_* fields above then describe that plan's kernels */
 } tsx_plan_info;
 
 int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);

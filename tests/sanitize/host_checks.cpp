@@ -165,7 +165,7 @@ int main()
     host_plan(geometry(1, tilted, 10, 14, 12), 1, 0);  // tilted detector
     tsx_geometry ss = geometry(0, par, 10, 14, 12);
     ss.det_supersampling = 3;
-    host_plan(ss, -1, 0);                              // supersampled
+    host_plan(ss, -1, 1);                              // supersampled: refined-detector plan
     // rays along z: parallel beam from above
     std::vector<double> zv(12 * 4, 0.0);
     for (int a = 0; a < 4; ++a) {

@@ -56,7 +56,10 @@ def test_kernel_selection():
     assert info["fp_kernel"] == "padded" and info["bp_kernel"] == "separable"
     tilted = ts.rotate(pos=0, axis=(0, 1, 1), angles=0.2) * base.to_vec()
     assert _plan(vg, tilted).info()["bp_kernel"] == "general"
-    assert _plan(vg, base, det_supersampling=2).info()["bp_kernel"] == "general"
+    # detector supersampling: the separable backprojector on the refined detector
+    info = _plan(vg, base, det_supersampling=2).info()
+    assert (info["bp_kernel"], info["bp_refined_detector"]) == ("separable", 2)
+    assert _plan(vg, tilted, det_supersampling=2).info()["bp_kernel"] == "general"
     # a c1 row of the x-stepping layout of 2048^2 slices is 16.8 MB: beyond the
     # 24-bit row offsets of the padded kernel, so the general kernel is chosen
     wide = ts.volume(shape=(8, 2048, 2048), size=(0.004, 1, 1))

@@ -244,10 +244,14 @@ __global__ void scale_by_length(const AngleParams *__restrict__ angles, int n_an
 constexpr int RPAD = 3;
 
 // One (tile, angle) pair of scale_by_length_t: the tile (i0, j0) of sorted
-// angle k of one batch entry (proj and out moved to the entry).
+// angle k of one batch entry (proj and out moved to the entry). up > 1: the
+// plan's detector is a detector of (rows / up) x (cols / up) pixels refined
+// up times, and proj holds that coarse detector's projections; fine pixel
+// (i, j) takes coarse pixel (i / up, j / up) with weight 1 / up^2.
 __device__ __forceinline__ void scale_t_tile(const AngleParams *__restrict__ sorted, int k, int n_angles,
                                              int rows, int cols, const float *__restrict__ proj,
-                                             float *__restrict__ out, int i0, int j0, float (&tile)[32][33])
+                                             float *__restrict__ out, int i0, int j0, float (&tile)[32][33],
+                                             int up = 1)
 {
     const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
     const int rp = rows + 2 * RPAD;
@@ -257,7 +261,12 @@ __device__ __forceinline__ void scale_t_tile(const AngleParams *__restrict__ sor
     for (int r = ty; r < 32; r += 8) {
         const int i = i0 + r, j = j0 + tx;
         float v = 0.0f;
-        if (i < rows && j < cols) v = ray_length(A, i, j) * proj[((int64_t)i * n_angles + a) * cols + j];
+        if (i < rows && j < cols) {
+            const float y = up == 1 ? proj[((int64_t)i * n_angles + a) * cols + j]
+                                    : proj[((int64_t)(i / up) * n_angles + a) * (cols / up) + j / up] /
+                                          (float)(up * up);
+            v = ray_length(A, i, j) * y;
+        }
         tile[r][tx] = v;
     }
     __syncthreads();
@@ -276,7 +285,7 @@ __device__ __forceinline__ void scale_t_tile(const AngleParams *__restrict__ sor
 __global__ void __launch_bounds__(256)
 scale_by_length_t(const AngleParams *__restrict__ sorted, int kn, int n_angles, int rows, int cols,
                   const float *__restrict__ proj, int64_t proj_stride, float *__restrict__ out,
-                  int64_t out_stride, int64_t lines)
+                  int64_t out_stride, int64_t lines, int up)
 {
     __shared__ float tile[32][33];
     const int tiles_i = (rows + 31) / 32;
@@ -284,7 +293,8 @@ scale_by_length_t(const AngleParams *__restrict__ sorted, int kn, int n_angles, 
     for (int64_t line = blockIdx.y; line < lines; line += gridDim.y) {
         const int64_t b = line / kn;
         const int k = (int)(line - b * kn);
-        scale_t_tile(sorted, k, n_angles, rows, cols, proj + b * proj_stride, out + b * out_stride, i0, j0, tile);
+        scale_t_tile(sorted, k, n_angles, rows, cols, proj + b * proj_stride, out + b * out_stride, i0, j0, tile,
+                     up);
     }
 }
 

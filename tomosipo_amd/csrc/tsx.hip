@@ -134,6 +134,12 @@ struct tsx_plan {
     // (separable backprojector; forward layouts with u along their rows)
     tsx_plan *perm = nullptr;
     int32_t swap = 0;
+    // detector supersampling ss > 1: the plan of the same scan on the detector
+    // refined ss times (ss = 1), whose separable backprojector (directly or
+    // through its axis swap) takes this plan's backprojection; bp_only marks
+    // such a plan (its forward projector never runs)
+    tsx_plan *fine = nullptr;
+    bool bp_only = false;
     std::string bp_reason;      // why bp_separable is off
     // kernel variants chosen by tsx_plan_autotune (-1: library default)
     int32_t fp_variant = -1, bp_bz = -1, fp_by = -1;
@@ -658,6 +664,7 @@ int64_t workspace_bytes(const tsx_plan *p, int op, int64_t batch)
     if (op == TSX_OP_FP && fp_swapped(p))
         return align_ws((int64_t)p->nx * p->ny * p->nz * batch * 4) + workspace_bytes(p->perm, op, batch);
     if (op == TSX_OP_FP) return p->fp_padded ? fp_workspace_elems(p) * batch * 4 : 0;
+    if (p->fine) return workspace_bytes(p->fine, op, batch);
     if (p->perm)
         return align_ws((int64_t)p->nx * p->ny * p->nz * batch * 4) + workspace_bytes(p->perm, op, batch);
     return p->bp_separable || bp_general_prescaled(p, batch) ? bp_workspace_elems(p) * batch * 4 : 0;
@@ -736,13 +743,13 @@ int build_range_table(tsx_plan *p, hipStream_t stream, uint32_t **out)
 // The transposed, padded length-weighted projections of the sorted angles
 // k0 .. k0 + kn - 1 of `batch` entries.
 void launch_scale_by_length_t(const tsx_plan *p, int k0, int kn, const float *proj, int64_t ps, float *ly,
-                              int64_t ly_stride, int64_t batch, hipStream_t stream)
+                              int64_t ly_stride, int64_t batch, hipStream_t stream, int up = 1)
 {
     const int64_t lines = (int64_t)kn * batch;
     const unsigned tiles = (unsigned)(((p->rows + 31) / 32) * ((p->cols + 31) / 32));
     dim3 grid(tiles, (unsigned)std::min(lines, GRID_DIM_MAX));
     hipLaunchKernelGGL(scale_by_length_t, grid, dim3(32, 8), 0, stream, p->dev + p->n_angles + k0, kn,
-                       p->n_angles, p->rows, p->cols, proj, ps, ly, ly_stride, lines);
+                       p->n_angles, p->rows, p->cols, proj, ps, ly, ly_stride, lines, up);
 }
 
 // Either the caller's workspace (checked for size) or stream-ordered memory.
@@ -1073,11 +1080,20 @@ int launch_bp_general(const tsx_plan *p, T *vol, int64_t vs, const T *proj, int6
 }
 
 
+// up > 1: p is the refined-detector plan of a plan with detector
+// supersampling up, and proj holds that plan's (coarse) projections.
 int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int64_t ps,
               int64_t batch, Epilogue ep, hipStream_t stream, void *wsp = nullptr,
-              int64_t ws_bytes = 0)
+              int64_t ws_bytes = 0, int up = 1)
 {
     if (batch <= 0) return TSX_OK;
+    if (p->fine && !force_general()) {
+        // detector supersampling ss: the backprojection of the ss x ss
+        // sub-rays of every pixel is that of the detector refined ss times
+        // (the sub-rays are its pixels' rays), each fine pixel taking its
+        // coarse pixel's value over ss^2 (in the transposing pre-pass)
+        return launch_bp(p->fine, vol, vs, proj, ps, batch, ep, stream, wsp, ws_bytes, p->ss);
+    }
     if (p->perm && !force_general()) {
         // the separable backprojector of the axis-swapped plan, then its
         // result moved into this plan's layout through the epilogue
@@ -1089,12 +1105,13 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
         float *tmp = (float *)wptr;
         char *rest = (char *)wptr + align_ws(ve * batch * 4);
         rc = launch_bp(p->perm, tmp, ve, proj, ps, batch, plain(0), stream, rest,
-                       workspace_bytes(p->perm, TSX_OP_BP, batch));
+                       workspace_bytes(p->perm, TSX_OP_BP, batch), up);
         if (rc) return rc;
         return launch_unswap(tmp, ve, vol, vs, p->nx, p->ny, p->nz, p->swap, batch, ep, stream);
     }
     const int64_t sino = (int64_t)p->rows * p->n_angles * p->cols;
     if (!p->bp_separable || force_general()) {
+        if (up != 1) return fail(TSX_ERR_UNSUPPORTED, "refined-detector plan without the separable backprojector");
         if (!force_general() && bp_general_prescaled(p, batch)) {
             Scratch owned;
             void *wptr = nullptr;
@@ -1123,7 +1140,7 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
     for (int k0 = 0; k0 < p->n_angles; k0 += kc) {
         const int kn = std::min(kc, p->n_angles - k0);
         const Epilogue e = k0 == 0 ? ep : later_chunk(ep);
-        launch_scale_by_length_t(p, k0, kn, proj, ps, ly, sino_t, batch, stream);
+        launch_scale_by_length_t(p, k0, kn, proj, ps, ly, sino_t, batch, stream, up);
         for (int64_t b0 = 0; b0 < batch; b0 += bc) {
             const int64_t nbat = std::min(bc, batch - b0);
             BpView v = bp_view(p, BZ, vol + b0 * vs, vs, ly + b0 * sino_t, (uint32_t)(sino_t * 4), (int)nbat,
@@ -1380,7 +1397,8 @@ std::string tune_key(const tsx_plan *p)
            ":" + std::to_string(p->rows) + "x" + std::to_string(p->cols) + ":" + std::to_string(p->nx) +
            "x" + std::to_string(p->ny) + "x" + std::to_string(p->nz) + ":" + std::to_string(p->ss) + ":" +
            std::to_string((int)p->fp_padded) + std::to_string((int)p->fp_pair) +
-           std::to_string((int)p->bp_separable) + ":" + std::to_string(angle_bucket(p->n_angles));
+           std::to_string((int)p->bp_separable) + std::to_string((int)p->bp_only) + ":" +
+           std::to_string(angle_bucket(p->n_angles));
 }
 
 // Mean time of `reps` launches after one warm-up launch, in milliseconds.
@@ -1502,7 +1520,8 @@ int autotune(tsx_plan *p, hipStream_t stream)
     TuneChoice best{-1, -1, -1};
     // (a plan whose forward projection runs on its axis-swapped plan leaves the
     // choice to that plan's own tuning)
-    const bool fp_choices = p->fp_padded && p->ss >= 1 && p->ss <= 4 && !force_general() && !fp_swapped(p);
+    const bool fp_choices =
+        p->fp_padded && p->ss >= 1 && p->ss <= 4 && !force_general() && !fp_swapped(p) && !p->bp_only;
     if (fp_choices) {
         // The forward kernel walks all angles of one detector tile before the
         // next tile, so its cache reuse depends on the angle count: the
@@ -1620,6 +1639,20 @@ int autotune(tsx_plan *p, hipStream_t stream)
     return TSX_OK;
 }
 
+// A plan and the plans that run its kernels (axis-swapped, refined-detector).
+int autotune_tree(tsx_plan *p, hipStream_t stream)
+{
+    int rc = autotune(p, stream);
+    if (rc == TSX_OK && p->perm) rc = autotune_tree(p->perm, stream);
+    if (rc == TSX_OK && p->fine) rc = autotune_tree(p->fine, stream);
+    return rc;
+}
+
+bool tuned_tree(const tsx_plan *p)
+{
+    return p->fp_variant >= 0 || p->bp_bz > 0 || (p->perm && tuned_tree(p->perm)) || (p->fine && tuned_tree(p->fine));
+}
+
 // The z-slabs of a plan whose forward layouts reach chunk_bytes: plans of the
 // same rays over slices [z0, z0 + depth) of the volume (the window's z range
 // cut at the slab's faces), each with its angle table on `device` (none for
@@ -1703,6 +1736,24 @@ tsx_geometry swapped_geometry(const tsx_geometry *g, int swap, std::vector<doubl
     std::swap(sg.window_min[a], sg.window_min[2]);
     std::swap(sg.window_max[a], sg.window_max[2]);
     return sg;
+}
+
+// The geometry on the detector refined ss times: rows and columns times ss,
+// pixel vectors u / ss and v / ss about the same detector centre, no
+// supersampling. Its pixel (i ss + p, j ss + q) is the sub-ray (p, q) of
+// pixel (i, j): centre + (j + (q + 1/2) / ss - cols / 2) u + (same for rows).
+tsx_geometry refined_geometry(const tsx_geometry *g, std::vector<double> &vec)
+{
+    const int ss = g->det_supersampling;
+    tsx_geometry fg = *g;
+    vec.assign(g->vectors, g->vectors + 12 * (int64_t)g->n_angles);
+    for (int64_t a = 0; a < g->n_angles; ++a)
+        for (int c = 6; c < 12; ++c) vec[12 * a + c] /= ss;
+    fg.vectors = vec.data();
+    fg.det_row_count = g->det_row_count * ss;
+    fg.det_col_count = g->det_col_count * ss;
+    fg.det_supersampling = 1;
+    return fg;
 }
 
 // The upright FDK backprojector reads row-paired filtered projections (8 bytes
@@ -1834,6 +1885,21 @@ int tsx_plan_create(const tsx_geometry *g, int device, tsx_plan **out)
         }
         g_last_error.clear();
     }
+    // detector supersampling: the separable backprojector of the same scan
+    // on the refined detector (upright, or v parallel to x or y)
+    if (p->ss > 1 && (int64_t)g->det_col_count * p->ss <= BP_RT_MAX_COLS &&
+        (int64_t)g->det_row_count * p->ss <= ((int64_t)1 << 24)) {
+        std::vector<double> vec;
+        tsx_geometry fg = refined_geometry(g, vec);
+        tsx_plan *q = nullptr;
+        if (tsx_plan_create(&fg, device, &q) == TSX_OK && (q->bp_separable || q->perm)) {
+            q->bp_only = true;
+            p->fine = q;
+        } else {
+            tsx_plan_destroy(q);
+        }
+        g_last_error.clear();
+    }
     *out = p;
     return TSX_OK;
 }
@@ -1843,6 +1909,7 @@ int tsx_plan_destroy(tsx_plan *p)
     if (!p) return TSX_OK;
     for (tsx_plan *s : p->slabs) tsx_plan_destroy(s);
     tsx_plan_destroy(p->perm);
+    tsx_plan_destroy(p->fine);
     if (p->dev) {
         DeviceGuard guard(p->device);
         (void)hipFree(p->dev);
@@ -1864,23 +1931,27 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->vol_shape[1] = p->ny;
     info->vol_shape[2] = p->nx;
     tsx_plan_info sub;
-    const int64_t perm_bytes = p->perm && tsx_plan_info_get(p->perm, &sub) == TSX_OK ? sub.plan_bytes : 0;
-    info->plan_bytes = perm_bytes + (p->dev ? (int64_t)(sizeof(AngleParams) * 2 * p->host.size()) *
+    int64_t sub_bytes = p->perm && tsx_plan_info_get(p->perm, &sub) == TSX_OK ? sub.plan_bytes : 0;
+    if (p->fine && tsx_plan_info_get(p->fine, &sub) == TSX_OK) sub_bytes += sub.plan_bytes;
+    info->plan_bytes = sub_bytes + (p->dev ? (int64_t)(sizeof(AngleParams) * 2 * p->host.size()) *
                                         (1 + (int64_t)p->slabs.size()) +
                                     (p->bp_rt ? p->bp_rt_stride * p->n_angles * 4 : 0)
                               : 0);
     for (int c = 0; c < 3; ++c) info->axis_count[c] = p->axis_count[c];
     const tsx_plan *f = fp_swapped(p) ? p->perm : p;  // the plan whose forward kernels run
     info->fp_kernel = f->fp_padded && !force_general() ? 1 : 0;
-    info->bp_kernel = (p->bp_separable || p->perm) && !force_general() ? 1 : 0;
+    const tsx_plan *b = p->fine && !force_general() ? p->fine : p;  // the plan whose backprojector runs,
+    const tsx_plan *bs = b->perm ? b->perm : b;                      // on its axis-swapped plan if any
+    info->bp_kernel = (b->bp_separable || b->perm) && !force_general() ? 1 : 0;
     info->fp_variant = fp_variant_of(f);
-    info->bp_brick_z = bp_brick_z_of(p->perm ? p->perm : p);
-    info->tuned = p->fp_variant >= 0 || p->bp_bz > 0 || (p->perm && (p->perm->fp_variant >= 0 || p->perm->bp_bz > 0));
+    info->bp_brick_z = bp_brick_z_of(bs);
+    info->tuned = tuned_tree(p);
     info->fp_slabs = info->fp_kernel ? std::max<int32_t>(1, (int32_t)f->slabs.size()) : 0;
     info->fp_tile_rows = info->fp_kernel ? fp_tile_rows_of(f) : 0;
     info->fp_axis_swap = fp_swapped(p) ? p->swap : 0;
-    info->bp_launch_angles = info->bp_kernel ? bp_chunk_of(p->perm ? p->perm : p) : 0;
-    info->bp_axis_swap = info->bp_kernel ? p->swap : 0;
+    info->bp_launch_angles = info->bp_kernel ? bp_chunk_of(bs) : 0;
+    info->bp_axis_swap = info->bp_kernel ? b->swap : 0;
+    info->bp_refined_detector = b != p ? p->ss : 0;
     return TSX_OK;
 }
 
@@ -1948,9 +2019,7 @@ int tsx_plan_autotune(tsx_plan *p, void *stream)
     TraceRange trace("tsx_plan_autotune");
     DeviceGuard dg(p->device);
     if (!dg.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
-    rc = autotune(p, (hipStream_t)stream);
-    if (rc == TSX_OK && p->perm) rc = autotune(p->perm, (hipStream_t)stream);
-    return rc;
+    return autotune_tree(p, (hipStream_t)stream);
 }
 
 int tsx_plan_axes(const tsx_plan *p, int32_t *axes)
