@@ -53,18 +53,24 @@ struct GeneralMath<float> {
     static __device__ __forceinline__ double recip64(double x) { return rcp64(x); }
     static __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
     static __device__ __forceinline__ float root(float x) { return __builtin_amdgcn_sqrtf(x); }
+    // components of U and V (the plan's float copies)
+    static __device__ __forceinline__ float u(const AngleParams &A, int k) { return A.fU[k]; }
+    static __device__ __forceinline__ float v(const AngleParams &A, int k) { return A.fV[k]; }
 };
 template <>
 struct GeneralMath<double> {
     static __device__ __forceinline__ double recip64(double x) { return 1.0 / x; }
     static __device__ __forceinline__ double recip(double x) { return 1.0 / x; }
     static __device__ __forceinline__ double root(double x) { return sqrt(x); }
+    static __device__ __forceinline__ double u(const AngleParams &A, int k) { return A.U[k]; }
+    static __device__ __forceinline__ double v(const AngleParams &A, int k) { return A.V[k]; }
 };
 
 // With PRESCALED (detector supersampling 1 only), proj holds the
 // length-weighted projections l(a, i, j) * y[i, a, j] and the per-pixel ray
-// length is not recomputed.
-template <typename T, bool PRESCALED = false>
+// length is not recomputed. CONE: the plan's beam (every angle of a plan is
+// of the same kind), so that each kernel holds only its own set-up.
+template <typename T, bool PRESCALED, bool CONE>
 __global__ void __launch_bounds__(BP_BX *BP_BY)
 bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int cols, int ss,
            int nx, int ny, int nz, T *__restrict__ vol, int64_t vol_stride,
@@ -85,79 +91,120 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
         // slopes r1, r2 and the homography terms are O(1) and use T
         using M = GeneralMath<T>;
         double cj, ci;
-        T r1, r2, Tc, Bjc = T(0), Bic = T(0), g = T(0);
-        if (A.cone) {
-            const double W0 = km - A.S[0], W1 = k1 - A.S[1], W2 = k2 - A.S[2];
-            const double dn = W0 * A.N[0] + W1 * A.N[1] + W2 * A.N[2];
-            if (dn == 0.0 || W0 == 0.0) continue;
-            const double lam = A.kappa * M::recip64(dn);
-            const double R0 = A.S[0] + lam * W0 - A.P0[0];
-            const double R1 = A.S[1] + lam * W1 - A.P0[1];
-            const double R2 = A.S[2] + lam * W2 - A.P0[2];
-            cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
-            ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
-            const T iW0 = M::recip((T)W0);
-            r1 = (T)W1 * iW0;
-            r2 = (T)W2 * iW0;
-            Tc = (T)dn * (T)M::recip64(A.kappa);  // 1 / lam
-            Bjc = (T)A.U[0] * Tc * iW0;
-            Bic = (T)A.V[0] * Tc * iW0;
-            g = iW0;
-        } else {
-            const double t0 = ((A.P0[0] - km) * A.N[0] + (A.P0[1] - k1) * A.N[1] +
-                               (A.P0[2] - k2) * A.N[2]) * M::recip64(A.EN);
-            const double R0 = km + t0 * A.S[0] - A.P0[0];
-            const double R1 = k1 + t0 * A.S[1] - A.P0[1];
-            const double R2 = k2 + t0 * A.S[2] - A.P0[2];
-            cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
-            ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
-            const double iS0 = M::recip64(A.S[0]);
-            r1 = (T)(A.S[1] * iS0);
-            r2 = (T)(A.S[2] * iS0);
-            Tc = T(1);
-        }
-        const T A1j = Tc * ((T)A.U[1] - r1 * (T)A.U[0]);
-        const T A1i = Tc * ((T)A.V[1] - r1 * (T)A.V[0]);
-        const T A2j = Tc * ((T)A.U[2] - r2 * (T)A.U[0]);
-        const T A2i = Tc * ((T)A.V[2] - r2 * (T)A.V[0]);
-        const T fBj = (T)Bjc, fBi = (T)Bic;
-        // the local homography (offsets of O(1) pixels and voxels) in the
-        // kernel's precision; only the voxel's own detector position (cj, ci)
-        // needs FP64
-        const T det = A1j * A2i - A1i * A2j;
-        if (det == T(0)) continue;
-        const T idet = GeneralMath<T>::recip(det);
-        const T M00 = A2i * idet, M01 = -A1i * idet, M10 = -A2j * idet, M11 = A1j * idet;
-        const T cv0 = fBj * M00 + fBi * M10, cv1 = fBj * M01 + fBi * M11;
-        T djmin = T(1e30), djmax = T(-1e30), dimin = T(1e30), dimax = T(-1e30);
+        T r1 = T(0), r2 = T(0), g = T(0);
+        T A1j, A1i, A2j, A2i, fBj = T(0), fBi = T(0), bj, bi;
         bool unbounded = false;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const T s1 = (s & 1) ? T(1) : T(-1), s2 = (s & 2) ? T(1) : T(-1);
-            const T den = T(1) - (cv0 * s1 + cv1 * s2);
-            if (!(den > T(1e-6))) unbounded = true;
+        if constexpr (!CONE && sizeof(T) == sizeof(float)) {
+            // parallel beam (float): the detector position is affine in the
+            // voxel and the local map the same for every voxel of the angle
+            if (!(A.par_b[0] >= 0.0f)) continue;  // singular map
+            cj = fma(k2, A.par_c[3], fma(k1, A.par_c[2], fma(km, A.par_c[1], A.par_c[0])));
+            ci = fma(k2, A.par_c[7], fma(k1, A.par_c[6], fma(km, A.par_c[5], A.par_c[4])));
+            A1j = A.par_a[0];
+            A1i = A.par_a[1];
+            A2j = A.par_a[2];
+            A2i = A.par_a[3];
+            bj = A.par_b[0];
+            bi = A.par_b[1];
+            if constexpr (!PRESCALED) {
+                r1 = (T)(A.S[1] * A.inv_S0);
+                r2 = (T)(A.S[2] * A.inv_S0);
+            }
+        } else {
+            T Tc, Bjc = T(0), Bic = T(0);
+            if constexpr (CONE) {
+                const double W0 = km - A.S[0], W1 = k1 - A.S[1], W2 = k2 - A.S[2];
+                const double dn = W0 * A.N[0] + W1 * A.N[1] + W2 * A.N[2];
+                if (dn == 0.0 || W0 == 0.0) continue;
+                const double lam = A.kappa * M::recip64(dn);
+                const double R0 = A.S[0] + lam * W0 - A.P0[0];
+                const double R1 = A.S[1] + lam * W1 - A.P0[1];
+                const double R2 = A.S[2] + lam * W2 - A.P0[2];
+                cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
+                ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
+                const T iW0 = M::recip((T)W0);
+                r1 = (T)W1 * iW0;
+                r2 = (T)W2 * iW0;
+                Tc = (T)(dn * A.inv_kappa);  // 1 / lam
+                Bjc = M::u(A, 0) * Tc * iW0;
+                Bic = M::v(A, 0) * Tc * iW0;
+                g = iW0;
+            } else {
+                const double t0 = ((A.P0[0] - km) * A.N[0] + (A.P0[1] - k1) * A.N[1] +
+                                   (A.P0[2] - k2) * A.N[2]) * A.inv_EN;
+                const double R0 = km + t0 * A.S[0] - A.P0[0];
+                const double R1 = k1 + t0 * A.S[1] - A.P0[1];
+                const double R2 = k2 + t0 * A.S[2] - A.P0[2];
+                cj = R0 * A.Bj[0] + R1 * A.Bj[1] + R2 * A.Bj[2];
+                ci = R0 * A.Bi[0] + R1 * A.Bi[1] + R2 * A.Bi[2];
+                r1 = (T)(A.S[1] * A.inv_S0);
+                r2 = (T)(A.S[2] * A.inv_S0);
+                Tc = T(1);
+            }
+            A1j = Tc * (M::u(A, 1) - r1 * M::u(A, 0));
+            A1i = Tc * (M::v(A, 1) - r1 * M::v(A, 0));
+            A2j = Tc * (M::u(A, 2) - r2 * M::u(A, 0));
+            A2i = Tc * (M::v(A, 2) - r2 * M::v(A, 0));
+            fBj = Bjc;
+            fBi = Bic;
+            // the local homography (offsets of O(1) pixels and voxels) in the
+            // kernel's precision; only the voxel's own detector position
+            // (cj, ci) needs FP64
+            const T det = A1j * A2i - A1i * A2j;
+            if (det == T(0)) continue;
+            const T idet = GeneralMath<T>::recip(det);
+            const T M00 = A2i * idet, M01 = -A1i * idet, M10 = -A2j * idet, M11 = A1j * idet;
+            const T cv0 = fBj * M00 + fBi * M10, cv1 = fBj * M01 + fBi * M11;
+            // The corners (s1, s2) = (+-1, +-1) of the square map to the
+            // detector offsets (dj, di) = (M00 s1 + M01 s2, M10 s1 + M11 s2) /
+            // (1 - cv0 s1 - cv1 s2); the image of the square is their convex
+            // hull (the map is projective, with a denominator of one sign over
+            // the square), so |dj| <= (|M00| + |M01|) / (1 - |cv0| - |cv1|) and
+            // likewise |di| bound it: a box at most O(|cv|) wider than the
+            // corners' own (cv is of the order of a voxel over the source
+            // distance), from one reciprocal instead of four.
+            const T den = T(1) - (fabs(cv0) + fabs(cv1));
+            unbounded = !(den > T(1e-6));
             const T iden = GeneralMath<T>::recip(den);
-            const T dj = (M00 * s1 + M01 * s2) * iden, di = (M10 * s1 + M11 * s2) * iden;
-            djmin = fmin(djmin, dj);
-            djmax = fmax(djmax, dj);
-            dimin = fmin(dimin, di);
-            dimax = fmax(dimax, di);
+            bj = (fabs(M00) + fabs(M01)) * iden;
+            bi = (fabs(M10) + fabs(M11)) * iden;
         }
         int j0, j1, i0, i1;
         if (unbounded) {
             j0 = 0; j1 = cols - 1; i0 = 0; i1 = rows - 1;
         } else {
-            const double mj = sub_margin + 1e-3, mi = sub_margin + 1e-3;
-            j0 = (int)fmax(0.0, ceil(cj + (double)djmin - mj));
-            j1 = (int)fmin((double)(cols - 1), floor(cj + (double)djmax + mj));
-            i0 = (int)fmax(0.0, ceil(ci + (double)dimin - mi));
-            i1 = (int)fmin((double)(rows - 1), floor(ci + (double)dimax + mi));
+            const double mj = sub_margin + 1e-3 + (double)bj, mi = sub_margin + 1e-3 + (double)bi;
+            j0 = (int)fmax(0.0, ceil(cj - mj));
+            j1 = (int)fmin((double)(cols - 1), floor(cj + mj));
+            i0 = (int)fmax(0.0, ceil(ci - mi));
+            i1 = (int)fmin((double)(rows - 1), floor(ci + mi));
         }
         if (j0 > j1 || i0 > i1) continue;
+        const T dj0 = (T)((double)j0 - cj), di0 = (T)((double)i0 - ci);
+        if constexpr (PRESCALED) {
+            // detector supersampling 1 (the only case with length-weighted
+            // projections): one ray per pixel, through its centre
+            T acc_a = T(0);
+            for (int i = i0; i <= i1; ++i) {
+                const T *row = pr + ((int64_t)i * n_angles + a) * cols;
+                const T di = di0 + (T)(i - i0);
+                const T e = di * fBi + T(1), b1 = di * A1i, b2 = di * A2i;
+                for (int j = j0; j <= j1; ++j) {
+                    const T dj = dj0 + (T)(j - j0);
+                    const T iden = GeneralMath<T>::recip(dj * fBj + e);
+                    const T dq1 = (dj * A1j + b1) * iden;
+                    const T dq2 = (dj * A2j + b2) * iden;
+                    const T w1 = T(1) - fabs(dq1), w2 = T(1) - fabs(dq2);
+                    const T w = (w1 > T(0) ? w1 : T(0)) * (w2 > T(0) ? w2 : T(0));
+                    acc_a += w * row[j];
+                }
+            }
+            acc += acc_a;
+            continue;
+        }
         const T fr1 = r1, fr2 = r2, fg = g;
         const T h0 = (T)(A.h[0] * A.h[0]), h1 = (T)(A.h[1] * A.h[1]), h2 = (T)(A.h[2] * A.h[2]);
         const T len_par = GeneralMath<T>::root(h0 + h1 * fr1 * fr1 + h2 * fr2 * fr2);
-        const T dj0 = (T)((double)j0 - cj), di0 = (T)((double)i0 - ci);
         const T tinv = (T)1 / (T)ss;
         T acc_a = T(0);
         for (int i = i0; i <= i1; ++i) {
@@ -166,25 +213,21 @@ bp_general(const AngleParams *__restrict__ angles, int n_angles, int rows, int c
                 const T yv = row[j];
                 T wsum = T(0);
                 for (int p = 0; p < ss; ++p) {
-                    const T di = di0 + (T)(i - i0) + ((T)(p + 0.5) * tinv - (T)0.5);
+                    const T di = di0 + (T)(i - i0) + (((T)p + T(0.5)) * tinv - T(0.5));
                     for (int q = 0; q < ss; ++q) {
-                        const T dj = dj0 + (T)(j - j0) + ((T)(q + 0.5) * tinv - (T)0.5);
+                        const T dj = dj0 + (T)(j - j0) + (((T)q + T(0.5)) * tinv - T(0.5));
                         const T den = dj * fBj + (di * fBi + T(1));
                         const T iden = GeneralMath<T>::recip(den);
                         const T dq1 = (dj * A1j + di * A1i) * iden;
                         const T dq2 = (dj * A2j + di * A2i) * iden;
                         const T w1 = T(1) - fabs(dq1), w2 = T(1) - fabs(dq2);
                         const T w = (w1 > T(0) ? w1 : T(0)) * (w2 > T(0) ? w2 : T(0));
-                        if constexpr (PRESCALED) {
-                            wsum += w;
-                        } else {
-                            T len = len_par;
-                            if (A.cone) {
-                                const T s1 = dq1 * fg + fr1, s2 = dq2 * fg + fr2;
-                                len = GeneralMath<T>::root(h0 + h1 * s1 * s1 + h2 * s2 * s2);
-                            }
-                            wsum += w * len;
+                        T len = len_par;
+                        if constexpr (CONE) {
+                            const T s1 = dq1 * fg + fr1, s2 = dq2 * fg + fr2;
+                            len = GeneralMath<T>::root(h0 + h1 * s1 * s1 + h2 * s2 * s2);
                         }
+                        wsum += w * len;
                     }
                 }
                 acc_a += wsum * yv;

@@ -171,6 +171,39 @@ V3 cross(V3 a, V3 b)
 }
 bool finite3(V3 a) { return std::isfinite(a[0]) && std::isfinite(a[1]) && std::isfinite(a[2]); }
 
+// The per-angle terms of a parallel-beam angle that the general
+// backprojector would otherwise form per voxel: the detector position as an
+// affine function of the grid point (t0 = (P0 - k) . N / EN, R = k + t0 S -
+// P0, j = R . Bj, i = R . Bi), the local map from detector offsets to
+// in-slice offsets (rays with slopes r = S / S_m) and the bound of the
+// image of a voxel's +-1 square on the detector.
+void parallel_map(AngleParams &A)
+{
+    for (int k = 0; k < 8; ++k) A.par_c[k] = 0.0;
+    for (int k = 0; k < 4; ++k) A.par_a[k] = 0.0f;
+    A.par_b[0] = A.par_b[1] = -1.0f;
+    if (A.cone || A.EN == 0.0 || A.S[0] == 0.0) return;
+    const double *B[2] = {A.Bj, A.Bi};
+    const double PN = A.P0[0] * A.N[0] + A.P0[1] * A.N[1] + A.P0[2] * A.N[2];
+    for (int r = 0; r < 2; ++r) {
+        const double SB = A.S[0] * B[r][0] + A.S[1] * B[r][1] + A.S[2] * B[r][2];
+        const double PB = A.P0[0] * B[r][0] + A.P0[1] * B[r][1] + A.P0[2] * B[r][2];
+        A.par_c[4 * r] = PN * SB * A.inv_EN - PB;
+        for (int k = 0; k < 3; ++k) A.par_c[4 * r + 1 + k] = B[r][k] - A.N[k] * SB * A.inv_EN;
+    }
+    const double r1 = A.S[1] / A.S[0], r2 = A.S[2] / A.S[0];
+    const double a1j = A.U[1] - r1 * A.U[0], a1i = A.V[1] - r1 * A.V[0];
+    const double a2j = A.U[2] - r2 * A.U[0], a2i = A.V[2] - r2 * A.V[0];
+    A.par_a[0] = (float)a1j;
+    A.par_a[1] = (float)a1i;
+    A.par_a[2] = (float)a2j;
+    A.par_a[3] = (float)a2i;
+    const double det = a1j * a2i - a1i * a2j;
+    if (det == 0.0 || (float)a1j * (float)a2i - (float)a1i * (float)a2j == 0.0f) return;
+    A.par_b[0] = (float)((std::fabs(a2i) + std::fabs(a1i)) / std::fabs(det));
+    A.par_b[1] = (float)((std::fabs(a2j) + std::fabs(a1j)) / std::fabs(det));
+}
+
 // Dominant axis: argmax |e_c|, ties resolved in the order x, y, z.
 int dominant_axis(V3 e)
 {
@@ -361,6 +394,14 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
         }
         A.kappa = cone ? dot(P0 - S, N) : 0.0;
         A.EN = cone ? 0.0 : dot(S, N);
+        A.inv_kappa = A.kappa != 0.0 ? 1.0 / A.kappa : 0.0;
+        A.inv_EN = A.EN != 0.0 ? 1.0 / A.EN : 0.0;
+        A.inv_S0 = !cone && A.S[0] != 0.0 ? 1.0 / A.S[0] : 0.0;
+        for (int k = 0; k < 3; ++k) {
+            A.fU[k] = (float)A.U[k];
+            A.fV[k] = (float)A.V[k];
+        }
+        parallel_map(A);
         A.m = m;
         A.c1 = c1;
         A.c2 = c2;
@@ -1071,7 +1112,8 @@ int launch_bp_general(const tsx_plan *p, T *vol, int64_t vs, const T *proj, int6
         const int64_t nb = std::min(bc, batch - b0);
         dim3 grid((unsigned)((p->nx + BP_BX - 1) / BP_BX), (unsigned)((p->ny + BP_BY - 1) / BP_BY),
                   (unsigned)(p->nz * nb));
-        hipLaunchKernelGGL((bp_general<T, PRESCALED>), grid, dim3(BP_BX, BP_BY), 0, stream, p->dev,
+        auto kernel = p->kind == TSX_CONE_VEC ? bp_general<T, PRESCALED, true> : bp_general<T, PRESCALED, false>;
+        hipLaunchKernelGGL(kernel, grid, dim3(BP_BX, BP_BY), 0, stream, p->dev,
                            p->n_angles, p->rows, p->cols, p->ss, p->nx, p->ny, p->nz, vol + b0 * vs, vs,
                            proj + b0 * ps, ps, ep_from(ep, b0));
     }

@@ -25,6 +25,18 @@ struct AngleParams {
     int32_t pad;
     int64_t st[3];        // element strides along (m, c1, c2) in the [z][y][x] volume
     int64_t pst[3];       // element strides along (m, c1, c2) in the padded layout of axis m
+    // per-angle constants of the general backprojector
+    double inv_kappa;     // cone: 1 / kappa
+    double inv_EN;        // parallel: 1 / EN
+    double inv_S0;        // parallel: 1 / S[0] (the ray's component along m)
+    float fU[3], fV[3];   // U and V rounded to float
+    // parallel beam: detector position of grid point k = (k_m, k_c1, k_c2),
+    // j = par_c[0] + k . par_c[1..3], i = par_c[4] + k . par_c[5..7]; the
+    // local map's terms (A1j, A1i, A2j, A2i) and the half-widths (in
+    // pixels) of the image of a voxel's +-1 square, par_b[0] < 0 if the map
+    // is singular
+    double par_c[8];
+    float par_a[4], par_b[2];
 };
 
 // Per-angle FDK quantities of a cone-beam plan, world (x, y, z) coordinates.
