@@ -134,6 +134,12 @@ typedef struct {
     int32_t fp_axis_swap;      /* padded forward projector run on that plan (the volume moved
                                   into its layout first): 0 = none, 1 or 2 as bp_axis_swap; the
                                   fp_* fields above then describe that plan's kernels */
+    int32_t fp_shear_rows;     /* padded forward projector on a detector whose rows are not
+                                  parallel to the slices' rows: each wave's rays follow the
+                                  detector direction that keeps them on one row of the slices
+                                  (sheared tiles, rows of a tile column taken cyclically), a
+                                  lane's row offset at most this many rows; 0 for tiles along
+                                  the detector rows */
     int32_t bp_refined_detector; /* detector supersampling ss > 1 with the separable
                                   backprojector: ss, the backprojection run on the detector
                                   refined ss times (its sub-rays as pixels); 0 otherwise. The
@@ -149,15 +155,19 @@ int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);
  * on tiles of 8 detector rows, then tiles of 4 rows for the chosen march
  * where it takes them, kept if more than 3 % faster) on every angle over bands of 64 and 32
  * detector rows, compared on the predicted time of the whole plan (fixed
- * layout-building cost plus cost per row); the backprojector's brick depth
+ * layout-building cost plus cost per row); for a detector whose rows cross
+ * the slices' rows, sheared against plain tiles on the whole plan (sheared
+ * kept unless plain ones are more than 2 % faster); the backprojector's brick depth
  * (32, 24 or 16 slices) on at most 64 of its angles, in runs of consecutive
  * angles from four quadrants of the scan. Trials use zero-filled scratch
  * buffers allocated and freed on `stream`. Choices are cached per (type,
  * detector, volume, angle count rounded up to a power of two) shape, so a
  * second plan of the same shape takes them without timing. Synchronises
  * `stream`. Must not run while launches of the same plan are being issued
- * from another thread. The TSX_FP_VARIANT, TSX_FP_TILE_ROWS and
- * TSX_BP_BRICK_Z environment variables, when set, override any choice. */
+ * from another thread. The TSX_FP_VARIANT, TSX_FP_TILE_ROWS, TSX_FP_SHEAR
+ * (0 or 1) and TSX_BP_BRICK_Z environment variables, when set, override any
+ * choice. Plans held by the plan (axis-swapped, refined detector) are tuned
+ * too. */
 int tsx_plan_autotune(tsx_plan *plan, void *stream);
 
 /* Replaces astra.experimental.accumulate_FDK(projector, vol, proj)

@@ -113,6 +113,22 @@ def test_swapped_axes_separable_backprojector():
     assert (info["bp_kernel"], info["fp_kernel"], info["fp_axis_swap"]) == ("general", "padded", 0)
 
 
+def test_sheared_forward_tiles():
+    """Forward tiles follow the detector direction that keeps a wave's rays
+    on one row of the slices: none for an upright detector (or one moved to
+    the upright plan by the axis swap), rows added at the tile columns' ends
+    for a detector rolled about its normal."""
+    vg = ts.volume(shape=(40, 48, 56), size=(40 / 48, 1, 56 / 48))
+    pg = ts.cone(angles=30, shape=(48, 72), size=(np.sqrt(2), 1.5 * np.sqrt(2)), cone_angle=0.5).to_vec()
+    assert _plan(vg, pg).info()["fp_shear_rows"] == 0
+    turned = ts.rotate(pos=0, axis=(0, 1, 0), angles=np.pi / 2) * pg
+    assert _plan(vg, turned).info()["fp_shear_rows"] == 0
+    rolled = ts.rotate(pos=0, axis=(0, 1, 1), angles=0.2) * pg
+    e = _plan(vg, rolled).info()["fp_shear_rows"]
+    # |slope| about tan(0.2)-ish over a wave of 64: a few rows each side
+    assert 2 <= e <= 16, e
+
+
 def test_chunk_limit_override(monkeypatch):
     """TSX_CHUNK_BYTES lowers the bound at plan creation: a 48^3 cone plan
     then takes 3 slabs of 16 (row-paired x layout of a 16-slice slab:

@@ -101,6 +101,7 @@ class TsxPlanInfo(ctypes.Structure):
         ("fp_tile_rows", ctypes.c_int32),
         ("bp_axis_swap", ctypes.c_int32),
         ("fp_axis_swap", ctypes.c_int32),
+        ("fp_shear_rows", ctypes.c_int32),
         ("bp_refined_detector", ctypes.c_int32),
     ]
 
@@ -229,6 +230,7 @@ class Plan:
             "tuned": bool(inf.tuned),
             "fp_slabs": inf.fp_slabs, "bp_launch_angles": inf.bp_launch_angles,
             "fp_tile_rows": inf.fp_tile_rows, "bp_axis_swap": inf.bp_axis_swap, "fp_axis_swap": inf.fp_axis_swap,
+            "fp_shear_rows": inf.fp_shear_rows,
             "bp_refined_detector": inf.bp_refined_detector,
         }
 

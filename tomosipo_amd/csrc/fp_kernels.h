@@ -767,6 +767,7 @@ __device__ __forceinline__ float march_lds(__amdgpu_buffer_rsrc_t rs, const RayS
 struct FpView {
     const AngleParams *angles;
     int n_angles, rows, cols, tiles_u, tiles_v, batch;
+    int shear_rows;  // sheared tiles: bound on a lane's row offset (< tiles_v * BY; 0: no shear)
     int a0, na;  // angles a0 .. a0 + na - 1 of the n_angles are projected by this launch
     Layouts L;
     float *proj;
@@ -786,11 +787,23 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
     const int tile = rest % tiles;
     const int b = rest / tiles;
     const int j = (tile % v.tiles_u) * FP_BX + threadIdx.x;
-    const int i = (tile / v.tiles_u) * BY + threadIdx.y;
+    int i = (tile / v.tiles_u) * BY + threadIdx.y;
     if (b >= v.batch) return;  // uniform over the block
+    const AngleParams &A = v.angles[a];
+    if (v.shear_rows) {
+        // sheared tiles: the wave's rays follow the detector direction that
+        // keeps them on one row of the slices. For a fixed column the offset
+        // s is fixed, and rows are taken modulo the tiled height R = tiles_v
+        // * BY (|s| <= E < R), so the tiles of the column still cover each
+        // of its rows exactly once; the waves at the column's ends hold rows
+        // from both ends.
+        const int E = v.shear_rows, R = v.tiles_v * BY;
+        const int s = (int)floorf(A.shear * (float)((int)threadIdx.x - FP_BX / 2) + 0.5f);
+        i += min(max(s, -E), E);
+        i += i < 0 ? R : (i >= R ? -R : 0);
+    }
     const bool inside = j < v.cols && i < v.rows;
     if (PIPE < 3 && !inside) return;
-    const AngleParams &A = v.angles[a];
     const float *lp = uniform_ptr(v.L.ptr[A.m] + (int64_t)b * v.L.stride);
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void *)lp, (short)0, (int)v.L.bytes[A.m], 0x00020000);

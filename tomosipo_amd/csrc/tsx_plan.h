@@ -37,6 +37,10 @@ struct AngleParams {
     // is singular
     double par_c[8];
     float par_a[4], par_b[2];
+    // forward tiles: slope di/dj of the detector direction along which rays
+    // meet one c1 row of the slices (0: along the detector rows); a wave's
+    // lane tx takes row offset round(shear (tx - 32)) within its tile
+    float shear;
 };
 
 // Per-angle FDK quantities of a cone-beam plan, world (x, y, z) coordinates.
