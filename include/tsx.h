@@ -140,6 +140,10 @@ typedef struct {
                                   (sheared tiles, rows of a tile column taken cyclically), a
                                   lane's row offset at most this many rows; 0 for tiles along
                                   the detector rows */
+    int32_t fp_refined_detector; /* detector supersampling ss > 1 with the padded forward
+                                  projector on the detector refined ss times: ss (each pixel
+                                  the mean of its ss x ss fine pixels); 0 otherwise. The fp_*
+                                  fields above then describe that plan's kernels */
     int32_t bp_refined_detector; /* detector supersampling ss > 1 with the separable
                                   backprojector: ss, the backprojection run on the detector
                                   refined ss times (its sub-rays as pixels); 0 otherwise. The

@@ -129,6 +129,27 @@ def test_sheared_forward_tiles():
     assert 2 <= e <= 16, e
 
 
+def test_refined_detector_workspace():
+    """Detector supersampling ss runs both projectors on the plan of the
+    detector refined ss times: the forward workspace holds the refined
+    projections (rounded up to 256 bytes) and that plan's own workspace, the
+    backward workspace is that plan's (a regression check: the forward
+    requirement was once the coarse plan's layouts, smaller than what the
+    refined path writes)."""
+    vg = ts.volume(shape=(20, 24, 28), size=(20 / 24, 1, 28 / 24))
+    for ss in (2, 3):
+        pg = ts.cone(angles=16, shape=(12, 18), size=(1.4, 2.1), cone_angle=0.5).to_vec()
+        fine = ts.cone(angles=16, shape=(12 * ss, 18 * ss), size=(1.4, 2.1), cone_angle=0.5).to_vec()
+        p, q = _plan(vg, pg, det_supersampling=ss), _plan(vg, fine)
+        info = p.info()
+        assert (info["fp_refined_detector"], info["bp_refined_detector"]) == (ss, ss)
+        for batch in (1, 3):
+            sino = 12 * ss * 16 * 18 * ss * 4 * batch
+            need = (sino + 255) // 256 * 256 + q.workspace_size(_tsx.TSX_OP_FP, batch)
+            assert p.workspace_size(_tsx.TSX_OP_FP, batch) == need
+            assert p.workspace_size(_tsx.TSX_OP_BP, batch) == q.workspace_size(_tsx.TSX_OP_BP, batch)
+
+
 def test_chunk_limit_override(monkeypatch):
     """TSX_CHUNK_BYTES lowers the bound at plan creation: a 48^3 cone plan
     then takes 3 slabs of 16 (row-paired x layout of a 16-slice slab:

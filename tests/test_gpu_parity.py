@@ -205,7 +205,7 @@ def test_random_geometries(seed):
     _check_fp_bp(vg, pg, x)
 
 
-@pytest.mark.parametrize("ss", [2, 3])
+@pytest.mark.parametrize("ss", [2, 3, 6])
 def test_detector_supersampling(ss):
     vg = ts.volume(shape=24, size=1)
     pg = ts.cone(angles=12, shape=(16, 20), size=(1.4, 2.0), cone_angle=0.5)
@@ -213,13 +213,14 @@ def test_detector_supersampling(ss):
     _check_fp_bp(vg, pg, x, ss=ss)
 
 
-@pytest.mark.parametrize("case", ["cone2", "parallel3", "turned2"])
-def test_refined_detector_backprojector(monkeypatch, case):
-    """Detector supersampling ss on a detector the separable backprojector
-    takes: the backprojection runs on the plan of the detector refined ss
-    times (each fine pixel takes its coarse pixel's value over ss^2). Parity
-    with the oracle and the matched adjoint (1e-5), agreement with the
-    general backprojector of the same plan (1e-6), ADD and scaled-add
+@pytest.mark.parametrize("case", ["cone2", "parallel3", "turned2", "cone5"])
+def test_refined_detector_projectors(monkeypatch, case):
+    """Detector supersampling ss: both projectors run on the plan of the
+    detector refined ss times -- the forward projection as the mean of each
+    pixel's ss x ss fine pixels, the backprojection with each fine pixel
+    taking its coarse pixel's value over ss^2. Parity with the oracle and the
+    matched adjoint (1e-5), agreement with the general kernels of the same
+    plan (forward 1e-5, backward 1e-6), ADD, residual and scaled-add
     epilogues, a batch of two, a fused launch holding the instance, and the
     refined plan's angle ranges under a lowered chunk bound."""
     import os
@@ -238,21 +239,40 @@ def test_refined_detector_backprojector(monkeypatch, case):
     info = plan.info()
     print(f"[refined] {case}: {info}", flush=True)
     assert (info["bp_kernel"], info["bp_refined_detector"]) == ("separable", ss)
-    assert info["bp_axis_swap"] == (1 if case.startswith("turned") else 0)
+    assert (info["fp_kernel"], info["fp_refined_detector"]) == ("padded", ss)
+    assert info["bp_axis_swap"] == info["fp_axis_swap"] == (1 if case.startswith("turned") else 0)
     x = ts.phantom.ellipsoids(28, count=5, seed=ss)[:20, :24, :28].copy()
     _check_fp_bp(vg, pg, x, ss=ss)
     g = torch.Generator(device=DEV)
     g.manual_seed(31)
     y = torch.rand(A.range_shape, generator=g, device=DEV)
+    xd = torch.from_numpy(x).to(DEV)
+    fx = A(xd)
     b = A.T(y)
     os.environ["TSX_FORCE_GENERAL"] = "1"
     try:
         assert plan.info()["bp_kernel"] == "general"
+        fx_gen = A(xd)
         b_gen = A.T(y)
     finally:
         del os.environ["TSX_FORCE_GENERAL"]
+    assert (torch.linalg.norm(fx - fx_gen) / torch.linalg.norm(fx_gen)).item() <= 1e-5
     assert (torch.linalg.norm(b - b_gen) / torch.linalg.norm(b_gen)).item() <= 1e-6
     s = torch.cuda.current_stream().cuda_stream
+    accp = torch.rand(A.range_shape, generator=g, device=DEV)
+    op_ = accp.clone()
+    plan.forward(xd.data_ptr(), op_.data_ptr(), True, s)
+    assert (torch.linalg.norm(op_ - accp - fx) / torch.linalg.norm(fx)).item() <= 1e-6
+    w = torch.rand(A.range_shape, generator=g, device=DEV)
+    r = torch.empty(A.range_shape, device=DEV)
+    plan.forward_residual(xd.data_ptr(), y.data_ptr(), w.data_ptr(), r.data_ptr(), s)
+    assert (torch.linalg.norm(r - w * (y - fx)) / torch.linalg.norm(w * (y - fx))).item() <= 1e-6
+    xs = torch.rand((2,) + tuple(A.domain_shape), generator=g, device=DEV)
+    fs = torch.empty((2,) + tuple(A.range_shape), device=DEV)
+    plan.forward(xs.data_ptr(), fs.data_ptr(), False, s, batch=2, vol_stride=xs[0].numel(),
+                 proj_stride=fs[0].numel())
+    for k in range(2):
+        assert torch.equal(fs[k], A(xs[k]))
     acc = torch.rand(A.domain_shape, generator=g, device=DEV)
     o = acc.clone()
     plan.backward(o.data_ptr(), y.data_ptr(), True, s)
@@ -274,6 +294,11 @@ def test_refined_detector_backprojector(monkeypatch, case):
     outs = fused_backward(ops, ins)
     for op, yi, bi in zip(ops, ins, outs):
         assert torch.equal(bi, op.T(yi))
+    from tomosipo_amd.fused import fused_forward
+    vins = [torch.rand(op.domain_shape, generator=g, device=DEV) for op in ops]
+    fouts = fused_forward(ops, vins)
+    for op, xi, yi in zip(ops, vins, fouts):
+        assert torch.equal(yi, op(xi))
     # the refined plan's transposed projections in ranges of 5 angles
     per_angle = (12 * ss + 6) * 18 * ss * 4 if not case.startswith("parallel") else (10 * ss + 6) * 18 * ss * 4
     monkeypatch.setenv("TSX_CHUNK_BYTES", str(5 * per_angle + 1))

@@ -837,6 +837,33 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
          v.ep);
 }
 
+// Projections of a plan with detector supersampling ss from those of the
+// same scan on the detector refined ss times (fine: [rows ss][n_angles][cols
+// ss] per batch entry, entries fs floats apart): pixel (i, a, j) is the mean
+// of fine pixels (i ss + p, a, j ss + q), p, q < ss, written through the
+// epilogue. A block per 256 columns of one (row, angle) line, block ids from
+// `first` (launches over ranges of them stay within the dispatch limit);
+// blockIdx.z: batch entry.
+__global__ void __launch_bounds__(256) reduce_refined(const float *__restrict__ fine, int64_t fs,
+                                                       float *__restrict__ out, int64_t os, int rows,
+                                                       int n_angles, int cols, int ss, int64_t first,
+                                                       Epilogue ep)
+{
+    const int64_t b = blockIdx.z;
+    const int xb = (cols + 255) / 256;
+    const int64_t id = first + blockIdx.x;
+    const int64_t line = id / xb;  // i * n_angles + a
+    const int j = (int)(id % xb) * 256 + (int)threadIdx.x;
+    if (j >= cols) return;
+    const int i = (int)(line / n_angles), a = (int)(line % n_angles);
+    const int64_t fcols = (int64_t)cols * ss;
+    const float *src = fine + b * fs + ((int64_t)i * ss * n_angles + a) * fcols + (int64_t)j * ss;
+    float acc = 0.0f;
+    for (int p = 0; p < ss; ++p)
+        for (int q = 0; q < ss; ++q) acc += src[(int64_t)p * n_angles * fcols + q];
+    emit(out + b * os, line * cols + j, b, acc * (1.0f / (float)(ss * ss)), ep);
+}
+
 // XCD-aware order: consecutive hardware block ids go round-robin to the 8
 // XCDs. Logical ids are dealt to the XCDs in chunks of 2^shift consecutive
 // ids, chunk k to XCD k % 8, and each XCD runs its chunks in order. The

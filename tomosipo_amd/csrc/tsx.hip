@@ -135,11 +135,11 @@ struct tsx_plan {
     tsx_plan *perm = nullptr;
     int32_t swap = 0;
     // detector supersampling ss > 1: the plan of the same scan on the detector
-    // refined ss times (ss = 1), whose separable backprojector (directly or
-    // through its axis swap) takes this plan's backprojection; bp_only marks
-    // such a plan (its forward projector never runs)
+    // refined ss times (ss = 1); its separable backprojector (directly or
+    // through its axis swap) takes this plan's backprojection, its padded
+    // forward projector this plan's forward projection (bp_refined,
+    // fp_refined)
     tsx_plan *fine = nullptr;
-    bool bp_only = false;
     // forward tiles sheared along each angle's shear: the largest row offset
     // of a lane, ceil(32 max |shear|) (0: unsheared tiles; also 0 where it
     // would reach half the detector's rows)
@@ -742,14 +742,36 @@ int64_t align_ws(int64_t v) { return (v + 255) / 256 * 256; }
 // rays read consecutive elements.
 bool fp_swapped(const tsx_plan *p) { return p->perm && p->perm->fp_padded && !force_general(); }
 
+// Projections of a plan with detector supersampling run on its
+// refined-detector plan where that plan has the fast kernels: the
+// backprojection where it has the separable one, the forward projection
+// where it has the padded one (directly or through its axis swap).
+bool bp_refined(const tsx_plan *p)
+{
+    return p->fine && (p->fine->bp_separable || p->fine->perm) && !force_general();
+}
+bool fp_refined(const tsx_plan *p)
+{
+    return p->fine && (p->fine->fp_padded || fp_swapped(p->fine)) && !force_general();
+}
+
+// Elements of one batch entry's projections on the refined detector.
+int64_t refined_sino(const tsx_plan *p)
+{
+    return (int64_t)p->fine->rows * p->fine->n_angles * p->fine->cols;
+}
+
 int64_t workspace_bytes(const tsx_plan *p, int op, int64_t batch)
 {
     if (force_general()) return 0;
     // the axis-swapped plan's volume, then that plan's workspace
     if (op == TSX_OP_FP && fp_swapped(p))
         return align_ws((int64_t)p->nx * p->ny * p->nz * batch * 4) + workspace_bytes(p->perm, op, batch);
+    // the refined detector's projections, then that plan's workspace
+    if (op == TSX_OP_FP && fp_refined(p))
+        return align_ws(refined_sino(p) * batch * 4) + workspace_bytes(p->fine, op, batch);
     if (op == TSX_OP_FP) return p->fp_padded ? fp_workspace_elems(p) * batch * 4 : 0;
-    if (p->fine) return workspace_bytes(p->fine, op, batch);
+    if (op == TSX_OP_BP && bp_refined(p)) return workspace_bytes(p->fine, op, batch);
     if (p->perm)
         return align_ws((int64_t)p->nx * p->ny * p->nz * batch * 4) + workspace_bytes(p->perm, op, batch);
     return p->bp_separable || bp_general_prescaled(p, batch) ? bp_workspace_elems(p) * batch * 4 : 0;
@@ -1088,18 +1110,46 @@ int launch_fp(const tsx_plan *p, const float *vol, int64_t vs, float *proj, int6
               int64_t ws_bytes = 0)
 {
     if (batch <= 0) return TSX_OK;
-    if (fp_swapped(p)) {
-        const int64_t ve = (int64_t)p->nx * p->ny * p->nz;
+    if (fp_refined(p)) {
+        // detector supersampling ss: the projections of the detector refined
+        // ss times (its pixels are the sub-rays), averaged over each pixel's
+        // ss x ss fine pixels into the caller's projections through the
+        // epilogue
+        // (the requirement formed here from its two parts, so that a short
+        // workspace is refused rather than overrun)
+        const int64_t fe = refined_sino(p);
+        const int64_t fine_bytes = align_ws(fe * batch * 4), rest_bytes = workspace_bytes(p->fine, TSX_OP_FP, batch);
         Scratch owned;
         void *wptr = nullptr;
-        int rc = get_workspace(p, workspace_bytes(p, TSX_OP_FP, batch), wsp, ws_bytes, stream, owned, &wptr);
+        int rc = get_workspace(p, fine_bytes + rest_bytes, wsp, ws_bytes, stream, owned, &wptr);
+        if (rc) return rc;
+        float *fine = (float *)wptr;
+        rc = launch_fp(p->fine, vol, vs, fine, fe, batch, plain(0), stream, (char *)wptr + fine_bytes, rest_bytes);
+        if (rc) return rc;
+        const int64_t blocks = (int64_t)((p->cols + 255) / 256) * p->rows * p->n_angles;
+        const int64_t cap = max_blocks(256);
+        for (int64_t b0 = 0; b0 < batch; b0 += GRID_DIM_MAX) {
+            const int64_t nb = std::min(GRID_DIM_MAX, batch - b0);
+            for (int64_t f = 0; f < blocks; f += cap)
+                hipLaunchKernelGGL(reduce_refined, dim3((unsigned)std::min(cap, blocks - f), 1, (unsigned)nb),
+                                   dim3(256), 0, stream, fine + b0 * fe, fe, proj + b0 * ps, ps, p->rows,
+                                   p->n_angles, p->cols, p->ss, f, ep_from(ep, b0));
+        }
+        TSX_HIP_CHECK(hipGetLastError());
+        return TSX_OK;
+    }
+    if (fp_swapped(p)) {
+        const int64_t ve = (int64_t)p->nx * p->ny * p->nz;
+        const tsx_plan *q = p->perm;
+        const int64_t tmp_bytes = align_ws(ve * batch * 4), rest_bytes = workspace_bytes(q, TSX_OP_FP, batch);
+        Scratch owned;
+        void *wptr = nullptr;
+        int rc = get_workspace(p, tmp_bytes + rest_bytes, wsp, ws_bytes, stream, owned, &wptr);
         if (rc) return rc;
         float *tmp = (float *)wptr;
-        const tsx_plan *q = p->perm;
         rc = launch_unswap(vol, vs, tmp, ve, q->nx, q->ny, q->nz, p->swap, batch, plain(0), stream);
         if (rc) return rc;
-        return launch_fp(q, tmp, ve, proj, ps, batch, ep, stream, (char *)wptr + align_ws(ve * batch * 4),
-                         workspace_bytes(q, TSX_OP_FP, batch));
+        return launch_fp(q, tmp, ve, proj, ps, batch, ep, stream, (char *)wptr + tmp_bytes, rest_bytes);
     }
     if (!p->fp_padded || force_general() || p->ss > 4)
         return launch_fp_general<float>(p, vol, vs, proj, ps, batch, ep, stream);
@@ -1176,7 +1226,7 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
               int64_t ws_bytes = 0, int up = 1)
 {
     if (batch <= 0) return TSX_OK;
-    if (p->fine && !force_general()) {
+    if (bp_refined(p)) {
         // detector supersampling ss: the backprojection of the ss x ss
         // sub-rays of every pixel is that of the detector refined ss times
         // (the sub-rays are its pixels' rays), each fine pixel taking its
@@ -1187,14 +1237,14 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
         // the separable backprojector of the axis-swapped plan, then its
         // result moved into this plan's layout through the epilogue
         const int64_t ve = (int64_t)p->nx * p->ny * p->nz;
+        const int64_t tmp_bytes = align_ws(ve * batch * 4), rest_bytes = workspace_bytes(p->perm, TSX_OP_BP, batch);
         Scratch owned;
         void *wptr = nullptr;
-        int rc = get_workspace(p, workspace_bytes(p, TSX_OP_BP, batch), wsp, ws_bytes, stream, owned, &wptr);
+        int rc = get_workspace(p, tmp_bytes + rest_bytes, wsp, ws_bytes, stream, owned, &wptr);
         if (rc) return rc;
         float *tmp = (float *)wptr;
-        char *rest = (char *)wptr + align_ws(ve * batch * 4);
-        rc = launch_bp(p->perm, tmp, ve, proj, ps, batch, plain(0), stream, rest,
-                       workspace_bytes(p->perm, TSX_OP_BP, batch), up);
+        rc = launch_bp(p->perm, tmp, ve, proj, ps, batch, plain(0), stream, (char *)wptr + tmp_bytes, rest_bytes,
+                       up);
         if (rc) return rc;
         return launch_unswap(tmp, ve, vol, vs, p->nx, p->ny, p->nz, p->swap, batch, ep, stream);
     }
@@ -1488,7 +1538,7 @@ std::string tune_key(const tsx_plan *p)
            ":" + std::to_string(p->rows) + "x" + std::to_string(p->cols) + ":" + std::to_string(p->nx) +
            "x" + std::to_string(p->ny) + "x" + std::to_string(p->nz) + ":" + std::to_string(p->ss) + ":" +
            std::to_string((int)p->fp_padded) + std::to_string((int)p->fp_pair) +
-           std::to_string((int)p->bp_separable) + std::to_string((int)p->bp_only) + ":" +
+           std::to_string((int)p->bp_separable) + ":" +
            std::to_string(angle_bucket(p->n_angles)) + ":" + std::to_string(p->fp_shear_rows);
 }
 
@@ -1613,7 +1663,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
     // (a plan whose forward projection runs on its axis-swapped plan leaves the
     // choice to that plan's own tuning)
     const bool fp_choices =
-        p->fp_padded && p->ss >= 1 && p->ss <= 4 && !force_general() && !fp_swapped(p) && !p->bp_only;
+        p->fp_padded && p->ss >= 1 && p->ss <= 4 && !force_general() && !fp_swapped(p) && !fp_refined(p);
     if (fp_choices) {
         // The forward kernel walks all angles of one detector tile before the
         // next tile, so its cache reuse depends on the angle count: the
@@ -1995,15 +2045,15 @@ int tsx_plan_create(const tsx_geometry *g, int device, tsx_plan **out)
         }
         g_last_error.clear();
     }
-    // detector supersampling: the separable backprojector of the same scan
-    // on the refined detector (upright, or v parallel to x or y)
+    // detector supersampling: the same scan on the refined detector, for its
+    // fast kernels
     if (p->ss > 1 && (int64_t)g->det_col_count * p->ss <= BP_RT_MAX_COLS &&
         (int64_t)g->det_row_count * p->ss <= ((int64_t)1 << 24)) {
         std::vector<double> vec;
         tsx_geometry fg = refined_geometry(g, vec);
         tsx_plan *q = nullptr;
-        if (tsx_plan_create(&fg, device, &q) == TSX_OK && (q->bp_separable || q->perm)) {
-            q->bp_only = true;
+        if (tsx_plan_create(&fg, device, &q) == TSX_OK &&
+            (q->bp_separable || q->perm || q->fp_padded || (q->perm && q->perm->fp_padded))) {
             p->fine = q;
         } else {
             tsx_plan_destroy(q);
@@ -2048,9 +2098,11 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
                                     (p->bp_rt ? p->bp_rt_stride * p->n_angles * 4 : 0)
                               : 0);
     for (int c = 0; c < 3; ++c) info->axis_count[c] = p->axis_count[c];
-    const tsx_plan *f = fp_swapped(p) ? p->perm : p;  // the plan whose forward kernels run
-    info->fp_kernel = f->fp_padded && !force_general() ? 1 : 0;
-    const tsx_plan *b = p->fine && !force_general() ? p->fine : p;  // the plan whose backprojector runs,
+    // the plan whose forward kernels run
+    const tsx_plan *f0 = fp_refined(p) ? p->fine : p;
+    const tsx_plan *f = fp_swapped(f0) ? f0->perm : f0;
+    info->fp_kernel = f->fp_padded && f->ss <= 4 && !force_general() ? 1 : 0;
+    const tsx_plan *b = bp_refined(p) ? p->fine : p;  // the plan whose backprojector runs,
     const tsx_plan *bs = b->perm ? b->perm : b;                      // on its axis-swapped plan if any
     info->bp_kernel = (b->bp_separable || b->perm) && !force_general() ? 1 : 0;
     info->fp_variant = fp_variant_of(f);
@@ -2058,7 +2110,8 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->tuned = tuned_tree(p);
     info->fp_slabs = info->fp_kernel ? std::max<int32_t>(1, (int32_t)f->slabs.size()) : 0;
     info->fp_tile_rows = info->fp_kernel ? fp_tile_rows_of(f) : 0;
-    info->fp_axis_swap = fp_swapped(p) ? p->swap : 0;
+    info->fp_axis_swap = fp_swapped(f0) ? f0->swap : 0;
+    info->fp_refined_detector = f0 != p ? p->ss : 0;
     info->fp_shear_rows = info->fp_kernel ? fp_shear_rows_of(f) : 0;
     info->bp_launch_angles = info->bp_kernel ? bp_chunk_of(bs) : 0;
     info->bp_axis_swap = info->bp_kernel ? b->swap : 0;
