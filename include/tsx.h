@@ -140,6 +140,10 @@ typedef struct {
                                   (sheared tiles, rows of a tile column taken cyclically), a
                                   lane's row offset at most this many rows; 0 for tiles along
                                   the detector rows */
+    int32_t fp_column_waves;   /* padded forward projector: 1 if a wave's rays run along a
+                                  detector column (every angle's rays keep to one row of the
+                                  slices along a direction closer to v than to u; the shear
+                                  of fp_shear_rows is then in columns), 0 along a row */
     int32_t fp_refined_detector; /* detector supersampling ss > 1 with the padded forward
                                   projector on the detector refined ss times: ss (each pixel
                                   the mean of its ss x ss fine pixels); 0 otherwise. The fp_*

@@ -124,9 +124,15 @@ def test_sheared_forward_tiles():
     turned = ts.rotate(pos=0, axis=(0, 1, 0), angles=np.pi / 2) * pg
     assert _plan(vg, turned).info()["fp_shear_rows"] == 0
     rolled = ts.rotate(pos=0, axis=(0, 1, 1), angles=0.2) * pg
-    e = _plan(vg, rolled).info()["fp_shear_rows"]
+    info = _plan(vg, rolled).info()
     # |slope| about tan(0.2)-ish over a wave of 64: a few rows each side
-    assert 2 <= e <= 16, e
+    assert 2 <= info["fp_shear_rows"] <= 16 and not info["fp_column_waves"], info
+    # turned 90 degrees about y and rolled a little: no axis swap (v is not
+    # parallel to x), the detector's rows run along the slices' rows, so the
+    # waves run along its columns, sheared by a few columns
+    both = ts.rotate(pos=0, axis=(0, 1, 1), angles=0.15) * turned
+    info = _plan(vg, both).info()
+    assert info["fp_axis_swap"] == 0 and info["fp_column_waves"] and 1 <= info["fp_shear_rows"] <= 16, info
 
 
 def test_refined_detector_workspace():

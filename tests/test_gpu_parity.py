@@ -357,6 +357,46 @@ def test_sheared_forward_tiles(monkeypatch, kind):
         assert torch.equal(yi, op(xi))
 
 
+@pytest.mark.parametrize("case", ["rolled_cone_ss2", "rolled_parallel_ss3", "turned_x_cone_ss2",
+                                  "turned_y_parallel_ss2", "rolled_turned_cone"])
+def test_geometry_path_combinations(case):
+    """The plans a plan holds, combined: a rolled detector with supersampling
+    (forward on the refined plan with sheared tiles, backward on the general
+    kernel with sub-rays), a turned scan with supersampling (both projectors
+    on the refined plan's axis-swapped plan) and a turned scan whose detector
+    is also rolled a little (no swap: waves along detector columns, sheared;
+    general backprojector).
+    Parity with the oracle and the matched adjoint at 1e-5."""
+    vg = ts.volume(shape=(24, 28, 20), size=(24 / 28, 1, 20 / 28))
+    kind, ss = ("parallel" if "parallel" in case else "cone"), (int(case[-1]) if "_ss" in case else 1)
+    # (detector rows enough for the shear of a 0.5 rad roll: at most half the rows)
+    if kind == "cone":
+        base = ts.cone(angles=14, shape=(36, 24), size=(1.4, 1.4), cone_angle=0.5).to_vec()
+    else:
+        base = ts.parallel(angles=14, shape=(36, 24), size=(1.0, 1.0)).to_vec()
+    pg = base
+    if "turned_x" in case:
+        pg = ts.rotate(pos=0, axis=(0, 0, 1), angles=np.pi / 2) * pg
+    if "turned_y" in case or case == "rolled_turned_cone":
+        pg = ts.rotate(pos=0, axis=(0, 1, 0), angles=np.pi / 2) * pg
+    if "rolled" in case:
+        pg = ts.rotate(pos=0, axis=(0, 1, 1), angles=0.5 if case != "rolled_turned_cone" else 0.15) * pg
+    info = ts.operator(vg, pg, detector_supersampling=ss).projector.plan(0).info()
+    print(f"[combo] {case}: {info}", flush=True)
+    if "turned" in case and "rolled" not in case:
+        assert info["fp_axis_swap"] and info["bp_axis_swap"]
+    if ss > 1:
+        assert info["fp_refined_detector"] == ss
+        assert info["bp_refined_detector"] == (ss if "turned" in case else 0)
+    if "rolled" in case:
+        assert info["bp_kernel"] == "general" and info["fp_shear_rows"] > 0
+    # on the turned scan rolled a little the detector's rows run along the
+    # slices' rows: waves along its columns, sheared in columns
+    assert info["fp_column_waves"] == (case == "rolled_turned_cone")
+    x = ts.phantom.ellipsoids(28, count=5, seed=len(case))[:24, :28, :20].copy()
+    _check_fp_bp(vg, pg, x, ss=ss)
+
+
 def test_additive_and_out():
     vg = ts.volume(shape=32, size=1)
     pg = ts.parallel(angles=20, shape=(32, 48), size=(1, 1.5))

@@ -102,6 +102,7 @@ class TsxPlanInfo(ctypes.Structure):
         ("bp_axis_swap", ctypes.c_int32),
         ("fp_axis_swap", ctypes.c_int32),
         ("fp_shear_rows", ctypes.c_int32),
+        ("fp_column_waves", ctypes.c_int32),
         ("fp_refined_detector", ctypes.c_int32),
         ("bp_refined_detector", ctypes.c_int32),
     ]
@@ -232,6 +233,7 @@ class Plan:
             "fp_slabs": inf.fp_slabs, "bp_launch_angles": inf.bp_launch_angles,
             "fp_tile_rows": inf.fp_tile_rows, "bp_axis_swap": inf.bp_axis_swap, "fp_axis_swap": inf.fp_axis_swap,
             "fp_shear_rows": inf.fp_shear_rows,
+            "fp_column_waves": bool(inf.fp_column_waves),
             "fp_refined_detector": inf.fp_refined_detector,
             "bp_refined_detector": inf.bp_refined_detector,
         }

@@ -767,7 +767,8 @@ __device__ __forceinline__ float march_lds(__amdgpu_buffer_rsrc_t rs, const RayS
 struct FpView {
     const AngleParams *angles;
     int n_angles, rows, cols, tiles_u, tiles_v, batch;
-    int shear_rows;  // sheared tiles: bound on a lane's row offset (< tiles_v * BY; 0: no shear)
+    int shear_rows;    // sheared tiles: bound on a lane's offset across the waves (<= tiles_v * BY; 0: none)
+    int column_waves;  // 1: a wave's lanes take consecutive rows of a column (tiles_u along rows)
     int a0, na;  // angles a0 .. a0 + na - 1 of the n_angles are projected by this launch
     Layouts L;
     float *proj;
@@ -786,22 +787,26 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
     const int tiles = v.tiles_u * v.tiles_v;
     const int tile = rest % tiles;
     const int b = rest / tiles;
-    const int j = (tile % v.tiles_u) * FP_BX + threadIdx.x;
-    int i = (tile / v.tiles_u) * BY + threadIdx.y;
+    // lanes along a detector row (columns j) or, for column waves, along a
+    // column (rows i); the block's BY waves take neighbouring rows (columns)
+    const int along = (tile % v.tiles_u) * FP_BX + threadIdx.x;
+    int across = (tile / v.tiles_u) * BY + threadIdx.y;
     if (b >= v.batch) return;  // uniform over the block
     const AngleParams &A = v.angles[a];
     if (v.shear_rows) {
         // sheared tiles: the wave's rays follow the detector direction that
-        // keeps them on one row of the slices. For a fixed column the offset
-        // s is fixed, and rows are taken modulo the tiled height R = tiles_v
-        // * BY (|s| <= E < R), so the tiles of the column still cover each
-        // of its rows exactly once; the waves at the column's ends hold rows
-        // from both ends.
+        // keeps them on one row of the slices. For a fixed lane position
+        // `along` the offset s is fixed, and the positions across the waves
+        // are taken modulo the tiled extent R = tiles_v * BY (|s| <= E <= R),
+        // so the tiles still cover each pixel exactly once; the waves at the
+        // ends hold pixels from both ends.
         const int E = v.shear_rows, R = v.tiles_v * BY;
         const int s = (int)floorf(A.shear * (float)((int)threadIdx.x - FP_BX / 2) + 0.5f);
-        i += min(max(s, -E), E);
-        i += i < 0 ? R : (i >= R ? -R : 0);
+        across += min(max(s, -E), E);
+        across += across < 0 ? R : (across >= R ? -R : 0);
     }
+    const int j = v.column_waves ? across : along;
+    const int i = v.column_waves ? along : across;
     const bool inside = j < v.cols && i < v.rows;
     if (PIPE < 3 && !inside) return;
     const float *lp = uniform_ptr(v.L.ptr[A.m] + (int64_t)b * v.L.stride);

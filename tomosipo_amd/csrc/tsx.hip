@@ -145,6 +145,10 @@ struct tsx_plan {
     // would reach half the detector's rows)
     int32_t fp_shear_rows = 0;
     int32_t fp_shear = -1;  // sheared tiles chosen by tsx_plan_autotune: 1 on, 0 off (-1: on)
+    // forward tiles whose waves run along detector columns (every angle's rays
+    // keep to one row of the slices along a direction closer to the
+    // detector's v than to its u); the shear is then in columns per row
+    bool fp_column_waves = false;
     std::string bp_reason;      // why bp_separable is off
     // kernel variants chosen by tsx_plan_autotune (-1: library default)
     int32_t fp_variant = -1, bp_bz = -1, fp_by = -1;
@@ -176,14 +180,14 @@ V3 cross(V3 a, V3 b)
 }
 bool finite3(V3 a) { return std::isfinite(a[0]) && std::isfinite(a[1]) && std::isfinite(a[2]); }
 
-// Slope di/dj of the detector direction along which the rays of an angle
-// meet one c1 row of the slices (at the detector's centre), the forward
-// projector's tile shear: with Q1 = S1 + (n - S0) D1 / D0 (cone, D = P - S)
-// or P1 + (n - P0_m) S1 / S0 (parallel, D = S), dQ1 = 0 along di/dj =
-// -(U1 - U0 D1 / D0) / (V1 - V0 D1 / D0). An upright detector (v along c1)
-// has slope 0; slopes under 1/64 (less than half a row across a wave) are
-// taken as 0 and the slope is held to [-1, 1].
-float tile_shear(const AngleParams &A, int rows, int cols)
+// Derivatives (dQ1/dj, dQ1/di) of the c1 position at which the rays of an
+// angle meet a slice, at the detector's centre (up to a common factor): with
+// Q1 = S1 + (n - S0) D1 / D0 (cone, D = P - S) or P1 + (n - P0_m) S1 / S0
+// (parallel, D = S), dQ1/dj is proportional to U1 - U0 D1 / D0 and dQ1/di to
+// V1 - V0 D1 / D0. The rays keep to one row of the slices along di/dj =
+// -dj/di: the forward projector's tiles follow that direction (an upright
+// detector has dQ1/dj = 0: waves along its rows, unsheared).
+void tile_slopes(const AngleParams &A, int rows, int cols, double *dj, double *di)
 {
     const double jc = 0.5 * (cols - 1), ic = 0.5 * (rows - 1);
     double D[3];
@@ -191,11 +195,21 @@ float tile_shear(const AngleParams &A, int rows, int cols)
         const double P = A.P0[k] + jc * A.U[k] + ic * A.V[k];
         D[k] = A.cone ? P - A.S[k] : A.S[k];
     }
-    if (D[0] == 0.0) return 0.0f;
-    const double dj = A.U[1] - A.U[0] * D[1] / D[0];
-    const double di = A.V[1] - A.V[0] * D[1] / D[0];
-    if (!(std::fabs(di) > 0.0)) return 0.0f;
-    const double t = -dj / di;
+    *dj = *di = 0.0;
+    if (D[0] == 0.0) return;
+    *dj = A.U[1] - A.U[0] * D[1] / D[0];
+    *di = A.V[1] - A.V[0] * D[1] / D[0];
+}
+
+// Shear of a wave (offset across the wave per lane) from the slopes: waves
+// along detector rows take -dj/di rows per column, waves along columns -di/dj
+// columns per row; shears under 1/64 (less than half a row across a wave) are
+// taken as 0 and shears are held to [-1, 1].
+float tile_shear(double dj, double di, bool column)
+{
+    const double num = column ? -di : -dj, den = column ? dj : di;
+    if (!(std::fabs(den) > 0.0)) return 0.0f;
+    const double t = num / den;
     if (!(std::fabs(t) >= 1.0 / 64.0)) return 0.0f;
     return (float)std::max(-1.0, std::min(1.0, t));
 }
@@ -326,6 +340,7 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
     const V3 centre = mk(0.5 * (n3[0] - 1), 0.5 * (n3[1] - 1), 0.5 * (n3[2] - 1));
     const bool cone = g->type == TSX_CONE_VEC;
     p->host.resize(g->n_angles);
+    std::vector<double> slope_j(g->n_angles), slope_i(g->n_angles);  // tile_slopes per angle
     for (int a = 0; a < g->n_angles; ++a) {
         const double *vec = g->vectors + 12 * (int64_t)a;
         V3 first, det, u, v;
@@ -435,12 +450,24 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
             A.fV[k] = (float)A.V[k];
         }
         parallel_map(A);
-        A.shear = tile_shear(A, g->det_row_count, g->det_col_count);
-        p->fp_shear_rows = std::max(p->fp_shear_rows,
-                                    (int32_t)std::ceil(32.0 * std::fabs((double)A.shear) - 1e-9));
+        tile_slopes(A, g->det_row_count, g->det_col_count, &slope_j[a], &slope_i[a]);
         p->axis_count[m] += 1;
     }
-    if (2 * p->fp_shear_rows > p->rows) p->fp_shear_rows = 0;
+    // forward tiles: waves along detector columns where every angle's rays
+    // keep to a slice row along a direction closer to v than to u, else along
+    // rows; the largest offset across a wave, dropped where it would reach
+    // half the detector's extent across the waves
+    p->fp_column_waves = p->n_angles > 0;
+    for (int a = 0; a < p->n_angles; ++a)
+        p->fp_column_waves = p->fp_column_waves && std::fabs(slope_j[a]) > std::fabs(slope_i[a]);
+    p->fp_shear_rows = 0;
+    for (int a = 0; a < p->n_angles; ++a) {
+        AngleParams &A = p->host[a];
+        A.shear = tile_shear(slope_j[a], slope_i[a], p->fp_column_waves);
+        p->fp_shear_rows =
+            std::max(p->fp_shear_rows, (int32_t)std::ceil(32.0 * std::fabs((double)A.shear) - 1e-9));
+    }
+    if (2 * p->fp_shear_rows > (p->fp_column_waves ? p->cols : p->rows)) p->fp_shear_rows = 0;
     for (int c = 0; c < 3; ++c) {
         p->vlo[c] = g->window_min[c];
         p->vh[c] = h[c];
@@ -1019,8 +1046,11 @@ FpMultiKernel fp_multi_kernel(int variant, int by)
 int launch_fp_padded(const tsx_plan *p, int variant, int by, bool shear, const float *vol, int64_t vs, float *proj,
                      int64_t ps, int64_t batch, Epilogue ep, float *base, hipStream_t stream)
 {
-    const int tiles_u = (p->cols + FP_BX - 1) / FP_BX;
-    const int tiles_v = (p->rows + by - 1) / by;
+    // tiles: FP_BX lanes along the wave (columns, or rows for column waves)
+    // by BY waves
+    const int along = p->fp_column_waves ? p->rows : p->cols, across = p->fp_column_waves ? p->cols : p->rows;
+    const int tiles_u = (along + FP_BX - 1) / FP_BX;
+    const int tiles_v = (across + by - 1) / by;
     const int es = esize_of(variant);
     // padded copies of every batch entry for each stepping axis in use
     int64_t per_entry = 0;
@@ -1052,6 +1082,7 @@ int launch_fp_padded(const tsx_plan *p, int variant, int by, bool shear, const f
             v.tiles_u = tiles_u;
             v.tiles_v = tiles_v;
             v.shear_rows = shear ? p->fp_shear_rows : 0;
+            v.column_waves = p->fp_column_waves ? 1 : 0;
             v.batch = (int)nbat;
             for (int c = 0; c < 3; ++c) {
                 v.L.ptr[c] = p->axis_count[c] > 0 ? base + b0 * per_entry + offs[c] : nullptr;
@@ -1376,8 +1407,9 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         if (!vols[k] || !projs[k]) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
         const int64_t wb = align_up(workspace_bytes(p, TSX_OP_FP, 1));
         const int pby = fp_tile_rows_of(p);
-        const int tiles_u = (p->cols + FP_BX - 1) / FP_BX;
-        const int tiles_v = (p->rows + pby - 1) / pby;
+        const int along = p->fp_column_waves ? p->rows : p->cols, across = p->fp_column_waves ? p->cols : p->rows;
+        const int tiles_u = (along + FP_BX - 1) / FP_BX;
+        const int tiles_v = (across + pby - 1) / pby;
         const int64_t blocks = (int64_t)tiles_u * tiles_v * p->n_angles;
         const int64_t cap = max_blocks(FP_BX * pby);
         if (!(fast && p->fp_padded && p->ss == 1 && p->slabs.empty() && !fp_swapped(p)) || blocks > cap) {
@@ -1409,6 +1441,7 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         v.tiles_u = tiles_u;
         v.tiles_v = tiles_v;
         v.shear_rows = fp_shear_rows_of(p);
+        v.column_waves = p->fp_column_waves ? 1 : 0;
         v.batch = 1;
         for (int c = 0; c < 3; ++c) {
             v.L.ptr[c] = p->axis_count[c] > 0 ? base + offs[c] : nullptr;
@@ -1539,7 +1572,8 @@ std::string tune_key(const tsx_plan *p)
            "x" + std::to_string(p->ny) + "x" + std::to_string(p->nz) + ":" + std::to_string(p->ss) + ":" +
            std::to_string((int)p->fp_padded) + std::to_string((int)p->fp_pair) +
            std::to_string((int)p->bp_separable) + ":" +
-           std::to_string(angle_bucket(p->n_angles)) + ":" + std::to_string(p->fp_shear_rows);
+           std::to_string(angle_bucket(p->n_angles)) + ":" + std::to_string(p->fp_shear_rows) +
+           (p->fp_column_waves ? "c" : "r");
 }
 
 // Mean time of `reps` launches after one warm-up launch, in milliseconds.
@@ -1693,6 +1727,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
         band.fp_padded = p->fp_padded;
         band.fp_pair = p->fp_pair;
         band.fp_shear_rows = fpp->fp_shear_rows;
+        band.fp_column_waves = fpp->fp_column_waves;
         band.dev = t.band_angles;
         auto time_band = [&](int cand, int by, int band_rows, float *ms) -> int {
             const int r0 = (p->rows - band_rows) / 2;
@@ -2113,6 +2148,7 @@ int tsx_plan_info_get(const tsx_plan *p, tsx_plan_info *info)
     info->fp_axis_swap = fp_swapped(f0) ? f0->swap : 0;
     info->fp_refined_detector = f0 != p ? p->ss : 0;
     info->fp_shear_rows = info->fp_kernel ? fp_shear_rows_of(f) : 0;
+    info->fp_column_waves = info->fp_kernel && f->fp_column_waves ? 1 : 0;
     info->bp_launch_angles = info->bp_kernel ? bp_chunk_of(bs) : 0;
     info->bp_axis_swap = info->bp_kernel ? b->swap : 0;
     info->bp_refined_detector = b != p ? p->ss : 0;
