@@ -140,9 +140,10 @@ struct tsx_plan {
     // forward projector this plan's forward projection (bp_refined,
     // fp_refined)
     tsx_plan *fine = nullptr;
-    // forward tiles sheared along each angle's shear: the largest row offset
-    // of a lane, ceil(32 max |shear|) (0: unsheared tiles; also 0 where it
-    // would reach half the detector's rows)
+    // forward tiles sheared along each angle's shear: the largest offset of a
+    // lane across the waves (rows, or columns for column waves), ceil(32 max
+    // |shear|) (0: unsheared tiles; also 0 where it would reach half the
+    // detector's extent across the waves)
     int32_t fp_shear_rows = 0;
     int32_t fp_shear = -1;  // sheared tiles chosen by tsx_plan_autotune: 1 on, 0 off (-1: on)
     // forward tiles whose waves run along detector columns (every angle's rays

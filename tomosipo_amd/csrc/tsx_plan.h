@@ -37,9 +37,10 @@ struct AngleParams {
     // is singular
     double par_c[8];
     float par_a[4], par_b[2];
-    // forward tiles: slope di/dj of the detector direction along which rays
-    // meet one c1 row of the slices (0: along the detector rows); a wave's
-    // lane tx takes row offset round(shear (tx - 32)) within its tile
+    // forward tiles: slope of the detector direction along which rays meet
+    // one c1 row of the slices, di/dj for waves along detector rows (0: the
+    // rows themselves), dj/di for waves along columns; a wave's lane tx takes
+    // the offset round(shear (tx - 32)) across the waves
     float shear;
 };
 
