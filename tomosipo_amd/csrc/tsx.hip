@@ -2220,7 +2220,15 @@ int tsx_plan_autotune(tsx_plan *p, void *stream)
     TraceRange trace("tsx_plan_autotune");
     DeviceGuard dg(p->device);
     if (!dg.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
-    return autotune_tree(p, (hipStream_t)stream);
+    rc = autotune_tree(p, (hipStream_t)stream);
+    // the trials' stream-ordered scratch (freed, the stream synchronised) goes
+    // back to the device instead of staying in the default pool, which the
+    // library keeps from shrinking, for the rest of the process
+    hipMemPool_t pool;
+    if (hipStreamSynchronize((hipStream_t)stream) == hipSuccess &&
+        hipDeviceGetDefaultMemPool(&pool, p->device) == hipSuccess)
+        (void)hipMemPoolTrimTo(pool, 0);
+    return rc;
 }
 
 int tsx_plan_axes(const tsx_plan *p, int32_t *axes)
