@@ -60,11 +60,11 @@ def test_kernel_selection():
     info = _plan(vg, base, det_supersampling=2).info()
     assert (info["bp_kernel"], info["bp_refined_detector"]) == ("separable", 2)
     assert _plan(vg, tilted, det_supersampling=2).info()["bp_kernel"] == "general"
-    # a c1 row of the x-stepping layout of 2048^2 slices is 16.8 MB: beyond the
-    # 24-bit row offsets of the padded kernel, so the general kernel is chosen
+    # a c1 row of the row-paired x-stepping layout of 2048^2 slices is 33.6 MB,
+    # beyond 24-bit row offsets: the padded kernel with 32-bit row offsets
     wide = ts.volume(shape=(8, 2048, 2048), size=(0.004, 1, 1))
     par = ts.parallel(angles=4, shape=(8, 2048), size=(0.004, 1.5))
-    assert _plan(wide, par).info()["fp_kernel"] == "general"
+    assert _plan(wide, par).info()["fp_kernel"] == "padded"
     assert _plan(ts.volume(shape=(8, 1024, 1024), size=(0.008, 1, 1)),
                  ts.parallel(angles=4, shape=(8, 1024), size=(0.008, 1.5))).info()["fp_kernel"] == "padded"
 

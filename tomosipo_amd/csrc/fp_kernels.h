@@ -438,12 +438,21 @@ struct SyncRay {
     float a1, a2, R1, R2, lo1, hi1, lo2, hi2;
 };
 
+// Byte offset of row k1: a 24-bit multiply, or (WIDE: rows of 2^24 bytes or
+// more) a 32-bit one.
+template <bool WIDE>
+__device__ __forceinline__ uint32_t row_offset(uint32_t k1, uint32_t rowb)
+{
+    return WIDE ? k1 * rowb : __umul24(k1, rowb);
+}
+
+template <bool WIDE>
 __device__ __forceinline__ void request_sync(__amdgpu_buffer_rsrc_t rs, const SyncRay &r, float dn,
                                              uint32_t rowoff, uint32_t rowb, Corners &c)
 {
     const float q1 = __builtin_amdgcn_fmed3f(fmaf(dn, r.a1, r.R1), r.lo1, r.hi1);
     const float q2 = __builtin_amdgcn_fmed3f(fmaf(dn, r.a2, r.R2), r.lo2, r.hi2);
-    const uint32_t off = rowoff + __umul24((uint32_t)floor_to_int(q1), rowb) +
+    const uint32_t off = rowoff + row_offset<WIDE>((uint32_t)floor_to_int(q1), rowb) +
                          ((uint32_t)floor_to_int(q2) << 2);
     c.v00 = bload(rs, off);
     c.v01 = bload(rs, off + 4u);
@@ -455,12 +464,13 @@ __device__ __forceinline__ void request_sync(__amdgpu_buffer_rsrc_t rs, const Sy
 
 // Paired layout: the 16 bytes at element (k1, k2) are v[k1][k2], v[k1+1][k2],
 // v[k1][k2+1], v[k1+1][k2+1] (one wave-wide 16-byte load per sample).
+template <bool WIDE>
 __device__ __forceinline__ void request_rowpair(__amdgpu_buffer_rsrc_t rs, const SyncRay &r, float dn,
                                                 uint32_t rowoff, uint32_t rowb, Corners &c)
 {
     const float q1 = __builtin_amdgcn_fmed3f(fmaf(dn, r.a1, r.R1), r.lo1, r.hi1);
     const float q2 = __builtin_amdgcn_fmed3f(fmaf(dn, r.a2, r.R2), r.lo2, r.hi2);
-    const uint32_t off = rowoff + __umul24((uint32_t)floor_to_int(q1), rowb) +
+    const uint32_t off = rowoff + row_offset<WIDE>((uint32_t)floor_to_int(q1), rowb) +
                          ((uint32_t)floor_to_int(q2) << 3);
     c.v00 = bload(rs, off);
     c.v10 = bload(rs, off + 4u);
@@ -494,7 +504,7 @@ __device__ __forceinline__ void block_range(int &lo, int &hi)
     hi = __builtin_amdgcn_readfirstlane(hi);
 }
 
-template <bool ROWPAIR, int BY>
+template <bool ROWPAIR, int BY, bool WIDE = false>
 __device__ __forceinline__ float march_padded_sync(__amdgpu_buffer_rsrc_t rs, const RaySetup &r,
                                                    bool valid, int nref, int n1, int n2,
                                                    uint32_t st_m, uint32_t st_1)
@@ -521,9 +531,9 @@ __device__ __forceinline__ float march_padded_sync(__amdgpu_buffer_rsrc_t rs, co
     const uint32_t rowb = st_1 * es, stepb = st_m * es;
     auto req = [&](float dn_, uint32_t ro_, Corners &c_) {
         if (ROWPAIR)
-            request_rowpair(rs, q, dn_, ro_, rowb, c_);
+            request_rowpair<WIDE>(rs, q, dn_, ro_, rowb, c_);
         else
-            request_sync(rs, q, dn_, ro_, rowb, c_);
+            request_sync<WIDE>(rs, q, dn_, ro_, rowb, c_);
     };
     float dn = (float)(wlo - nref);
     uint32_t rowoff = (uint32_t)wlo * stepb;
@@ -779,9 +789,10 @@ struct FpView {
 // Work item `logical` of one instance: (batch entry, detector tile, angle),
 // angle fastest so that blocks resident together share a detector tile and
 // therefore a slab of the volume.
-template <int SS, int PIPE, int BY>
+template <int SS, int PIPE, int BY, bool WIDE = false>
 __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
 {
+    static_assert(!WIDE || PIPE == 3 || PIPE == 4, "rows of 2^24 bytes or more: the wave-synchronous marches");
     const int a = v.a0 + logical % v.na;
     const int rest = logical / v.na;
     const int tiles = v.tiles_u * v.tiles_v;
@@ -830,7 +841,7 @@ __device__ __forceinline__ void fp_padded_work(const FpView &v, int logical)
                 // every lane of the block joins the march (block-wide range,
                 // barriers inside; uniform calls)
                 const bool ok = inside && ray_setup(A, fj, fi, (double)PAD, r);
-                acc += march_padded_sync<PIPE == 4, BY>(rs, r, ok, A.n[0] / 2, A.n[1], A.n[2], st_m, st_1);
+                acc += march_padded_sync<PIPE == 4, BY, WIDE>(rs, r, ok, A.n[0] / 2, A.n[1], A.n[2], st_m, st_1);
             } else if (ray_setup(A, fj, fi, (double)PAD, r)) {
                 acc += march_padded_pipe4(rs, r, st_m, st_1);
             }
@@ -886,12 +897,14 @@ __device__ __forceinline__ int xcd_logical_of(unsigned vb, int shift)
 __device__ __forceinline__ int xcd_logical(int shift) { return xcd_logical_of(blockIdx.x, shift); }
 
 // Block of FP_BX x BY threads: a tile of BY detector rows (one wave each).
-template <int SS, int PIPE, int BY>
+// WIDE: layouts whose rows reach 2^24 bytes (slices of 2048 x 2048 and up),
+// row offsets formed with 32-bit multiplies.
+template <int SS, int PIPE, int BY, bool WIDE = false>
 __global__ void __launch_bounds__(FP_BX *BY) fp_padded(FpView v, int n_blocks, int shift)
 {
     const int logical = xcd_logical(shift);
     if (logical >= n_blocks) return;
-    fp_padded_work<SS, PIPE, BY>(v, logical);
+    fp_padded_work<SS, PIPE, BY, WIDE>(v, logical);
 }
 
 // Fused launch over up to MAX_FUSED instances (any shapes): instance k owns
@@ -910,7 +923,7 @@ struct FpMulti {
 };
 
 template <int PIPE, int BY>
-__global__ void __launch_bounds__(FP_BX *BY) fp_padded_multi(FpMulti m)
+__global__ void __launch_bounds__(FP_BX *BY) fp_padded_multi(FpMulti m)  // (rows under 2^24 bytes)
 {
     const int logical = xcd_logical(m.shift);
     if (logical >= m.begin[m.n]) return;

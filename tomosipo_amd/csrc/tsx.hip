@@ -150,6 +150,9 @@ struct tsx_plan {
     // keep to one row of the slices along a direction closer to the
     // detector's v than to its u); the shear is then in columns per row
     bool fp_column_waves = false;
+    // padded layouts whose rows reach 2^24 bytes: kernels with 32-bit row
+    // offset multiplies, marches 3 and 4 on tiles of 8 rows
+    bool fp_wide = false;
     std::string bp_reason;      // why bp_separable is off
     // kernel variants chosen by tsx_plan_autotune (-1: library default)
     int32_t fp_variant = -1, bp_bz = -1, fp_by = -1;
@@ -480,16 +483,14 @@ int build_angles(const tsx_geometry *g, tsx_plan *p)
     p->layout_elems[1] = (n3[2] + P2) * n3[1] * (n3[0] + P2);
     p->layout_elems[2] = n3[2] * (n3[1] + P2) * (n3[0] + P2);
     // elements between rows of c1 in each layout; the march forms row offsets
-    // with 24-bit multiplies, so a row must stay below 2^24 bytes
+    // with 24-bit multiplies, or with 32-bit ones where a row-paired row
+    // reaches 2^24 bytes (fp_wide: slices of 2048 x 2048 and up)
     const int64_t row1[3] = {n3[0] * (n3[1] + P2), n3[1] * (n3[0] + P2), n3[0] + P2};
     p->fp_padded = true;
     p->fp_pair = true;
-    for (int c = 0; c < 3; ++c) {
-        if (p->axis_count[c] == 0) continue;
-        if (row1[c] * 4 >= ((int64_t)1 << 24)) p->fp_padded = false;
-        if (row1[c] * 8 >= ((int64_t)1 << 24)) p->fp_pair = false;
-    }
-    p->fp_pair = p->fp_pair && p->fp_padded;
+    p->fp_wide = false;
+    for (int c = 0; c < 3; ++c)
+        if (p->axis_count[c] > 0 && row1[c] * 8 >= ((int64_t)1 << 24)) p->fp_wide = true;
     // The march addresses a layout with 32-bit byte offsets. A volume whose
     // layouts reach chunk_bytes is projected in z-slabs of at most `depth`
     // slices: the layouts of a slab of depth d hold (d + 2 PAD) K_c elements
@@ -611,6 +612,7 @@ int fp_variant_of(const tsx_plan *p)
 {
     const int e = env_int("TSX_FP_VARIANT");
     int v = (e >= 2 && e <= 5) ? e : (p->fp_variant >= 0 ? p->fp_variant : 4);
+    if (p->fp_wide && v != 3) v = 4;  // (wide rows: marches 3 and 4 only)
     if (v == 4 && !p->fp_pair) v = 3;
     return v;
 }
@@ -624,7 +626,7 @@ int fp_esize(const tsx_plan *p) { return fp_variant_of(p) == 4 ? 2 : 1; }
 int fp_tile_rows_of(const tsx_plan *p)
 {
     const int v = fp_variant_of(p);
-    if (p->ss != 1 || (v != 3 && v != 4)) return 8;
+    if (p->ss != 1 || p->fp_wide || (v != 3 && v != 4)) return 8;
     const int e = env_int("TSX_FP_TILE_ROWS");
     if (e == 4 || e == 8 || e == 16) return e;
     return p->fp_by > 0 ? p->fp_by : 8;
@@ -1018,8 +1020,24 @@ FpKernel fp_kernel_ss(int variant)
     return fp_padded<SS, 2, 8>;
 }
 
-FpKernel fp_kernel(int ss, int variant, int by)
+// Kernels for layouts whose rows reach 2^24 bytes: the wave-synchronous
+// marches (row-paired or plain) on tiles of 8 rows.
+template <int SS>
+FpKernel fp_kernel_wide_ss(int variant)
 {
+    return variant == 4 ? fp_padded<SS, 4, 8, true> : fp_padded<SS, 3, 8, true>;
+}
+
+FpKernel fp_kernel(int ss, int variant, int by, bool wide = false)
+{
+    if (wide) {
+        switch (ss) {
+        case 1: return fp_kernel_wide_ss<1>(variant);
+        case 2: return fp_kernel_wide_ss<2>(variant);
+        case 3: return fp_kernel_wide_ss<3>(variant);
+        default: return fp_kernel_wide_ss<4>(variant);
+        }
+    }
     if (ss == 1 && by == 4) return variant == 4 ? fp_padded<1, 4, 4> : fp_padded<1, 3, 4>;
     if (ss == 1 && by == 16) return variant == 4 ? fp_padded<1, 4, 16> : fp_padded<1, 3, 16>;
     switch (ss) {
@@ -1096,7 +1114,7 @@ int launch_fp_padded(const tsx_plan *p, int variant, int by, bool shear, const f
             const int n_blocks = (int)(tiles * v.na * nbat);
             int shift;
             const unsigned grid = xcd_grid(n_blocks, &shift);
-            hipLaunchKernelGGL(fp_kernel(p->ss, variant, by), dim3(grid), dim3(FP_BX, by), 0, stream, v, n_blocks,
+            hipLaunchKernelGGL(fp_kernel(p->ss, variant, by, p->fp_wide), dim3(grid), dim3(FP_BX, by), 0, stream, v, n_blocks,
                                shift);
             TSX_HIP_CHECK(hipGetLastError());
         }
@@ -1413,7 +1431,8 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         const int tiles_v = (across + pby - 1) / pby;
         const int64_t blocks = (int64_t)tiles_u * tiles_v * p->n_angles;
         const int64_t cap = max_blocks(FP_BX * pby);
-        if (!(fast && p->fp_padded && p->ss == 1 && p->slabs.empty() && !fp_swapped(p)) || blocks > cap) {
+        if (!(fast && p->fp_padded && p->ss == 1 && p->slabs.empty() && !fp_swapped(p) && !p->fp_wide) ||
+            blocks > cap) {
             // this instance takes its own launch(es)
             int rc = launch_fp(p, vols[k], 0, projs[k], 0, 1, eps[k], stream, wptr ? (char *)wptr + woff : nullptr, wb);
             if (rc) return rc;
@@ -1729,6 +1748,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
         band.fp_pair = p->fp_pair;
         band.fp_shear_rows = fpp->fp_shear_rows;
         band.fp_column_waves = fpp->fp_column_waves;
+        band.fp_wide = fpp->fp_wide;
         band.dev = t.band_angles;
         auto time_band = [&](int cand, int by, int band_rows, float *ms) -> int {
             const int r0 = (p->rows - band_rows) / 2;
@@ -1760,6 +1780,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
         double best_ms = 1e300;
         for (int cand : {4, 5, 3, 2}) {
             if (cand == 4 && !p->fp_pair) continue;
+            if (p->fp_wide && (cand == 5 || cand == 2)) continue;
             double predicted = 0.0;
             int rc = predict(cand, 8, &predicted);
             if (rc) return rc;
@@ -1769,7 +1790,7 @@ int autotune(tsx_plan *p, hipStream_t stream)
                 best.fp_by = 8;
             }
         }
-        if (p->ss == 1 && (best.fp_variant == 3 || best.fp_variant == 4)) {
+        if (p->ss == 1 && !p->fp_wide && (best.fp_variant == 3 || best.fp_variant == 4)) {
             // Tile rows change how many blocks share a detector tile's volume
             // data and how the short rays near the detector's edges group;
             // they are timed on the whole plan (every angle and row, at most
