@@ -156,6 +156,18 @@ def test_refined_detector_workspace():
             assert p.workspace_size(_tsx.TSX_OP_BP, batch) == q.workspace_size(_tsx.TSX_OP_BP, batch)
 
 
+def test_refined_forward_bound():
+    """The refined forward path keeps ss^2 times the projections in its
+    workspace: a 512 x 768 detector with 720 angles (1.1 GB of projections)
+    takes it up to ss = 3 (10 GB) and keeps the native supersampled kernel at
+    ss = 4 (18 GB of refined projections, beyond 16 GiB)."""
+    vg = ts.volume(shape=64, size=1)
+    pg = ts.cone(angles=720, shape=(512, 768), size=(np.sqrt(2), 1.5 * np.sqrt(2)), cone_angle=0.5)
+    for ss, refined in ((3, 3), (4, 0)):
+        info = _plan(vg, pg, det_supersampling=ss).info()
+        assert info["fp_refined_detector"] == refined and info["fp_kernel"] == "padded", (ss, info)
+
+
 def test_chunk_limit_override(monkeypatch):
     """TSX_CHUNK_BYTES lowers the bound at plan creation: a 48^3 cone plan
     then takes 3 slabs of 16 (row-paired x layout of a 16-slice slab:

@@ -780,15 +780,21 @@ bool bp_refined(const tsx_plan *p)
 {
     return p->fine && (p->fine->bp_separable || p->fine->perm) && !force_general();
 }
-bool fp_refined(const tsx_plan *p)
-{
-    return p->fine && (p->fine->fp_padded || fp_swapped(p->fine)) && !force_general();
-}
-
 // Elements of one batch entry's projections on the refined detector.
 int64_t refined_sino(const tsx_plan *p)
 {
     return (int64_t)p->fine->rows * p->fine->n_angles * p->fine->cols;
+}
+
+// (the refined forward path holds the refined projections in its workspace:
+// used while they stay within 16 GiB per batch entry, the native supersampled
+// kernels beyond)
+constexpr int64_t REFINED_FP_MAX_BYTES = (int64_t)16 << 30;
+
+bool fp_refined(const tsx_plan *p)
+{
+    return p->fine && (p->fine->fp_padded || fp_swapped(p->fine)) && !force_general() &&
+           refined_sino(p) * 4 <= REFINED_FP_MAX_BYTES;
 }
 
 int64_t workspace_bytes(const tsx_plan *p, int op, int64_t batch)
