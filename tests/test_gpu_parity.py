@@ -722,6 +722,39 @@ def test_autotune_keeps_parity_and_caches_choices():
         assert torch.equal(bi, op.T(yi))
 
 
+@pytest.mark.parametrize("case", ["rolled", "turned_rolled", "turned", "supersampled"])
+def test_autotune_plans_of_every_kind(case):
+    """Autotuning the plans that take the newer paths -- sheared tiles (the
+    shear timed against plain tiles), waves along detector columns, the
+    axis-swapped plan, the refined-detector plan: the tuned plan reports
+    tuned, and its results stay within the oracle tolerance and within 1e-6
+    of the untuned ones."""
+    vg = ts.volume(shape=(40, 48, 44), size=(40 / 48, 1, 44 / 48))
+    base = ts.cone(angles=30, shape=(44, 60), size=(1.4, 1.9), cone_angle=0.5).to_vec()
+    turned = ts.rotate(pos=0, axis=(0, 1, 0), angles=np.pi / 2) * base
+    pg = {"rolled": ts.rotate(pos=0, axis=(0, 1, 1), angles=0.5) * base,
+          "turned_rolled": ts.rotate(pos=0, axis=(0, 1, 1), angles=0.15) * turned,
+          "turned": turned, "supersampled": base}[case]
+    ss = 2 if case == "supersampled" else 1
+    A = ts.operator(vg, pg, detector_supersampling=ss)
+    plan = A.projector.plan(0)
+    x = ts.phantom.ellipsoids(48, count=6, seed=61)[:40, :48, :44].copy()
+    xd = torch.from_numpy(x).to(DEV)
+    y0 = A(xd)
+    b0 = A.T(y0)
+    plan.autotune(torch.cuda.current_stream().cuda_stream)
+    info = plan.info()
+    print(f"[autotune] {case}: {info}", flush=True)
+    assert info["tuned"] is True
+    y1 = A(xd)
+    b1 = A.T(y0)
+    g = oracle.oracle_geometry(vg, pg, det_supersampling=ss)
+    assert rel_l2(y1.cpu().numpy(), oracle.fp(g, x)) <= TOL
+    assert rel_l2(b1.cpu().numpy(), oracle.bp(g, y0.cpu().numpy())) <= TOL
+    assert rel_l2(y1.cpu().numpy(), y0.cpu().numpy()) <= 1e-6
+    assert rel_l2(b1.cpu().numpy(), b0.cpu().numpy()) <= 1e-6
+
+
 @pytest.mark.parametrize("swap", [1, 2])
 def test_swapped_axes_projectors(swap):
     """Detector v parallel to x (swap 1: the scan turned 90 degrees about y)
