@@ -36,6 +36,9 @@
  * those calls of these entry points fail with TSX_ERR_HIP before enqueueing
  * anything (calls numbered from 1 while the variable keeps its value; the
  * count restarts when the value changes) -- for tests of failure handling.
+ * With TSX_TIMING=1 each of these entry points times the device work it
+ * enqueues with HIP events and writes "tsx: <entry point> <ms> ms" to stderr,
+ * waiting for that work (a debugging aid; not while the stream is captured).
  *
  * Launch limits: batches, angle ranges and fused instance lists larger than
  * one kernel launch takes (65535 per grid dimension, 2^32 work-items per

@@ -84,3 +84,26 @@ def test_batches_beyond_one_launch_are_split():
         f = to_autograd(A, num_extra_dims=1)(x)
         for k in np.linspace(0, batch - 1, 7).astype(int):
             assert torch.equal(f[k], A(x[k].contiguous())), k
+
+
+def test_entry_point_timing(monkeypatch, capfd):
+    """TSX_TIMING=1: an entry point times the work it enqueued with HIP events
+    and writes one line per call to stderr; results are unchanged."""
+    import re
+    import tomosipo_amd as ts
+    vg = ts.volume(shape=32, size=1)
+    pg = ts.cone(angles=20, shape=(32, 48), size=(1.4, 2.1), cone_angle=0.5)
+    A = ts.operator(vg, pg)
+    x = torch.rand(A.domain_shape, device="cuda")
+    y0 = A(x)
+    torch.cuda.synchronize()
+    capfd.readouterr()
+    monkeypatch.setenv("TSX_TIMING", "1")
+    y1 = A(x)
+    b = A.T(y1)
+    torch.cuda.synchronize()
+    err = capfd.readouterr().err
+    monkeypatch.delenv("TSX_TIMING")
+    assert re.search(r"tsx: tsx_forward_batched \d+\.\d+ ms", err), err
+    assert re.search(r"tsx: tsx_backward_batched \d+\.\d+ ms", err), err
+    assert torch.equal(y0, y1)

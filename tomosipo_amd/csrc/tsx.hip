@@ -60,6 +60,46 @@ struct TraceRange {
     ~TraceRange() { roctxRangePop(); }
 };
 
+// Timing: with TSX_TIMING=1 an entry point times the device work it enqueues
+// on its stream with HIP events and writes "tsx: <entry point> <ms> ms" to
+// stderr; the call then waits for its work (a debugging aid, off by default;
+// skipped while the stream is being captured into a graph). Failures of the
+// event calls only drop the line.
+struct EntryTimer {
+    const char *name;
+    hipStream_t stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    EntryTimer(const char *n, void *s) : name(n), stream((hipStream_t)s)
+    {
+        const char *e = std::getenv("TSX_TIMING");
+        if (!(e && e[0] == '1')) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone ||
+            hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+            hipEventRecord(e0, stream) != hipSuccess) {
+            release();
+            (void)hipGetLastError();
+        }
+    }
+    void release()
+    {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        e0 = e1 = nullptr;
+    }
+    ~EntryTimer()
+    {
+        if (!e0 || !e1) return;
+        float ms = 0.0f;
+        if (hipEventRecord(e1, stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+            hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+            std::fprintf(stderr, "tsx: %s %.3f ms\n", name, ms);
+        else
+            (void)hipGetLastError();
+        release();
+    }
+};
+
 // Fault injection for tests of failure handling: with TSX_FAIL_LAUNCH set to
 // a comma-separated list of call numbers, those calls of the entry points
 // that enqueue device work fail with TSX_ERR_HIP before enqueueing anything.
@@ -2208,6 +2248,7 @@ int tsx_fdk(const tsx_plan *p, float *vol, const float *proj, int additive, void
     TSX_ENTRY("tsx_fdk");
     DeviceGuard dg(p->device);
     if (!dg.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
+    EntryTimer entry_timer_("tsx_fdk", stream);
     hipStream_t s = (hipStream_t)stream;
     const bool upright = fdk_paired(p);
     const int64_t need = (int64_t)p->rows * p->n_angles * p->cols * (upright ? 8 : 4);
@@ -2283,6 +2324,7 @@ int tsx_forward_batched(const tsx_plan *p, const float *vol, int64_t vs, float *
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
     TSX_ENTRY("tsx_forward_batched");
     DeviceGuard guard(p->device);
+    EntryTimer entry_timer_("tsx_forward_batched", stream);
     return launch_fp(p, vol, vs, proj, ps, batch, plain(additive), (hipStream_t)stream, workspace,
                      workspace_bytes);
 }
@@ -2296,6 +2338,7 @@ int tsx_backward_batched(const tsx_plan *p, float *vol, int64_t vs, const float 
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
     TSX_ENTRY("tsx_backward_batched");
     DeviceGuard guard(p->device);
+    EntryTimer entry_timer_("tsx_backward_batched", stream);
     return launch_bp(p, vol, vs, proj, ps, batch, plain(additive), (hipStream_t)stream, workspace,
                      workspace_bytes);
 }
@@ -2308,6 +2351,7 @@ int tsx_forward_batched_f64(const tsx_plan *p, const double *vol, int64_t vs, do
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
     TSX_ENTRY("tsx_forward_batched_f64");
     DeviceGuard guard(p->device);
+    EntryTimer entry_timer_("tsx_forward_batched_f64", stream);
     return launch_general<double>(p, true, (double *)vol, vs, proj, ps, batch, plain(additive),
                                   (hipStream_t)stream);
 }
@@ -2320,6 +2364,7 @@ int tsx_backward_batched_f64(const tsx_plan *p, double *vol, int64_t vs, const d
     if (!vol || !proj) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
     TSX_ENTRY("tsx_backward_batched_f64");
     DeviceGuard guard(p->device);
+    EntryTimer entry_timer_("tsx_backward_batched_f64", stream);
     return launch_general<double>(p, false, vol, vs, (double *)proj, ps, batch, plain(additive),
                                   (hipStream_t)stream);
 }
@@ -2339,6 +2384,7 @@ int tsx_forward_residual(const tsx_plan *p, const float *vol, int64_t vs, const 
     e.stride = ps;
     TSX_ENTRY("tsx_forward_residual");
     DeviceGuard guard(p->device);
+    EntryTimer entry_timer_("tsx_forward_residual", stream);
     return launch_fp(p, vol, vs, proj, ps, batch, e, (hipStream_t)stream, workspace,
                      workspace_bytes);
 }
@@ -2357,6 +2403,7 @@ int tsx_backward_scaled_add(const tsx_plan *p, float *vol, int64_t vs, const flo
     e.stride = vs;
     TSX_ENTRY("tsx_backward_scaled_add");
     DeviceGuard guard(p->device);
+    EntryTimer entry_timer_("tsx_backward_scaled_add", stream);
     return launch_bp(p, vol, vs, proj, ps, batch, e, (hipStream_t)stream, workspace,
                      workspace_bytes);
 }
@@ -2427,6 +2474,7 @@ int multi_entry(const tsx_plan *const *plans, const void *const *vols, const voi
     if (rc) return rc;
     TSX_ENTRY(op == TSX_OP_FP ? "tsx_multi_forward" : "tsx_multi_backward");
     DeviceGuard guard(plans[0]->device);
+    EntryTimer entry_timer_(op == TSX_OP_FP ? "tsx_multi_forward" : "tsx_multi_backward", stream);
     if (op == TSX_OP_FP)
         return multi_fp(plans, (const float *const *)vols, (float *const *)projs, ep.data(), n, workspace,
                         workspace_bytes, (hipStream_t)stream);
@@ -2551,6 +2599,7 @@ int tsx_direct_fpbp3d_noalloc(const tsx_plan *p, float *vol, float *proj, int mo
     TSX_ENTRY("tsx_direct_fpbp3d_noalloc");
     DeviceGuard guard(p->device);
     if (!guard.ok) return fail(TSX_ERR_HIP, "cannot select the plan's device");
+    EntryTimer entry_timer_("tsx_direct_fpbp3d_noalloc", stream);
     void *ws = nullptr;
     rc = scratch_address(p->device, &ws);
     if (rc) return rc;
