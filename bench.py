@@ -472,6 +472,9 @@ def main():
         if key in t:
             roofline["traffic"] = float(t[key]) * args.instances
             roofline["traffic_unit"] = "bytes per launch"
+            # the kernel's HBM rate at this run's duration (bytes from the counters)
+            roofline["hbm_gbps"] = round(roofline["traffic"] / (dom_ms * 1e-3) / 1e9, 1)
+            roofline["hbm_frac"] = round(roofline["hbm_gbps"] / 8000.0, 4)
             roofline["traffic_source"] = ("read from profiles/hbm_traffic.json: rocprofv3 --pmc "
                                           "FETCH_SIZE + WRITE_SIZE of a one-instance run, copy-"
                                           f"calibrated, times {args.instances} instances; not "
