@@ -4,9 +4,12 @@ Workload (one "step"): 16 independent cone_vec instances, each a 512^3
 float32 unit-cube volume, 720 angles over 2*pi, a 512 x 768 detector of size
 (sqrt 2, 1.5 sqrt 2) through the origin with source distance sqrt(2)/0.5,
 the source shifted linearly in z over +-0.25 (helical-like) plus seeded
-per-angle jitter sigma = 1e-3. One forward projection of every instance, then
-one backprojection of every instance. Inputs are resident in device memory
-before timing starts.
+per-angle jitter sigma = 1e-3. One forward projection of every instance and
+one backprojection of its result. The instances run in 8 groups of fused
+launches: a group's backprojection overlaps the next group's forward
+projection on a second stream (the forward projector is bound by texture
+addressing, the backprojector by vector ALU issue). Inputs are resident in
+device memory before timing starts.
 
 Metric: GUPS = 2 * 512^3 * 720 * 16 / (time of one step) (voxel-angle
 updates per second over FP and BP together).
@@ -347,6 +350,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--instances", type=int, default=16)
+    ap.add_argument("--groups", type=int, default=8,
+                    help="instance groups of the pipelined FP/BP schedule (1: FP launch, then BP)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--serving-requests", type=int, default=1000,
@@ -378,39 +383,48 @@ def main():
         bs.append(torch.empty(A.domain_shape, device=dev))
     torch.cuda.synchronize()
 
-    from tomosipo_amd.fused import fused_forward, fused_backward
+    from tomosipo_amd.fused import fused_normal
 
-    def step(ev=None):
-        # one fused launch for the 16 forward projections, one for the 16 backprojections
-        if ev is not None:
-            ev[0].record()
-        fused_forward(ops, xs, out=ys)
-        if ev is not None:
-            ev[1].record()
-        fused_backward(ops, ys, out=bs)
-        if ev is not None:
-            ev[2].record()
+    groups = max(1, min(args.groups, args.instances))
+
+    def step(events=None, g=groups):
+        # instances in `g` groups: group k's fused forward launch on this stream,
+        # its fused backprojection launch on a side stream, overlapping the
+        # forward launch of group k + 1 (g = 1: one FP launch, then one BP launch)
+        fused_normal(ops, xs, ys, bs, groups=g, events=events)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    evs = [[] for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         step(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    fp_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evs])) / args.instances
-    bp_ms = float(np.mean([e[1].elapsed_time(e[2]) for e in evs])) / args.instances
+    # per-launch durations from events on the stream each launch runs on
+    fp_launch = [e[2].elapsed_time(e[3]) for ev in evs for e in ev]
+    bp_launch = [e[4].elapsed_time(e[5]) for ev in evs for e in ev]
+    per_launch = args.instances / groups
+    fp_ms = float(np.mean(fp_launch)) / per_launch
+    bp_ms = float(np.mean(bp_launch)) / per_launch
     ms_per_step = wall * 1e3 / args.steps
     va = float(N_VOX) ** 3 * N_ANGLES * args.instances
     gups = 2 * va / (ms_per_step * 1e-3) / 1e9
+    # the same work unpipelined (one FP launch, then one BP launch), for comparison
+    seq_ms = None
+    if groups > 1:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        step(g=1)
+        torch.cuda.synchronize()
+        seq_ms = (time.perf_counter() - t0) * 1e3
 
-    # roofline of the dominant kernel; one fused launch covers all instances
-    samples = float(DET[0]) * DET[1] * N_VOX * N_ANGLES * args.instances
-    dom, dom_ms = ("backprojection", bp_ms * args.instances) if bp_ms >= fp_ms else \
-        ("forward", fp_ms * args.instances)
+    # roofline of the dominant kernel: its average launch over `per_launch` instances
+    samples = float(DET[0]) * DET[1] * N_VOX * N_ANGLES * per_launch
+    dom, dom_ms = ("backprojection", bp_ms * per_launch) if bp_ms >= fp_ms else \
+        ("forward", fp_ms * per_launch)
     achieved_tf = samples * FLOP_PER_SAMPLE / (dom_ms * 1e-3) / 1e12
     gather_tbs = samples * GATHER_B_PER_SAMPLE / (dom_ms * 1e-3) / 1e12
     lds_tbs, lds_source = lds_peak()
@@ -425,12 +439,17 @@ def main():
         "peak_nonpacked": FP32_NONPACKED_PEAK_TFLOPS,
         "frac_nonpacked": round(achieved_tf / FP32_NONPACKED_PEAK_TFLOPS, 4),
         "traffic": None,
-        "kernel": f"fused {dom} over {args.instances} instances (incl. its pre-pass)",
+        "kernel": f"fused {dom} launch over {per_launch:g} instances (incl. its pre-pass), "
+                  f"average of {len(bp_launch)} launches, concurrent with the other projector",
         "kernel_ms": round(dom_ms, 3),
+        # both projectors' flops over the step time: the GPU's use while they overlap
+        "step_achieved": round(2 * samples / per_launch * args.instances * FLOP_PER_SAMPLE /
+                               (ms_per_step * 1e-3) / 1e12, 3),
         "basis": ("FP32 vector flops, 14 per bilinear sample, every detector ray x every slice of "
                   "its stepping axis (no MFMA: projection is a gather); peak = published packed-FP32 "
                   "rate, peak_nonpacked = the rate without packed FP32, which this build does not use"),
     }
+    roofline["step_frac"] = round(roofline["step_achieved"] / FP32_PEAK_TFLOPS, 4)
     rec = {
         "metric": "GUPS (voxels x angles / s), FP+BP, 512^3 cone_vec, 720 angles, 512x768 detector",
         "value": round(gups, 2),
@@ -446,10 +465,16 @@ def main():
         "data": "synthetic (seeded box + N(0, 0.01) phantoms, helical cone_vec with seeded jitter)",
         "config": {
             "workload": f"{args.instances} x cone_vec 512^3, 720 angles, 512x768 detector, "
-                        "fused FP launch then fused BP launch",
+                        f"one FP and one BP of each, in {groups} groups of fused launches "
+                        "(a group's BP overlaps the next group's FP on a second stream)",
             "instances": args.instances,
+            "groups": groups,
+            "sequential_ms_per_step": None if seq_ms is None else round(seq_ms, 2),
+            "sequential_gups": None if seq_ms is None else round(2 * va / (seq_ms * 1e-3) / 1e9, 1),
             "fp_ms_per_instance": round(fp_ms, 3),
             "bp_ms_per_instance": round(bp_ms, 3),
+            "launch_times_note": ("per-instance FP and BP times are launch durations while the two "
+                                  "projectors share the GPU; their sum exceeds the step time"),
             "gups_fp": round(N_VOX ** 3 * N_ANGLES / (fp_ms * 1e-3) / 1e9, 2),
             "gups_bp": round(N_VOX ** 3 * N_ANGLES / (bp_ms * 1e-3) / 1e9, 2),
             "kernels": kernel_choices(ops[0]),
@@ -470,14 +495,14 @@ def main():
             t = json.load(f)
         key = "bp_separable_bytes_per_instance" if dom == "backprojection" else "fp_padded_bytes_per_instance"
         if key in t:
-            roofline["traffic"] = float(t[key]) * args.instances
+            roofline["traffic"] = float(t[key]) * per_launch
             roofline["traffic_unit"] = "bytes per launch"
             # the kernel's HBM rate at this run's duration (bytes from the counters)
             roofline["hbm_gbps"] = round(roofline["traffic"] / (dom_ms * 1e-3) / 1e9, 1)
             roofline["hbm_frac"] = round(roofline["hbm_gbps"] / 8000.0, 4)
             roofline["traffic_source"] = ("read from profiles/hbm_traffic.json: rocprofv3 --pmc "
                                           "FETCH_SIZE + WRITE_SIZE of a one-instance run, copy-"
-                                          f"calibrated, times {args.instances} instances; not "
+                                          f"calibrated, times {per_launch:g} instances; not "
                                           "measured in this run")
     progress(f"throughput workload: {gups:.1f} GUPS FP+BP")
     if args.serving_requests > 0:

@@ -335,6 +335,28 @@ def test_fused_calls_check_output_lists_and_shapes():
         fused_forward([A], [xa], epilogues=[Epilogue("residual", y=xa, w=torch.zeros(A.range_shape))])
 
 
+def test_fused_normal_checks_arguments():
+    """fused_normal rejects mismatched lists and wrongly shaped tensors before
+    anything reaches the library or a stream."""
+    import torch
+    from tomosipo_amd.fused import fused_normal
+
+    A = ts.operator(ts.volume(shape=8, size=1), ts.parallel(angles=4, shape=(8, 12)))
+    B = ts.operator(ts.volume(shape=6, size=1), ts.parallel(angles=3, shape=(6, 9)))
+    xa, xb = torch.zeros(A.domain_shape), torch.zeros(B.domain_shape)
+    assert fused_normal([], []) == ([], [])
+    with pytest.raises(ValueError, match="one volume per operator"):
+        fused_normal([A, B], [xa])
+    with pytest.raises(ValueError, match="input 1"):
+        fused_normal([A, B], [xa, xa])
+    with pytest.raises(ValueError, match="one projection and one output"):
+        fused_normal([A, B], [xa, xb], projections=[torch.zeros(A.range_shape)])
+    with pytest.raises(ValueError, match="projection 1"):
+        fused_normal([A, B], [xa, xb], projections=[torch.zeros(A.range_shape)] * 2)
+    with pytest.raises(ValueError, match="output 0"):
+        fused_normal([A, B], [xa, xb], out=[xb, xb])
+
+
 def test_fused_c_abi_rejects_bad_epilogues_and_pointer_lists():
     lib = _tsx.load()
     vg, pg = ts.volume(shape=8, size=1), ts.parallel(angles=4, shape=(8, 12))

@@ -598,6 +598,54 @@ def test_fused_multi_instance_equals_single():
         assert torch.equal(b, A.T(y))
 
 
+def test_fused_normal_pipeline_equals_sequential():
+    """fused_normal over instance groups (each group's backprojection on a side
+    stream, overlapping the next group's forward projection) gives the results
+    of one fused forward call followed by one fused backward call, bitwise, for
+    1, 3 and more groups than instances; work queued on the caller's stream
+    afterwards sees every result; one instance is checked against the oracle."""
+    from tomosipo_amd.fused import fused_forward, fused_backward, fused_normal
+
+    ops, xs, geos = [], [], []
+    for k in range(6):
+        n = 24 + 8 * k
+        vg = ts.volume(shape=(n, n + 2, n - 2), size=1)
+        if k % 2:
+            pg = ts.parallel(angles=20 + k, shape=(n, n + 8), size=(1.2, 1.6))
+        else:
+            pg = _cone_vec_helix(24 + k, n, n + 10, 1.4, 2.0, 0.1, 1e-3, seed=k)
+        ops.append(ts.operator(vg, pg))
+        geos.append((vg, pg))
+        xs.append(torch.rand((n, n + 2, n - 2), device=DEV))
+    vg = ts.volume(shape=20, size=1)
+    ops.append(ts.operator(vg, ts.rotate(pos=0, axis=(0, 1, 1), angles=0.3) *
+                           ts.cone(angles=9, shape=(20, 30), size=(1.4, 2.1), cone_angle=0.5).to_vec()))
+    xs.append(torch.rand(20, 20, 20, device=DEV))
+    ref_y = fused_forward(ops, xs)
+    ref_b = fused_backward(ops, ref_y)
+    for groups in (1, 3, 20):
+        print(f"[fused_normal] groups {groups}", flush=True)
+        events = []
+        ys, bs = fused_normal(ops, xs, groups=groups, events=events)
+        assert len(events) == min(groups, len(ops))
+        assert [e[0] for e in events][0] == 0 and events[-1][1] == len(ops) - 1
+        sums = torch.stack([b.sum() for b in bs])      # queued on the caller's stream
+        for k in range(len(ops)):
+            assert torch.equal(ys[k], ref_y[k]), (groups, k)
+            assert torch.equal(bs[k], ref_b[k]), (groups, k)
+        assert torch.equal(sums, torch.stack([b.sum() for b in ref_b]))
+    # outputs given by the caller are filled in place
+    ys = [torch.full(A.range_shape, 7.0, device=DEV) for A in ops]
+    bs = [torch.full(A.domain_shape, 7.0, device=DEV) for A in ops]
+    fused_normal(ops, xs, ys, bs, groups=4)
+    for k in range(len(ops)):
+        assert torch.equal(ys[k], ref_y[k]) and torch.equal(bs[k], ref_b[k])
+    g = oracle.oracle_geometry(*geos[2])
+    y64 = oracle.fp(g, xs[2].double().cpu().numpy())
+    assert rel_l2(ys[2].cpu().numpy(), y64) <= TOL
+    assert rel_l2(bs[2].cpu().numpy(), oracle.bp(g, ys[2].double().cpu().numpy())) <= TOL
+
+
 def test_fused_more_instances_than_one_launch_takes():
     """20 instances of assorted shapes: more than one fused launch holds (16).
     Two of them are 128^3 scans with 200 angles (9600 forward blocks each), so
