@@ -195,12 +195,24 @@ class Server:
 
     ``operators``: a dict of operators by request_key shared with other
     servers (e.g. filled by warm_plans), so that a long-running server keeps
-    the plans of the shapes it has seen; None gives the server its own."""
+    the plans of the shapes it has seen; None gives the server its own.
 
-    def __init__(self, hbm_budget_bytes=16 << 30, device=0, max_active=64, keep_results=False, operators=None):
+    ``overlap`` (default): the server works in phases; a phase takes every
+    active request's next step, the forward steps in one fused launch on the
+    device's current stream and the backprojection steps in one fused launch on
+    a side stream, running at the same time (the forward projector is bound by
+    texture addressing, the backprojector by vector ALU issue), and the next
+    phase starts when both have finished, so a SIRT request advances one step
+    per phase. ``overlap=False``: a tick runs one fused forward launch and then
+    one fused backprojection launch on the current stream that also takes the
+    SIRT requests whose forward step just ran (a SIRT iteration per tick)."""
+
+    def __init__(self, hbm_budget_bytes=16 << 30, device=0, max_active=64, keep_results=False, operators=None,
+                 overlap=True):
         import torch
 
         self.torch = torch
+        self.overlap = overlap
         self.dev = torch.device("cuda", device)
         self.budget = int(hbm_budget_bytes)
         self.max_active = max_active
@@ -260,6 +272,10 @@ class Server:
         used = 0
         deferred = 0
         with torch.cuda.device(self.dev):
+            from .fused import _side_stream
+
+            main = torch.cuda.current_stream(self.dev)
+            side = _side_stream(self.dev) if self.overlap else None
             t0 = time.perf_counter()
             while pending or active or finishing:
                 now = time.perf_counter() - t0
@@ -304,18 +320,37 @@ class Server:
                 # arrivals join the next launch instead of queueing behind many
                 while len(ticks) >= 2:
                     ticks.popleft().synchronize()
-                # one tick: one fused forward launch, then one fused
-                # backprojection launch that also takes the SIRT requests
-                # whose forward step just ran (a SIRT iteration per tick)
                 failed = []
                 fps = [j for j in active if j.steps[0] == "fp"]
-                if fps:
-                    for j in self._launch(fps, True, records, failed):
-                        j.steps.popleft()
-                bps = [j for j in active if j.steps and j.steps[0] == "bp"]
-                if bps:
-                    for j in self._launch(bps, False, records, failed):
-                        j.steps.popleft()
+                if self.overlap:
+                    # one phase: the forward steps here, the backprojection steps
+                    # on the side stream after everything queued before the phase
+                    # (their inputs come from earlier phases or admission)
+                    bps = [j for j in active if j.steps[0] == "bp"]
+                    start = torch.cuda.Event()
+                    start.record(main)
+                    if fps:
+                        for j in self._launch(fps, True, records, failed):
+                            j.steps.popleft()
+                    if bps:
+                        with torch.cuda.stream(side):
+                            side.wait_event(start)
+                            for j in self._launch(bps, False, records, failed):
+                                j.steps.popleft()
+                            bp_done = torch.cuda.Event()
+                            bp_done.record(side)
+                        main.wait_event(bp_done)
+                else:
+                    # one tick: one fused forward launch, then one fused
+                    # backprojection launch that also takes the SIRT requests
+                    # whose forward step just ran (a SIRT iteration per tick)
+                    if fps:
+                        for j in self._launch(fps, True, records, failed):
+                            j.steps.popleft()
+                    bps = [j for j in active if j.steps and j.steps[0] == "bp"]
+                    if bps:
+                        for j in self._launch(bps, False, records, failed):
+                            j.steps.popleft()
                 if failed:
                     torch.cuda.synchronize()
                     t = time.perf_counter() - t0
