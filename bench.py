@@ -442,6 +442,9 @@ def main():
         "kernel": f"fused {dom} launch over {per_launch:g} instances (incl. its pre-pass), "
                   f"average of {len(bp_launch)} launches, concurrent with the other projector",
         "kernel_ms": round(dom_ms, 3),
+        "note": ("kernel_ms is the launch's duration while the other projector's launch runs "
+                 "beside it on a second stream; step_achieved / step_frac count both projectors' "
+                 "flops over the step time"),
         # both projectors' flops over the step time: the GPU's use while they overlap
         "step_achieved": round(2 * samples / per_launch * args.instances * FLOP_PER_SAMPLE /
                                (ms_per_step * 1e-3) / 1e12, 3),

@@ -50,9 +50,13 @@ def _check_mix(reqs, mixed, sirt_tol=1e-5):
             assert torch.equal(a, b), f"request {r.rid} ({r.op}, {r.kind}, {r.side}^3)"
 
 
-def test_results_identical_alone_and_in_the_mix():
+@pytest.mark.parametrize("overlap", [True, False])
+def test_results_identical_alone_and_in_the_mix(overlap):
+    """Phases with the forward and backprojection steps on two streams
+    (overlap) and ticks of one forward then one backprojection launch give
+    the same results."""
     reqs = _stream()
-    mixed = serving.Server(keep_results=True).run(reqs)
+    mixed = serving.Server(keep_results=True, overlap=overlap).run(reqs)
     _check_mix(reqs, mixed)
     print(f"[serving] mixed {mixed['req_per_s']:.1f} req/s", flush=True)
 
