@@ -177,7 +177,7 @@ int tsx_plan_info_get(const tsx_plan *plan, tsx_plan_info *info);
  * `stream`. Must not run while launches of the same plan are being issued
  * from another thread. The TSX_FP_VARIANT, TSX_FP_TILE_ROWS, TSX_FP_SHEAR
  * (0 or 1) and TSX_BP_BRICK_Z environment variables, when set, override any
- * choice. Plans held by the plan (axis-swapped, refined detector) are tuned
+ * choice (TSX_BP_PARTS=1: no angle parts in the backprojector). Plans held by the plan (axis-swapped, refined detector) are tuned
  * too. */
 int tsx_plan_autotune(tsx_plan *plan, void *stream);
 
@@ -217,8 +217,9 @@ int tsx_backward(const tsx_plan *plan, float *vol, const float *proj, int additi
 
 /* Device scratch bytes that one launch of `op` over `batch` entries needs
  * (padded volume copies for the padded FP kernels, length-weighted
- * projections for BP without detector supersampling; 0 for the general FP
- * kernel and for supersampled BP). */
+ * projections for BP without detector supersampling, plus, for volumes of
+ * few bricks, the sums of the backprojector's angle parts; 0 for the general
+ * FP kernel and for supersampled BP). */
 int tsx_workspace_size(const tsx_plan *plan, int op, int64_t batch, int64_t *bytes);
 
 /* Batched variants: `batch` independent volumes / projection sets with the

@@ -355,6 +355,8 @@ def test_fused_normal_checks_arguments():
         fused_normal([A, B], [xa, xb], projections=[torch.zeros(A.range_shape)] * 2)
     with pytest.raises(ValueError, match="output 0"):
         fused_normal([A, B], [xa, xb], out=[xb, xb])
+    with pytest.raises(ValueError, match="one HIP device"):
+        fused_normal([A, B], [xa, xb])
 
 
 def test_fused_c_abi_rejects_bad_epilogues_and_pointer_lists():

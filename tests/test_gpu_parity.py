@@ -9,6 +9,7 @@ import pytest
 
 import oracle
 import tomosipo_amd as ts
+from tomosipo_amd import _tsx
 from tests.conftest import rel_l2
 
 torch = pytest.importorskip("torch")
@@ -568,6 +569,64 @@ def test_batched_gradient_matches_oracle():
     for k in range(8):
         ref = 2 * oracle.bp(g, oracle.fp(g, xs[k]) - ys[k])
         assert rel_l2(x.grad[k].cpu().numpy(), ref) <= 1e-5
+
+
+def test_backprojector_angle_parts(monkeypatch):
+    """Volumes of few bricks (64^3: 32) sum the angles of the separable
+    backprojector in parts of consecutive angles: a single launch runs the
+    parts in separate blocks and adds their sums in order, a fused launch runs
+    them in turn in one block. Single, batched and fused launches give the
+    same sums bitwise (plain, additive and SIRT scaled-add epilogues); they
+    agree with one part (TSX_BP_PARTS=1) to FP32 rounding and with the
+    oracle."""
+    from tomosipo_amd.fused import fused_backward, Epilogue
+
+    vg1, pg1 = ts.volume(shape=64, size=1), ts.parallel(angles=96, shape=(64, 96), size=(1, 1.5)).to_vec()
+    vg2 = ts.volume(shape=(40, 48, 44), size=1)
+    pg2 = _cone_vec_helix(60, 40, 60, 1.4, 2.0, 0.1, 1e-3, seed=3)
+    A, B = ts.operator(vg1, pg1), ts.operator(vg2, pg2)
+    plan = A.projector.plan(0)
+    ve = 64 ** 3
+    # the workspace holds the part sums (12 parts of 8 angles)
+    assert plan.workspace_size(_tsx.TSX_OP_BP) >= 12 * ve * 4
+    g = torch.Generator(device=DEV)
+    g.manual_seed(5)
+    ya = torch.rand(A.range_shape, generator=g, device=DEV)
+    yb = torch.rand(B.range_shape, generator=g, device=DEV)
+    ba, bb = A.T(ya), B.T(yb)
+    fa, fb = fused_backward([A, B], [ya, yb])
+    assert torch.equal(fa, ba) and torch.equal(fb, bb)
+    # batched: three projection sets of A in one call
+    ys = torch.stack([ya, ya * 2, ya + 1])
+    out = torch.empty((3,) + A.domain_shape, device=DEV)
+    plan.backward(out.data_ptr(), ys.data_ptr(), False, torch.cuda.current_stream().cuda_stream, batch=3,
+                  vol_stride=ve, proj_stride=ya.numel())
+    for k in range(3):
+        assert torch.equal(out[k], A.T(ys[k])), k
+    # additive and SIRT scaled add through the reduction, single and fused
+    x0 = torch.rand(A.domain_shape, generator=g, device=DEV)
+    c = torch.rand(A.domain_shape, generator=g, device=DEV)
+    add_s, add_f = x0.clone(), x0.clone()
+    plan.backward(add_s.data_ptr(), ya.data_ptr(), True, torch.cuda.current_stream().cuda_stream)
+    fused_backward([A], [ya], out=[add_f], additive=True)
+    assert torch.equal(add_s, add_f) and torch.equal(add_s, x0 + ba)
+    sa_s, sa_f = x0.clone(), x0.clone()
+    plan.backward_scaled_add(sa_s.data_ptr(), ya.data_ptr(), c.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    fused_backward([A], [ya], out=[sa_f], epilogues=[Epilogue("scaled_add", w=c)])
+    assert torch.equal(sa_s, sa_f)
+    assert rel_l2(sa_s.cpu().numpy(), (x0 + c * ba).cpu().numpy()) <= 1e-6
+    # one part: the same sums to FP32 rounding
+    monkeypatch.setenv("TSX_BP_PARTS", "1")
+    assert plan.workspace_size(_tsx.TSX_OP_BP) < 12 * ve * 4
+    ba1, bb1 = A.T(ya), B.T(yb)
+    monkeypatch.delenv("TSX_BP_PARTS")
+    print(f"[parts] one part vs parts: {rel_l2(ba.cpu().numpy(), ba1.cpu().numpy()):.2e}, "
+          f"{rel_l2(bb.cpu().numpy(), bb1.cpu().numpy()):.2e}", flush=True)
+    assert rel_l2(ba.cpu().numpy(), ba1.cpu().numpy()) <= 1e-6
+    assert rel_l2(bb.cpu().numpy(), bb1.cpu().numpy()) <= 1e-6
+    for vg, pg, y, b in ((vg1, pg1, ya, ba), (vg2, pg2, yb, bb)):
+        geo = oracle.oracle_geometry(vg, pg)
+        assert rel_l2(b.cpu().numpy(), oracle.bp(geo, y.double().cpu().numpy())) <= TOL
 
 
 def test_fused_multi_instance_equals_single():

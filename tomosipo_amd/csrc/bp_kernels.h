@@ -443,6 +443,13 @@ struct BpView {
     const float *ly;    // length-weighted projections, batch entries ly_bytes apart
     uint32_t ly_bytes;
     Epilogue ep;
+    // angle parts: the sums over the angles are taken in parts of part_len
+    // consecutive angles, added in order. partial != nullptr: a block per
+    // (brick, entry, part) writes its part's sums to partial (entry e, part s
+    // at (e * parts + s) * nx * ny * nz) for bp_parts_reduce; otherwise one
+    // block per (brick, entry) runs the parts in turn (kernels with PARTS).
+    int parts, part_len;
+    float *partial;
 };
 
 // LDS layout of the separable backprojector: columns are UCOL = BZ + 2
@@ -806,7 +813,7 @@ __device__ __forceinline__ void bp_wave_decode(uint32_t w, int &jlo, int &cnt, i
     chunks = __builtin_amdgcn_readfirstlane(ch);
 }
 
-template <int BZ>
+template <int BZ, bool PARTS>
 __device__ __forceinline__ void bp_wave_work(const BpView &v, int logical, BpShared<BZ> &sh)
 {
     using S = BpShared<BZ>;
@@ -814,7 +821,14 @@ __device__ __forceinline__ void bp_wave_work(const BpView &v, int logical, BpSha
     const int bx = logical % v.bricks_x;
     const int by = (logical / v.bricks_x) % v.bricks_y;
     const int bz = (logical / (v.bricks_x * v.bricks_y)) % v.bricks_z;
-    const int b = logical / (v.bricks_x * v.bricks_y * v.bricks_z);
+    int b = logical / (v.bricks_x * v.bricks_y * v.bricks_z);
+    int ka = 0, kb = n_angles, part = 0;  // this block's angles
+    if (v.partial) {
+        part = b % v.parts;
+        b /= v.parts;
+        ka = part * v.part_len;
+        kb = min(n_angles, ka + v.part_len);
+    }
     if (b >= v.batch) return;
     // angle parameters through a constant-address-space pointer: the uniform
     // ones become scalar loads
@@ -826,21 +840,36 @@ __device__ __forceinline__ void bp_wave_work(const BpView &v, int logical, BpSha
         (void *)uniform_ptr(v.ly + (int64_t)b * ((int64_t)v.ly_bytes / 4)), (short)0,
         (int)__builtin_amdgcn_readfirstlane(v.ly_bytes), 0x00020000);
     float acc[BZ];
+    float tot[PARTS ? BZ : 1];  // sums of the finished parts
 #pragma unroll
     for (int k = 0; k < BZ; ++k) acc[k] = 0.0f;
-    int mode = angles[0].m;  // stepping axis whose voxel mapping acc follows
+    if (PARTS) {
+#pragma unroll
+        for (int k = 0; k < BZ; ++k) tot[PARTS ? k : 0] = 0.0f;
+    }
+    int mode = angles[ka].m;  // stepping axis whose voxel mapping acc follows
     // slice ranges from the table, read one angle ahead
-    uint32_t next = bp_wave_entry(v, 0, mode == 0, bx, by, ty);
-    for (int k = 0; k < n_angles; ++k) {
+    uint32_t next = bp_wave_entry(v, ka, mode == 0, bx, by, ty);
+    for (int k = ka; k < kb; ++k) {
         ConstAngle &A = angles[k];
         const int am = A.m;
+        if (PARTS && k > ka && k % v.part_len == 0) {
+            // a part ends: its sums (in the final voxel mapping) join the total
+            if (mode == 0) bp_wave_transpose<BZ>(acc, sh.u, tx, ty);
+#pragma unroll
+            for (int q = 0; q < BZ; ++q) {
+                tot[PARTS ? q : 0] += acc[q];
+                acc[q] = 0.0f;
+            }
+            mode = am;
+        }
         if (am != mode) {
             bp_wave_transpose<BZ>(acc, sh.u, tx, ty);
             mode = am;
         }
         int cur_jlo, cur_cnt, cur_inc, cur_chunks;
         bp_wave_decode(next, cur_jlo, cur_cnt, cur_inc, cur_chunks);
-        if (k + 1 < n_angles) next = bp_wave_entry(v, k + 1, angles[k + 1].m == 0, bx, by, ty);
+        if (k + 1 < kb) next = bp_wave_entry(v, k + 1, angles[k + 1].m == 0, bx, by, ty);
         const bool mx = am == 0;
         const int m0 = mx ? bx * BR : by * BR;
         const int c0 = mx ? by * BR : bx * BR;
@@ -887,16 +916,44 @@ __device__ __forceinline__ void bp_wave_work(const BpView &v, int logical, BpSha
         }
     }
     if (mode == 0) bp_wave_transpose<BZ>(acc, sh.u, tx, ty);
+    if (PARTS) {
+#pragma unroll
+        for (int q = 0; q < BZ; ++q) acc[q] = tot[PARTS ? q : 0] + acc[q];
+    }
     // thread (tx, ty) now holds voxel column (x, y) = (bx * BR + tx, by * BR + ty)
     const int x = bx * BR + tx, y = by * BR + ty;
     if (x < v.nx && y < v.ny) {
-        float *out = v.vol + (int64_t)b * v.vol_stride;
+        if (v.partial) {
+            const int64_t ve = (int64_t)v.nx * v.ny * v.nz;
+            float *out = v.partial + ((int64_t)b * v.parts + part) * ve;
 #pragma unroll
-        for (int q = 0; q < BZ; ++q) {
-            const int z = z0 + q;
-            if (z < v.nz) emit(out, ((int64_t)z * v.ny + y) * v.nx + x, b, acc[q], v.ep);
+            for (int q = 0; q < BZ; ++q) {
+                const int z = z0 + q;
+                if (z < v.nz) out[((int64_t)z * v.ny + y) * v.nx + x] = acc[q];
+            }
+        } else {
+            float *out = v.vol + (int64_t)b * v.vol_stride;
+#pragma unroll
+            for (int q = 0; q < BZ; ++q) {
+                const int z = z0 + q;
+                if (z < v.nz) emit(out, ((int64_t)z * v.ny + y) * v.nx + x, b, acc[q], v.ep);
+            }
         }
     }
+}
+
+// The sums of the angle parts of bp_wave (partial, parts per entry) added in
+// order and written through the epilogue: grid (voxels / 256, entries).
+__global__ void __launch_bounds__(256)
+bp_parts_reduce(const float *__restrict__ partial, int parts, int64_t ve, float *vol, int64_t vs, Epilogue ep)
+{
+    const int64_t o = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y;
+    if (o >= ve) return;
+    const float *pp = partial + (int64_t)b * parts * ve + o;
+    float t = pp[0];
+    for (int s = 1; s < parts; ++s) t += pp[(int64_t)s * ve];
+    emit(vol + (int64_t)b * vs, o, b, t, ep);
 }
 
 // Waves per SIMD the register allocation is held to: 4 (16 per CU, four
@@ -910,7 +967,7 @@ __global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(bp_
     __shared__ __align__(16) BpShared<BZ> sh;
     const int logical = xcd_logical(shift);
     if (logical >= n_blocks) return;
-    bp_wave_work<BZ>(v, logical, sh);
+    bp_wave_work<BZ, false>(v, logical, sh);
 }
 
 
@@ -924,14 +981,17 @@ struct BpMulti {
     int shift;  // chunk size of the XCD order (xcd_logical)
 };
 
-template <int BZ>
-__global__ void __launch_bounds__(BR *BR) __attribute__((amdgpu_waves_per_eu(bp_waves_per_eu(BZ)))) bp_wave_multi(BpMulti m)
+// PARTS: instances whose sums are taken in angle parts (BpView::parts > 1),
+// run in turn by each block.
+template <int BZ, bool PARTS>
+__global__ void __launch_bounds__(BR *BR)
+__attribute__((amdgpu_waves_per_eu(PARTS ? 1 : bp_waves_per_eu(BZ)))) bp_wave_multi(BpMulti m)
 {
     __shared__ __align__(16) BpShared<BZ> sh;
     const int logical = xcd_logical(m.shift);
     if (logical >= m.begin[m.n]) return;
     const int k = owner_of(m.begin, logical);
-    bp_wave_work<BZ>(m.v[k], logical - m.begin[k], sh);
+    bp_wave_work<BZ, PARTS>(m.v[k], logical - m.begin[k], sh);
 }
 
 }  // namespace tsx

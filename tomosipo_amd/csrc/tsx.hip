@@ -785,6 +785,32 @@ int64_t fp_workspace_elems(const tsx_plan *p)
 // Sorted angles per launch of the separable backprojector.
 int bp_chunk_of(const tsx_plan *p) { return std::max(1, std::min(p->bp_chunk, p->n_angles)); }
 
+int bp_brick_z_of(const tsx_plan *p);
+
+// Angle parts of the separable backprojector. A block owns a brick and loops
+// over the angles, so a volume of few bricks (64^3: 32) leaves most CUs idle;
+// its angles are then split into parts of part_len >= 8 consecutive sorted
+// angles, so that bricks x parts reaches about 512 blocks. Single launches run
+// the parts in separate blocks and add their sums in order (bp_parts_reduce);
+// fused launches run them in turn in one block and add them in the same order,
+// so both give the same sums. Volumes of 256 bricks or more, and plans whose
+// angles take several launches, keep one part. TSX_BP_PARTS=1 keeps one part
+// everywhere.
+int bp_part_len_of(const tsx_plan *p)
+{
+    if (!p->bp_separable || bp_chunk_of(p) < p->n_angles || env_int("TSX_BP_PARTS") == 1) return p->n_angles;
+    const int bz = bp_brick_z_of(p);
+    const int64_t bricks = (int64_t)((p->nx + BR - 1) / BR) * ((p->ny + BR - 1) / BR) * ((p->nz + bz - 1) / bz);
+    if (bricks >= 256) return p->n_angles;
+    const int64_t len = std::max<int64_t>(8, ((int64_t)p->n_angles * bricks + 511) / 512);
+    return (int)std::min<int64_t>(len, p->n_angles);
+}
+int bp_parts_of(const tsx_plan *p)
+{
+    const int len = bp_part_len_of(p);
+    return (p->n_angles + len - 1) / len;
+}
+
 // Length-weighted projections of one batch entry: transposed and padded
 // ([sorted angle][col][RPAD + row], scale_by_length_t) for the separable
 // kernel, for the angles of one launch; in the projections' own layout for
@@ -850,7 +876,13 @@ int64_t workspace_bytes(const tsx_plan *p, int op, int64_t batch)
     if (op == TSX_OP_BP && bp_refined(p)) return workspace_bytes(p->fine, op, batch);
     if (p->perm)
         return align_ws((int64_t)p->nx * p->ny * p->nz * batch * 4) + workspace_bytes(p->perm, op, batch);
-    return p->bp_separable || bp_general_prescaled(p, batch) ? bp_workspace_elems(p) * batch * 4 : 0;
+    if (p->bp_separable) {
+        // transposed projections, then the sums of the angle parts
+        const int parts = bp_parts_of(p);
+        const int64_t t = bp_workspace_elems(p) * batch * 4;
+        return parts > 1 ? align_ws(t) + (int64_t)parts * p->nx * p->ny * p->nz * batch * 4 : t;
+    }
+    return bp_general_prescaled(p, batch) ? bp_workspace_elems(p) * batch * 4 : 0;
 }
 
 // Launches of one block per logical id take the chunked XCD order of
@@ -1290,6 +1322,9 @@ BpView bp_view(const tsx_plan *p, int BZ, float *vol, int64_t vs, const float *l
     v.ly = ly;
     v.ly_bytes = ly_bytes;
     v.ep = ep;
+    v.parts = 1;
+    v.part_len = kn;
+    v.partial = nullptr;
     return v;
 }
 
@@ -1363,15 +1398,18 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
     // sums added to the volume of the previous ones)
     const int kc = bp_chunk_of(p);
     const int64_t sino_t = bp_workspace_elems(p);
+    const int parts = bp_parts_of(p);  // (> 1 only where kc covers every angle)
+    const int64_t ve = (int64_t)p->nx * p->ny * p->nz;
     Scratch owned;
     void *wptr = nullptr;
-    int rc = get_workspace(p, sino_t * batch * 4, wsp, ws_bytes, stream, owned, &wptr);
+    int rc = get_workspace(p, workspace_bytes(p, TSX_OP_BP, batch), wsp, ws_bytes, stream, owned, &wptr);
     if (rc) return rc;
     float *ly = (float *)wptr;
+    float *partial = parts > 1 ? (float *)((char *)wptr + align_ws(sino_t * batch * 4)) : nullptr;
     const int BZ = bp_brick_z_of(p);
     const int64_t bricks = (int64_t)((p->nx + BR - 1) / BR) * ((p->ny + BR - 1) / BR) * ((p->nz + BZ - 1) / BZ);
-    if (bricks > max_blocks(BR * BR)) return fail(TSX_ERR_UNSUPPORTED, "volume too large for one launch");
-    const int64_t bc = max_blocks(BR * BR) / bricks;  // entries per launch
+    if (bricks * parts > max_blocks(BR * BR)) return fail(TSX_ERR_UNSUPPORTED, "volume too large for one launch");
+    const int64_t bc = std::min<int64_t>(max_blocks(BR * BR) / (bricks * parts), 65535);  // entries per launch
     for (int k0 = 0; k0 < p->n_angles; k0 += kc) {
         const int kn = std::min(kc, p->n_angles - k0);
         const Epilogue e = k0 == 0 ? ep : later_chunk(ep);
@@ -1380,7 +1418,12 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
             const int64_t nbat = std::min(bc, batch - b0);
             BpView v = bp_view(p, BZ, vol + b0 * vs, vs, ly + b0 * sino_t, (uint32_t)(sino_t * 4), (int)nbat,
                                ep_from(e, b0), k0, kn);
-            const int n_blocks = (int)(bricks * nbat);
+            if (parts > 1) {
+                v.parts = parts;
+                v.part_len = bp_part_len_of(p);
+                v.partial = partial + b0 * parts * ve;
+            }
+            const int n_blocks = (int)(bricks * parts * nbat);
             int shift;
             const unsigned grid = xcd_grid(n_blocks, &shift);
             if (BZ == 32)
@@ -1390,6 +1433,11 @@ int launch_bp(const tsx_plan *p, float *vol, int64_t vs, const float *proj, int6
             else
                 hipLaunchKernelGGL(bp_wave<16>, dim3(grid), dim3(BR, BR), 0, stream, v, n_blocks, shift);
             TSX_HIP_CHECK(hipGetLastError());
+            if (parts > 1) {
+                hipLaunchKernelGGL(bp_parts_reduce, dim3((unsigned)((ve + 255) / 256), (unsigned)nbat), dim3(256), 0,
+                                   stream, v.partial, parts, ve, vol + b0 * vs, vs, ep_from(e, b0));
+                TSX_HIP_CHECK(hipGetLastError());
+            }
         }
     }
     return TSX_OK;
@@ -1540,6 +1588,7 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
     }
     const bool fast = !force_general();
     int BZ = 32;
+    bool PM = false;  // the group's instances sum in angle parts
     BpMulti m;
     m.n = 0;
     m.begin[0] = 0;
@@ -1554,12 +1603,10 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
                            sj);
         sj_max = 0;
         const unsigned grid = xcd_grid(m.begin[m.n], &m.shift);
-        if (BZ == 32)
-            hipLaunchKernelGGL(bp_wave_multi<32>, dim3(grid), dim3(BR, BR), 0, stream, m);
-        else if (BZ == 24)
-            hipLaunchKernelGGL(bp_wave_multi<24>, dim3(grid), dim3(BR, BR), 0, stream, m);
-        else
-            hipLaunchKernelGGL(bp_wave_multi<16>, dim3(grid), dim3(BR, BR), 0, stream, m);
+        auto kern = BZ == 32 ? (PM ? bp_wave_multi<32, true> : bp_wave_multi<32, false>)
+                  : BZ == 24 ? (PM ? bp_wave_multi<24, true> : bp_wave_multi<24, false>)
+                             : (PM ? bp_wave_multi<16, true> : bp_wave_multi<16, false>);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(BR, BR), 0, stream, m);
         TSX_HIP_CHECK(hipGetLastError());
         m.n = 0;
         return TSX_OK;
@@ -1586,12 +1633,14 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
             woff += wb;
             continue;
         }
-        if (m.n > 0 && (bz != BZ || (int64_t)m.begin[m.n] + blocks > cap ||
+        const int parts = bp_parts_of(p);
+        if (m.n > 0 && (bz != BZ || (parts > 1) != PM || (int64_t)m.begin[m.n] + blocks > cap ||
                         std::max<int64_t>(sj_max, tblocks) * (m.n + 1) > max_blocks(256))) {
             int rc = flush();
             if (rc) return rc;
         }
         BZ = bz;
+        PM = parts > 1;
         float *ly = (float *)((char *)wptr + woff);
         woff += wb;
         const int64_t sino_t = bp_workspace_elems(p);
@@ -1600,6 +1649,10 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         sj.job[m.n] = job;
         sj_max = std::max(sj_max, (int)tblocks);
         m.v[m.n] = bp_view(p, BZ, vols[k], 0, ly, (uint32_t)(sino_t * 4), 1, eps[k], 0, p->n_angles);
+        if (parts > 1) {
+            m.v[m.n].parts = parts;
+            m.v[m.n].part_len = bp_part_len_of(p);
+        }
         m.begin[m.n + 1] = m.begin[m.n] + (int)blocks;
         m.n += 1;
         if (m.n == MAX_FUSED) {
