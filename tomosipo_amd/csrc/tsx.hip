@@ -1480,8 +1480,8 @@ int64_t multi_workspace_bytes(const tsx_plan *const *plans, int n, int op)
     return total;
 }
 
-// Fused launches: consecutive instances on the fast kernels with the same
-// kernel variant share one launch (up to MAX_FUSED instances, as many blocks
+// Fused launches: instances on the fast kernels with the same kernel choice
+// share one launch (up to MAX_FUSED instances, as many blocks
 // as the dispatch limit takes), their padded layouts (forward: one launch of
 // transposes, one of copies) or transposed projections (backward: one
 // launch) built before it; an instance on the general kernels, with
@@ -1503,7 +1503,6 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
     m.n = 0;
     m.begin[0] = 0;
     LayoutBatch layouts(stream);
-    int64_t woff = 0;
     int variant = -1, by = 8;
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
@@ -1515,10 +1514,23 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         m.n = 0;
         return TSX_OK;
     };
+    // each instance's workspace slice; the fused instances are taken by kernel
+    // choice (march variant, tile rows), so that instances of one choice share
+    // launches however they are ordered (every instance writes only its own
+    // projections, so the order of the launches does not change any result)
+    std::vector<int64_t> off(n + 1, 0);
+    std::vector<std::pair<int, int>> keys;
+    for (int k = 0; k < n; ++k) {
+        if (!vols[k] || !projs[k]) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+        off[k + 1] = off[k] + align_up(workspace_bytes(plans[k], TSX_OP_FP, 1));
+        const std::pair<int, int> key(fp_variant_of(plans[k]), fp_tile_rows_of(plans[k]));
+        if (std::find(keys.begin(), keys.end(), key) == keys.end()) keys.push_back(key);
+    }
+    for (size_t pass = 0; pass < keys.size(); ++pass)
     for (int k = 0; k < n; ++k) {
         const tsx_plan *p = plans[k];
-        if (!vols[k] || !projs[k]) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
-        const int64_t wb = align_up(workspace_bytes(p, TSX_OP_FP, 1));
+        const int64_t wb = off[k + 1] - off[k];
+        const int64_t woff = off[k];
         const int pby = fp_tile_rows_of(p);
         const int along = p->fp_column_waves ? p->rows : p->cols, across = p->fp_column_waves ? p->cols : p->rows;
         const int tiles_u = (along + FP_BX - 1) / FP_BX;
@@ -1527,12 +1539,13 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         const int64_t cap = max_blocks(FP_BX * pby);
         if (!(fast && p->fp_padded && p->ss == 1 && p->slabs.empty() && !fp_swapped(p) && !p->fp_wide) ||
             blocks > cap) {
-            // this instance takes its own launch(es)
+            // this instance takes its own launch(es), in the first pass
+            if (pass > 0) continue;
             int rc = launch_fp(p, vols[k], 0, projs[k], 0, 1, eps[k], stream, wptr ? (char *)wptr + woff : nullptr, wb);
             if (rc) return rc;
-            woff += wb;
             continue;
         }
+        if (fp_variant_of(p) != keys[pass].first || pby != keys[pass].second) continue;
         if (m.n > 0 && (fp_variant_of(p) != variant || pby != by || (int64_t)m.begin[m.n] + blocks > cap)) {
             int rc = flush();
             if (rc) return rc;
@@ -1543,7 +1556,6 @@ int multi_fp(const tsx_plan *const *plans, const float *const *vols, float *cons
         int64_t offs[3], per_entry;
         layout_offsets(p, es, offs, &per_entry);
         float *base = (float *)((char *)wptr + woff);
-        woff += wb;
         layouts.add(p, es, vols[k], base, offs);
         FpView &v = m.v[m.n];
         v.angles = p->dev;
@@ -1594,7 +1606,6 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
     m.begin[0] = 0;
     ScaleJobs sj;
     int sj_max = 0;  // blocks of the largest job in sj
-    int64_t woff = 0;
     auto flush = [&]() -> int {
         if (m.n == 0) return TSX_OK;
         sj.n = m.n;
@@ -1611,10 +1622,23 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         m.n = 0;
         return TSX_OK;
     };
+    // each instance's workspace slice; the fused instances are taken by kernel
+    // choice (brick depth, angle parts), so that instances of one choice share
+    // launches however they are ordered (every instance writes only its own
+    // volume, so the order of the launches does not change any result)
+    std::vector<int64_t> off(n + 1, 0);
+    std::vector<std::pair<int, bool>> keys;
+    for (int k = 0; k < n; ++k) {
+        if (!vols[k] || !projs[k]) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
+        off[k + 1] = off[k] + align_up(workspace_bytes(plans[k], TSX_OP_BP, 1));
+        const std::pair<int, bool> key(bp_brick_z_of(plans[k]), bp_parts_of(plans[k]) > 1);
+        if (std::find(keys.begin(), keys.end(), key) == keys.end()) keys.push_back(key);
+    }
+    for (size_t pass = 0; pass < keys.size(); ++pass)
     for (int k = 0; k < n; ++k) {
         const tsx_plan *p = plans[k];
-        if (!vols[k] || !projs[k]) return fail(TSX_ERR_INVALID_ARGUMENT, "data pointers must not be NULL");
-        const int64_t wb = align_up(workspace_bytes(p, TSX_OP_BP, 1));
+        const int64_t wb = off[k + 1] - off[k];
+        const int64_t woff = off[k];
         const int bz = bp_brick_z_of(p);
         const int64_t blocks = (int64_t)((p->nx + BR - 1) / BR) * ((p->ny + BR - 1) / BR) * ((p->nz + bz - 1) / bz);
         ScaleJob job;
@@ -1628,12 +1652,13 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         const int64_t cap = max_blocks(BR * BR);
         if (!(fast && p->bp_separable && bp_chunk_of(p) == p->n_angles) || blocks > cap ||
             tblocks > max_blocks(256)) {
+            if (pass > 0) continue;  // (its own launch, in the first pass)
             int rc = launch_bp(p, vols[k], 0, projs[k], 0, 1, eps[k], stream, wptr ? (char *)wptr + woff : nullptr, wb);
             if (rc) return rc;
-            woff += wb;
             continue;
         }
         const int parts = bp_parts_of(p);
+        if (bz != keys[pass].first || (parts > 1) != keys[pass].second) continue;
         if (m.n > 0 && (bz != BZ || (parts > 1) != PM || (int64_t)m.begin[m.n] + blocks > cap ||
                         std::max<int64_t>(sj_max, tblocks) * (m.n + 1) > max_blocks(256))) {
             int rc = flush();
@@ -1642,7 +1667,6 @@ int multi_bp(const tsx_plan *const *plans, float *const *vols, const float *cons
         BZ = bz;
         PM = parts > 1;
         float *ly = (float *)((char *)wptr + woff);
-        woff += wb;
         const int64_t sino_t = bp_workspace_elems(p);
         job.out = ly;
         job.blocks = (int)tblocks;
