@@ -788,12 +788,13 @@ int bp_chunk_of(const tsx_plan *p) { return std::max(1, std::min(p->bp_chunk, p-
 int bp_brick_z_of(const tsx_plan *p);
 
 // Angle parts of the separable backprojector. A block owns a brick and loops
-// over the angles, so a volume of few bricks (64^3: 32) leaves most CUs idle;
-// its angles are then split into parts of part_len >= 8 consecutive sorted
-// angles, so that bricks x parts reaches about 512 blocks. Single launches run
+// over the angles, so a volume of few bricks (64^3: 32, 128^3: 256) leaves
+// most CUs idle or holds one block per CU; its angles are then split into
+// parts of part_len >= 8 consecutive sorted angles, so that bricks x parts
+// reaches about 1024 blocks (four per CU). Single launches run
 // the parts in separate blocks and add their sums in order (bp_parts_reduce);
 // fused launches run them in turn in one block and add them in the same order,
-// so both give the same sums. Volumes of 256 bricks or more, and plans whose
+// so both give the same sums. Volumes of 512 bricks or more, and plans whose
 // angles take several launches, keep one part. TSX_BP_PARTS=1 keeps one part
 // everywhere.
 int bp_part_len_of(const tsx_plan *p)
@@ -801,8 +802,8 @@ int bp_part_len_of(const tsx_plan *p)
     if (!p->bp_separable || bp_chunk_of(p) < p->n_angles || env_int("TSX_BP_PARTS") == 1) return p->n_angles;
     const int bz = bp_brick_z_of(p);
     const int64_t bricks = (int64_t)((p->nx + BR - 1) / BR) * ((p->ny + BR - 1) / BR) * ((p->nz + bz - 1) / bz);
-    if (bricks >= 256) return p->n_angles;
-    const int64_t len = std::max<int64_t>(8, ((int64_t)p->n_angles * bricks + 511) / 512);
+    if (bricks >= 512) return p->n_angles;
+    const int64_t len = std::max<int64_t>(8, ((int64_t)p->n_angles * bricks + 1023) / 1024);
     return (int)std::min<int64_t>(len, p->n_angles);
 }
 int bp_parts_of(const tsx_plan *p)
