@@ -15,8 +15,8 @@
   instance 0 against the oracle on every 90th angle (forward: those angles of
   its output; backprojection: the fused launch with instance 0's sinogram
   zero except on those angles).
-* the drop-in binding of INTEGRATION.md, run as written (only the library
-  path is filled in), SET and ADD, FP and BP, against the oracle; and its
+* the drop-in ctypes binding given in the repository's integration notes,
+  run as written (only the library path is filled in), SET and ADD, FP and BP, against the oracle; and its
   FDK binding against ts.fdk.
 
 Tolerances: FP and BP relative L2 <= 1e-5 against the FP64 oracle.
