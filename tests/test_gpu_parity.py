@@ -587,8 +587,8 @@ def test_backprojector_angle_parts(monkeypatch):
     A, B = ts.operator(vg1, pg1), ts.operator(vg2, pg2)
     plan = A.projector.plan(0)
     ve = 64 ** 3
-    # the workspace holds the part sums (12 parts of 8 angles)
-    assert plan.workspace_size(_tsx.TSX_OP_BP) >= 12 * ve * 4
+    # the workspace holds the part sums (24 parts of 4 angles)
+    assert plan.workspace_size(_tsx.TSX_OP_BP) >= 24 * ve * 4
     g = torch.Generator(device=DEV)
     g.manual_seed(5)
     ya = torch.rand(A.range_shape, generator=g, device=DEV)

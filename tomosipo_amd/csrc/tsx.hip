@@ -790,7 +790,7 @@ int bp_brick_z_of(const tsx_plan *p);
 // Angle parts of the separable backprojector. A block owns a brick and loops
 // over the angles, so a volume of few bricks (64^3: 32, 128^3: 256) leaves
 // most CUs idle or holds one block per CU; its angles are then split into
-// parts of part_len >= 8 consecutive sorted angles, so that bricks x parts
+// parts of part_len >= 4 consecutive sorted angles, so that bricks x parts
 // reaches about 1024 blocks (four per CU). Single launches run
 // the parts in separate blocks and add their sums in order (bp_parts_reduce);
 // fused launches run them in turn in one block and add them in the same order,
@@ -803,7 +803,7 @@ int bp_part_len_of(const tsx_plan *p)
     const int bz = bp_brick_z_of(p);
     const int64_t bricks = (int64_t)((p->nx + BR - 1) / BR) * ((p->ny + BR - 1) / BR) * ((p->nz + bz - 1) / bz);
     if (bricks >= 512) return p->n_angles;
-    const int64_t len = std::max<int64_t>(8, ((int64_t)p->n_angles * bricks + 1023) / 1024);
+    const int64_t len = std::max<int64_t>(4, ((int64_t)p->n_angles * bricks + 1023) / 1024);
     return (int)std::min<int64_t>(len, p->n_angles);
 }
 int bp_parts_of(const tsx_plan *p)
