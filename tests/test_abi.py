@@ -156,6 +156,27 @@ def test_refined_detector_workspace():
             assert p.workspace_size(_tsx.TSX_OP_BP, batch) == q.workspace_size(_tsx.TSX_OP_BP, batch)
 
 
+def test_backprojector_angle_parts_workspace(monkeypatch):
+    """The separable backprojector of a volume of fewer than 512 bricks
+    (16 x 16 x 32 voxels each) sums its angles in parts of at least 4 angles,
+    about 1024 blocks per launch; its workspace adds the part sums to the
+    transposed projections (rounded up to 256 bytes). 64^3 x 96 angles: 32
+    bricks, 24 parts; 128^3 x 180: 256 bricks, parts of 45 angles, 4 parts;
+    256^3: 2048 bricks, one part. TSX_BP_PARTS=1 keeps one part."""
+    cases = ((64, 96, 24), (128, 180, 4), (256, 40, 1))
+    for n, na, parts in cases:
+        p = _plan(ts.volume(shape=n, size=1), ts.parallel(angles=na, shape=(n, n + n // 2), size=(1, 1.5)))
+        for batch in (1, 2):
+            monkeypatch.setenv("TSX_BP_PARTS", "1")
+            one = p.workspace_size(_tsx.TSX_OP_BP, batch)
+            monkeypatch.delenv("TSX_BP_PARTS")
+            ws = p.workspace_size(_tsx.TSX_OP_BP, batch)
+            if parts == 1:
+                assert ws == one, n
+            else:
+                assert ws == (one + 255) // 256 * 256 + parts * n ** 3 * 4 * batch, (n, batch)
+
+
 def test_refined_forward_bound():
     """The refined forward path keeps ss^2 times the projections in its
     workspace: a 512 x 768 detector with 720 angles (1.1 GB of projections)
